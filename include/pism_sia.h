@@ -240,6 +240,25 @@ int sia_mass_continuity_step(sia_handle *h, const sia_geometry *g, double dt,
 int sia_step(sia_handle *h, const sia_geometry *g, const double *enthalpy,
              int fast, double t_end, double *t_device, double *dt_device);
 
+/* ---- profiling: the reference's Profiling events (Profiling.hh:29-39) --------
+   Event names as in SIAFD.cc:129-146, PISMStressBalance.cc:103-109 and
+   iceModel.cc:732-735. When on, each stage is bracketed by hipEvents on the
+   handle's stream; sia_profiling_read synchronises, returns the accumulated
+   milliseconds and launch counts per event, and clears them. Not for use
+   while a stream is being captured into a graph. */
+enum sia_event {
+  SIA_EV_BED_SMOOTHER = 0,   /* "SIA bed smoother" */
+  SIA_EV_GRADIENT = 1,       /* "SIA gradient" (+ theta, thksmooth) */
+  SIA_EV_FLUX = 2,           /* "SIA flux" */
+  SIA_EV_3D_HOR_VEL = 3,     /* "SIA 3D hor. vel." */
+  SIA_EV_VERT_VEL = 4,       /* "SB vert. vel." */
+  SIA_EV_MASS_TRANSPORT = 5, /* "mass transport" (+ time step, surface and mask) */
+  SIA_EV_COUNT = 6
+};
+int sia_profiling(sia_handle *h, int on);
+int sia_profiling_read(sia_handle *h, double *ms /* [SIA_EV_COUNT] */, int64_t *count /* [SIA_EV_COUNT] */);
+const char *sia_event_name(int event);
+
 /* Human-readable message for the last failing call on this handle. */
 const char *sia_last_error(const sia_handle *h);
 

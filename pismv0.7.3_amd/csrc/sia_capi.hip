@@ -1,0 +1,643 @@
+// sia_capi.hip -- the C ABI of include/pism_sia.h: handle, device buffers,
+// orchestration of SIAFD::update, vertical velocity, time step and mass
+// continuity on one HIP stream.
+#include "../../include/pism_sia.h"
+#include "sia_kernels.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+using namespace pism_sia;
+
+struct sia_handle {
+  sia_grid grid{};
+  std::vector<double> z;
+  sia_params params{};
+  DevParams dp{};
+  hipStream_t stream = nullptr;
+  int device = 0;
+  bool sync_check = true;
+  std::string last_error;
+
+  double *d_z = nullptr;
+  // bed smoother state (BedSmoother members, PISMBedSmoother.hh:79-124)
+  double *topgs = nullptr, *maxtl = nullptr, *C2 = nullptr, *C3 = nullptr, *C4 = nullptr;
+  int64_t bed_counter = -1;  // SIAFD::init sets it to -1 (SIAFD.cc:95-97)
+  bool bed_ready = false;
+  // SIAFD work and output fields
+  double *hx = nullptr, *hy = nullptr, *theta = nullptr, *thks = nullptr;
+  double *flux = nullptr, *Dstag = nullptr;
+  double *I0 = nullptr, *I1 = nullptr, *u = nullptr, *v = nullptr, *w = nullptr;
+  int *ks = nullptr;
+  // scalars / scratch
+  unsigned long long *dmax_bits = nullptr, *cfl_bits = nullptr;
+  int *err = nullptr;
+  double *dt_dev = nullptr, *acc_dev = nullptr, *partials = nullptr, *H_tmp = nullptr;
+  const double *last_ub = nullptr, *last_vb = nullptr, *last_mask = nullptr;
+  bool have_flux = false, have_3d = false;
+  // profiling events (Profiling::begin/end of the reference)
+  struct Rec {
+    int ev;
+    hipEvent_t a, b;
+  };
+  bool prof_on = false;
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> event_pool;
+};
+
+namespace {
+
+int set_err(sia_handle *h, int code, const std::string &msg) {
+  if (h) h->last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(h, call)                                                                        \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess) {                                                                     \
+      return set_err((h), SIA_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
+    }                                                                                           \
+  } while (0)
+
+hipEvent_t take_event(sia_handle *h) {
+  if (!h->event_pool.empty()) {
+    hipEvent_t e = h->event_pool.back();
+    h->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Brackets one profiling event with hipEvents on the handle's stream.
+struct Stage {
+  sia_handle *h;
+  size_t idx;
+  bool on;
+  Stage(sia_handle *hh, int ev) : h(hh), idx(0), on(hh->prof_on) {
+    if (!on) return;
+    sia_handle::Rec r{ev, take_event(h), take_event(h)};
+    if (!r.a || !r.b) {
+      on = false;
+      return;
+    }
+    (void)hipEventRecord(r.a, h->stream);
+    idx = h->recs.size();
+    h->recs.push_back(r);
+  }
+  ~Stage() {
+    if (on) (void)hipEventRecord(h->recs[idx].b, h->stream);
+  }
+};
+
+template <typename T>
+int dalloc(sia_handle *h, T **p, size_t count) {
+  if (*p) return SIA_OK;
+  hipError_t e = hipMalloc((void **)p, count * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return set_err(h, e == hipErrorOutOfMemory ? SIA_ERR_OUT_OF_MEMORY : SIA_ERR_HIP,
+                   std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  return SIA_OK;
+}
+
+void dfree(void *p) {
+  if (p) (void)hipFree(p);
+}
+
+double softness_pb_host(const sia_params &p, double T_pa) {
+  if (T_pa < p.Paterson_Budd_critical_temperature) {
+    return p.Paterson_Budd_A_cold * std::exp(-p.Paterson_Budd_Q_cold / (p.ideal_gas_constant * T_pa));
+  }
+  return p.Paterson_Budd_A_warm * std::exp(-p.Paterson_Budd_Q_warm / (p.ideal_gas_constant * T_pa));
+}
+
+void fill_dev_params(sia_handle *h) {
+  const sia_params &p = h->params;
+  DevParams &d = h->dp;
+  d.Mx = h->grid.Mx;
+  d.My = h->grid.My;
+  d.Mz = h->grid.Mz;
+  d.law = p.sia_flow_law;
+  d.ec = p.enthalpy_converter;
+  d.grad = p.surface_gradient_method;
+  d.is_dry = p.is_dry_simulation;
+  d.include_bmr = p.include_bmr_in_continuity;
+  d.dx = h->grid.dx;
+  d.dy = h->grid.dy;
+  d.Lz = h->z[h->grid.Mz - 1];
+  d.e = p.sia_enhancement_factor;
+  d.n = p.sia_Glen_exponent;
+  d.nm1 = p.sia_Glen_exponent - 1;
+  d.nm1_int = (d.nm1 == 1.0) ? 1 : (d.nm1 == 2.0) ? 2 : (d.nm1 == 3.0) ? 3 : -1;
+  d.gs = p.ice_grain_size;
+  d.p_air = p.surface_pressure;
+  d.rho_i = p.ice_density;
+  d.g = p.standard_gravity;
+  d.beta = p.beta_CC;
+  d.ci = p.ice_specific_heat_capacity;
+  d.T_melt = p.water_melting_point_temperature;
+  d.T0 = p.enthalpy_converter_reference_temperature;
+  d.L = p.water_latent_heat_fusion;
+  d.cw = p.water_specific_heat_capacity;
+  d.A_cold = p.Paterson_Budd_A_cold;
+  d.A_warm = p.Paterson_Budd_A_warm;
+  d.Q_cold = p.Paterson_Budd_Q_cold;
+  d.Q_warm = p.Paterson_Budd_Q_warm;
+  d.R = p.ideal_gas_constant;
+  d.T_crit = p.Paterson_Budd_critical_temperature;
+  {
+    const double beta_CC_grad = p.beta_CC * p.ice_density * p.standard_gravity;  // flowlaws.cc:71
+    d.pb_coeff = beta_CC_grad / (p.ice_density * p.standard_gravity);            // flowlaws.cc:242
+  }
+  d.A_T0 = softness_pb_host(p, p.water_melting_point_temperature);  // flowlaws.cc:219
+  d.wf_coeff = p.gpbld_water_frac_coeff;
+  d.wf_limit = p.gpbld_water_frac_observed_limit;
+  d.A_glen = p.ice_softness;
+  d.hooke_A = p.Hooke_A;
+  d.hooke_Q = p.Hooke_Q;
+  d.hooke_C = p.Hooke_C;
+  d.hooke_k = p.Hooke_k;
+  d.hooke_Tr = p.Hooke_Tr;
+  d.sec_to_ka = 1.0 / (1000.0 * p.seconds_per_year);
+  d.alpha_float = 1 - p.ice_density / p.sea_water_density;
+  d.sea_level = p.sea_level;
+  d.icefree_thk = p.mask_icefree_thickness_standard;
+  d.floating_thk = p.mask_is_floating_thickness_standard;
+  d.max_dt = p.maximum_time_step;
+  d.adapt_ratio = p.adaptive_timestepping_ratio;
+  d.Nx = -1;
+  d.Ny = -1;
+  d.use_age = 0;
+}
+
+// FlowLawUsesGrainSize (flowlaws.cc:33-42): only the Goldsby-Kohlstedt law
+// changes with grain size among the laws of flowlaw_factory.cc:77-83.
+bool law_uses_grain_size(int law) { return law == SIA_FLOW_GK; }
+
+int check_flags(sia_handle *h) {
+  int flags = 0;
+  HIP_TRY(h, hipMemcpyAsync(&flags, h->err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  if (flags & ERR_NEG_THK) {
+    return set_err(h, SIA_ERR_NEGATIVE_THICKNESS, "negative ice thickness (BedSmoother::get_smoothed_thk)");
+  }
+  if (flags & ERR_THK_LZ) {
+    return set_err(h, SIA_ERR_THICKNESS_ABOVE_LZ, "ice thickness is above the top of the computational grid");
+  }
+  if (flags & ERR_OMEGA) {
+    return set_err(h, SIA_ERR_OMEGA_NEGATIVE, "omega is negative in BedSmoother::get_theta()");
+  }
+  return SIA_OK;
+}
+
+// BedSmoother::preprocess_bed (PISMBedSmoother.cc:96-149)
+int preprocess_bed(sia_handle *h, const double *bed) {
+  const size_t n2 = (size_t)h->grid.Mx * h->grid.My;
+  const sia_params &p = h->params;
+  if (p.bed_smoother_range <= 0.0) {
+    HIP_TRY(h, hipMemcpyAsync(h->topgs, bed, n2 * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    // fill kernels, not memset nodes, so that a captured step replays correctly
+    launch_fill(h->maxtl, (long)n2, 0.0, h->stream);
+    launch_fill(h->C2, (long)n2, 0.0, h->stream);
+    launch_fill(h->C3, (long)n2, 0.0, h->stream);
+    launch_fill(h->C4, (long)n2, 0.0, h->stream);
+    HIP_TRY(h, hipGetLastError());
+    h->dp.Nx = -1;
+    h->dp.Ny = -1;
+    return SIA_OK;
+  }
+  int Nx = (int)std::ceil(p.bed_smoother_range / h->grid.dx);
+  int Ny = (int)std::ceil(p.bed_smoother_range / h->grid.dy);
+  if (Nx < 1) Nx = 1;
+  if (Ny < 1) Ny = 1;
+  if (Nx >= h->grid.Mx || Ny >= h->grid.My) {
+    return set_err(h, SIA_ERR_CONFIG,
+                   "input Nx, Ny in bed smoother is too large because domain of smoothing exceeds IceGrid domain");
+  }
+  h->dp.Nx = Nx;
+  h->dp.Ny = Ny;
+  const double n = p.sia_Glen_exponent, k = (n + 2) / n, s2 = k * (2 * n + 2) / (2 * n),
+               s3 = s2 * (3 * n + 2) / (3 * n), s4 = s3 * (4 * n + 2) / (4 * n);
+  launch_bed_smoother(h->dp, s2, s3, s4, bed, h->topgs, h->maxtl, h->C2, h->C3, h->C4, h->stream);
+  HIP_TRY(h, hipGetLastError());
+  return SIA_OK;
+}
+
+int update_impl(sia_handle *h, const sia_inputs *in, int fast, bool allow_sync) {
+  if (!h || !in) return set_err(h, SIA_ERR_ARGUMENT, "NULL argument");
+  if (!in->land_ice_thickness || !in->bedrock_altitude || !in->surface_altitude || !in->mask || !in->enthalpy) {
+    return set_err(h, SIA_ERR_ARGUMENT, "sia_update: a required input field is NULL");
+  }
+  const sia_params &p = h->params;
+  if (p.compute_grain_size_using_age && !p.do_age) {
+    return set_err(h, SIA_ERR_CONFIG,
+                   "SIAFD::compute_diffusive_flux(): do_age not set but age is needed for grain-size-based flow law");
+  }
+  const bool use_age = law_uses_grain_size(p.sia_flow_law) && p.compute_grain_size_using_age && p.do_age;
+  if (use_age && !in->age) return set_err(h, SIA_ERR_ARGUMENT, "sia_update: age field is required");
+  h->dp.use_age = use_age ? 1 : 0;
+  const bool full = (fast == 0);
+  const size_t n2 = (size_t)h->grid.Mx * h->grid.My, n3 = n2 * (size_t)h->grid.Mz;
+  int rc;
+  if (full) {
+    if ((rc = dalloc(h, &h->I0, n3)) || (rc = dalloc(h, &h->I1, n3)) || (rc = dalloc(h, &h->u, n3)) ||
+        (rc = dalloc(h, &h->v, n3)) || (rc = dalloc(h, &h->w, n3)) || (rc = dalloc(h, &h->ks, 2 * n2))) {
+      return rc;
+    }
+  }
+  hipStream_t st = h->stream;
+  h->have_flux = false;
+  launch_reset(h->dmax_bits, h->err, h->cfl_bits, h->dp.max_dt, st);
+  // SIAFD.cc:128-133: re-smooth the bed when its state counter increased
+  if (in->bed_state_counter > h->bed_counter || !h->bed_ready) {
+    Stage stage(h, SIA_EV_BED_SMOOTHER);
+    if ((rc = preprocess_bed(h, in->bedrock_altitude))) return rc;
+    h->bed_counter = in->bed_state_counter;
+    h->bed_ready = true;
+  }
+  const DevParams &P = h->dp;
+  {
+    Stage stage(h, SIA_EV_GRADIENT);
+    launch_gradient(P, in->land_ice_thickness, in->bedrock_altitude, in->surface_altitude, in->mask, h->hx,
+                    h->hy, st);
+    launch_theta_thk(P, in->land_ice_thickness, in->bedrock_altitude, in->surface_altitude, in->mask,
+                     h->topgs, h->maxtl, h->C2, h->C3, h->C4, h->theta, h->thks, h->err, st);
+  }
+  ColumnArgs A{};
+  A.z = h->d_z;
+  A.E = in->enthalpy;
+  A.age = use_age ? in->age : nullptr;
+  A.thks = h->thks;
+  A.theta = h->theta;
+  A.hx = h->hx;
+  A.hy = h->hy;
+  A.flux = h->flux;
+  A.Dstag = h->Dstag;
+  A.I0 = h->I0;
+  A.I1 = h->I1;
+  A.ks = h->ks;
+  A.dmax_bits = h->dmax_bits;
+  A.err = h->err;
+  {
+    Stage stage(h, SIA_EV_FLUX);
+    launch_column(P, A, full, st);
+  }
+  if (full) {
+    // "SIA 3D hor. vel." and "SB vert. vel." run as one fused kernel
+    Stage stage(h, SIA_EV_3D_HOR_VEL);
+    VelArgs V{};
+    V.z = h->d_z;
+    V.mask = in->mask;
+    V.I0 = h->I0;
+    V.I1 = h->I1;
+    V.ks = h->ks;
+    V.hx = h->hx;
+    V.hy = h->hy;
+    V.ub = in->vel_input_u;
+    V.vb = in->vel_input_v;
+    V.bmr = in->basal_melt_rate;
+    V.u = h->u;
+    V.v = h->v;
+    V.w = h->w;
+    launch_velocity_uvw(P, V, st);
+  }
+  if (in->vel_input_u && in->vel_input_v) {
+    launch_cfl2d(P, in->mask, in->vel_input_u, in->vel_input_v, h->cfl_bits, st);
+  }
+  HIP_TRY(h, hipGetLastError());
+  h->last_ub = in->vel_input_u;
+  h->last_vb = in->vel_input_v;
+  h->last_mask = in->mask;
+  h->have_flux = true;
+  h->have_3d = full;
+  if (allow_sync && h->sync_check) return check_flags(h);
+  return SIA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void sia_params_default(sia_params *p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  const double secpera = 3.15569259747e7;  // UDUNITS "year"
+  p->sia_flow_law = SIA_FLOW_GPBLD;                 // cdl:1094
+  p->surface_gradient_method = SIA_GRADIENT_HASELOFF;  // cdl:1272
+  p->enthalpy_converter = SIA_EC_DEFAULT;
+  p->compute_grain_size_using_age = 0;               // cdl:309
+  p->do_age = 0;                                     // cdl:34
+  p->is_dry_simulation = 0;                          // cdl:136
+  p->include_bmr_in_continuity = 1;                  // cdl:148
+  p->sia_enhancement_factor = 1.0;                   // cdl:281
+  p->sia_Glen_exponent = 3.0;                        // cdl:1009
+  p->bed_smoother_range = 5.0e3;                     // cdl:560
+  p->ice_grain_size = 1.0;                           // cdl:304
+  p->ice_softness = 3.1689e-24;                      // cdl:1020
+  p->Paterson_Budd_A_cold = 3.61e-13;                // cdl:1060
+  p->Paterson_Budd_A_warm = 1.73e3;                  // cdl:1065
+  p->Paterson_Budd_Q_cold = 6.0e4;                   // cdl:1070
+  p->Paterson_Budd_Q_warm = 13.9e4;                  // cdl:1075
+  p->Paterson_Budd_critical_temperature = 263.15;    // cdl:1080
+  p->gpbld_water_frac_coeff = 181.25;                // cdl:1115
+  p->gpbld_water_frac_observed_limit = 0.01;         // cdl:1120
+  p->Hooke_A = 4.42165e-9;                           // cdl:1025
+  p->Hooke_Q = 7.88e4;                               // cdl:1030
+  p->Hooke_C = 0.16612;                              // cdl:1035
+  p->Hooke_k = 1.17;                                 // cdl:1040
+  p->Hooke_Tr = 273.39;                              // cdl:1045
+  p->ideal_gas_constant = 8.31441;                   // cdl:1193
+  p->beta_CC = 7.9e-8;                               // cdl:963
+  p->ice_specific_heat_capacity = 2009.0;            // cdl:1003
+  p->water_specific_heat_capacity = 4170.0;          // cdl:988
+  p->water_latent_heat_fusion = 3.34e5;              // cdl:983
+  p->surface_pressure = 0.0;                         // cdl:968
+  p->ice_density = 910.0;                            // cdl:993
+  p->standard_gravity = 9.81;                        // cdl:1188
+  p->water_melting_point_temperature = 273.15;       // cdl:973
+  p->enthalpy_converter_reference_temperature = 223.15;  // cdl:978
+  p->sea_water_density = 1028.0;                     // cdl:1140
+  p->sea_level = 0.0;
+  p->mask_icefree_thickness_standard = 0.01;         // cdl:1225
+  p->mask_is_floating_thickness_standard = 1.0;      // cdl:1230
+  p->adaptive_timestepping_ratio = 0.12;             // cdl:330
+  p->maximum_time_step = 60.0 * secpera;             // cdl:341
+  p->seconds_per_year = secpera;
+}
+
+int sia_create(const sia_grid *grid, const sia_params *params, sia_handle **out) {
+  if (!grid || !params || !out || !grid->z) return SIA_ERR_ARGUMENT;
+  *out = nullptr;
+  if (grid->Mx < 3 || grid->My < 3 || grid->Mz < 2 || !(grid->dx > 0) || !(grid->dy > 0)) return SIA_ERR_ARGUMENT;
+  if (params->sia_flow_law < 0 || params->sia_flow_law > SIA_FLOW_GK) return SIA_ERR_CONFIG;
+  if (params->surface_gradient_method < 0 || params->surface_gradient_method > SIA_GRADIENT_ETA) return SIA_ERR_CONFIG;
+  if (params->enthalpy_converter < 0 || params->enthalpy_converter > SIA_EC_KIRCHHOFF) return SIA_ERR_CONFIG;
+  for (int k = 1; k < grid->Mz; k++) {
+    if (!(grid->z[k] > grid->z[k - 1])) return SIA_ERR_ARGUMENT;
+  }
+  sia_handle *h = new (std::nothrow) sia_handle();
+  if (!h) return SIA_ERR_OUT_OF_MEMORY;
+  h->grid = *grid;
+  h->z.assign(grid->z, grid->z + grid->Mz);
+  h->grid.z = h->z.data();
+  h->params = *params;
+  if (hipGetDevice(&h->device) != hipSuccess) {
+    delete h;
+    return SIA_ERR_HIP;
+  }
+  fill_dev_params(h);
+  const size_t n2 = (size_t)grid->Mx * grid->My;
+  int rc;
+  if ((rc = dalloc(h, &h->d_z, (size_t)grid->Mz)) || (rc = dalloc(h, &h->topgs, n2)) ||
+      (rc = dalloc(h, &h->maxtl, n2)) || (rc = dalloc(h, &h->C2, n2)) || (rc = dalloc(h, &h->C3, n2)) ||
+      (rc = dalloc(h, &h->C4, n2)) || (rc = dalloc(h, &h->hx, 2 * n2)) || (rc = dalloc(h, &h->hy, 2 * n2)) ||
+      (rc = dalloc(h, &h->theta, n2)) || (rc = dalloc(h, &h->thks, n2)) || (rc = dalloc(h, &h->flux, 2 * n2)) ||
+      (rc = dalloc(h, &h->Dstag, 2 * n2)) || (rc = dalloc(h, &h->dmax_bits, 1)) ||
+      (rc = dalloc(h, &h->cfl_bits, 1)) || (rc = dalloc(h, &h->err, 1)) || (rc = dalloc(h, &h->dt_dev, 1)) ||
+      (rc = dalloc(h, &h->acc_dev, 8)) || (rc = dalloc(h, &h->H_tmp, n2)) ||
+      (rc = dalloc(h, &h->partials, (size_t)mass_partial_blocks(h->dp) * 8))) {
+    sia_destroy(h);
+    return rc;
+  }
+  if (hipMemcpy(h->d_z, h->z.data(), sizeof(double) * grid->Mz, hipMemcpyHostToDevice) != hipSuccess) {
+    sia_destroy(h);
+    return SIA_ERR_HIP;
+  }
+  *out = h;
+  return SIA_OK;
+}
+
+int sia_destroy(sia_handle *h) {
+  if (!h) return SIA_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  else (void)hipDeviceSynchronize();
+  void *bufs[] = {h->d_z, h->topgs, h->maxtl, h->C2, h->C3, h->C4, h->hx, h->hy, h->theta, h->thks,
+                  h->flux, h->Dstag, h->I0, h->I1, h->u, h->v, h->w, h->ks, h->dmax_bits, h->cfl_bits, h->err,
+                  h->dt_dev, h->acc_dev, h->partials, h->H_tmp};
+  for (void *b : bufs) dfree(b);
+  for (auto &r : h->recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);
+  delete h;
+  return SIA_OK;
+}
+
+int sia_set_stream(sia_handle *h, void *hip_stream) {
+  if (!h) return SIA_ERR_ARGUMENT;
+  h->stream = (hipStream_t)hip_stream;
+  return SIA_OK;
+}
+
+int sia_set_sync_check(sia_handle *h, int check) {
+  if (!h) return SIA_ERR_ARGUMENT;
+  h->sync_check = check != 0;
+  return SIA_OK;
+}
+
+int sia_check_errors(sia_handle *h) {
+  if (!h) return SIA_ERR_ARGUMENT;
+  return check_flags(h);
+}
+
+int sia_update(sia_handle *h, const sia_inputs *in, int fast) { return update_impl(h, in, fast, true); }
+
+int sia_diffusive_flux(sia_handle *h, const double **flux) {
+  if (!h || !flux) return SIA_ERR_ARGUMENT;
+  if (!h->have_flux) return set_err(h, SIA_ERR_STATE, "no update has run");
+  *flux = h->flux;
+  return SIA_OK;
+}
+
+int sia_max_diffusivity(sia_handle *h, double *D_max) {
+  if (!h || !D_max) return SIA_ERR_ARGUMENT;
+  if (!h->have_flux) return set_err(h, SIA_ERR_STATE, "no update has run");
+  unsigned long long bits = 0;
+  HIP_TRY(h, hipMemcpyAsync(&bits, h->dmax_bits, sizeof(bits), hipMemcpyDeviceToHost, h->stream));
+  int rc = check_flags(h);  // synchronises
+  double d;
+  std::memcpy(&d, &bits, sizeof(d));
+  *D_max = d;
+  return rc;
+}
+
+int sia_max_diffusivity_device(sia_handle *h, const double **D_max) {
+  if (!h || !D_max) return SIA_ERR_ARGUMENT;
+  *D_max = (const double *)h->dmax_bits;
+  return SIA_OK;
+}
+
+static int get3d(sia_handle *h, const double **out, double *buf) {
+  if (!h || !out) return SIA_ERR_ARGUMENT;
+  if (!h->have_3d) return set_err(h, SIA_ERR_STATE, "3-D velocities need a full (fast = 0) update");
+  *out = buf;
+  return SIA_OK;
+}
+
+int sia_velocity_u(sia_handle *h, const double **u) { return get3d(h, u, h ? h->u : nullptr); }
+int sia_velocity_v(sia_handle *h, const double **v) { return get3d(h, v, h ? h->v : nullptr); }
+int sia_velocity_w(sia_handle *h, const double **w) { return get3d(h, w, h ? h->w : nullptr); }
+
+int sia_surface_gradient(sia_handle *h, const double **h_x, const double **h_y) {
+  if (!h || !h_x || !h_y) return SIA_ERR_ARGUMENT;
+  if (!h->have_flux) return set_err(h, SIA_ERR_STATE, "no update has run");
+  *h_x = h->hx;
+  *h_y = h->hy;
+  return SIA_OK;
+}
+
+int sia_diffusivity_staggered(sia_handle *h, const double **D) {
+  if (!h || !D) return SIA_ERR_ARGUMENT;
+  if (!h->have_flux) return set_err(h, SIA_ERR_STATE, "no update has run");
+  *D = h->Dstag;
+  return SIA_OK;
+}
+
+int sia_theta_thksmooth(sia_handle *h, const double **theta, const double **thksmooth) {
+  if (!h || !theta || !thksmooth) return SIA_ERR_ARGUMENT;
+  if (!h->have_flux) return set_err(h, SIA_ERR_STATE, "no update has run");
+  *theta = h->theta;
+  *thksmooth = h->thks;
+  return SIA_OK;
+}
+
+int sia_bed_smoother_fields(sia_handle *h, const double **topgsmooth, const double **maxtl, const double **C2,
+                            const double **C3, const double **C4, int32_t *Nx, int32_t *Ny) {
+  if (!h) return SIA_ERR_ARGUMENT;
+  if (!h->bed_ready) return set_err(h, SIA_ERR_STATE, "the bed has not been preprocessed yet");
+  if (topgsmooth) *topgsmooth = h->topgs;
+  if (maxtl) *maxtl = h->maxtl;
+  if (C2) *C2 = h->C2;
+  if (C3) *C3 = h->C3;
+  if (C4) *C4 = h->C4;
+  if (Nx) *Nx = h->dp.Nx;
+  if (Ny) *Ny = h->dp.Ny;
+  return SIA_OK;
+}
+
+int sia_max_timestep(sia_handle *h, double time_to_end, double *dt) {
+  if (!h || !dt) return SIA_ERR_ARGUMENT;
+  if (!h->have_flux) return set_err(h, SIA_ERR_STATE, "no update has run");
+  launch_dt(h->dp, h->dmax_bits, h->cfl_bits, 0.0, nullptr, h->dt_dev, h->stream);
+  HIP_TRY(h, hipGetLastError());
+  double d = 0.0;
+  HIP_TRY(h, hipMemcpyAsync(&d, h->dt_dev, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  if (time_to_end > 0.0 && time_to_end < d) d = time_to_end;
+  *dt = d;
+  return SIA_OK;
+}
+
+static int mass_step_impl(sia_handle *h, const sia_geometry *g, const double *dt_dev, sia_mass_accounting *acc) {
+  if (!h || !g || !g->land_ice_thickness || !g->surface_altitude || !g->mask || !g->bedrock_altitude) {
+    return set_err(h, SIA_ERR_ARGUMENT, "sia_mass_continuity_step: NULL geometry field");
+  }
+  if (!h->have_flux) return set_err(h, SIA_ERR_STATE, "no update has run");
+  MassArgs M{};
+  M.dt = dt_dev;
+  M.H = g->land_ice_thickness;
+  M.flux = h->flux;
+  M.mask = g->mask;
+  M.ub = h->last_ub;
+  M.vb = h->last_vb;
+  M.smb = g->climatic_mass_balance;
+  M.bmr = g->basal_melt_rate;
+  M.H_new = h->H_tmp;
+  M.partials = h->partials;
+  {
+    Stage stage(h, SIA_EV_MASS_TRANSPORT);
+    launch_mass_continuity(h->dp, M, h->acc_dev, h->stream);
+    launch_mask_surface(h->dp, h->H_tmp, g->bedrock_altitude, g->land_ice_thickness, g->mask,
+                        g->surface_altitude, h->err, h->stream);
+  }
+  HIP_TRY(h, hipGetLastError());
+  if (acc) {
+    double a[8];
+    HIP_TRY(h, hipMemcpyAsync(a, h->acc_dev, sizeof(a), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(h, hipStreamSynchronize(h->stream));
+    acc->grounded_basal_ice_flux = a[0];
+    acc->nonneg_rule_flux = a[1];
+    acc->sub_shelf_ice_flux = a[2];
+    acc->surface_ice_flux = a[3];
+    acc->sum_divQ_SIA = a[4];
+    acc->sum_divQ_SSA = a[5];
+    acc->Href_to_H_flux = a[6];
+    acc->H_to_Href_flux = a[7];
+  }
+  return SIA_OK;
+}
+
+int sia_mass_continuity_step(sia_handle *h, const sia_geometry *g, double dt, sia_mass_accounting *acc) {
+  if (!h) return SIA_ERR_ARGUMENT;
+  if (!(dt >= 0.0)) return set_err(h, SIA_ERR_ARGUMENT, "dt must be non-negative");
+  HIP_TRY(h, hipMemcpyAsync(h->dt_dev, &dt, sizeof(double), hipMemcpyHostToDevice, h->stream));
+  int rc = mass_step_impl(h, g, h->dt_dev, acc);
+  if (rc) return rc;
+  // the host dt must stay alive until the copy ran
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  if (h->sync_check) return check_flags(h);
+  return SIA_OK;
+}
+
+int sia_step(sia_handle *h, const sia_geometry *g, const double *enthalpy, int fast, double t_end,
+             double *t_device, double *dt_device) {
+  if (!h || !g || !enthalpy) return set_err(h, SIA_ERR_ARGUMENT, "sia_step: NULL argument");
+  sia_inputs in{};
+  in.land_ice_thickness = g->land_ice_thickness;
+  in.bedrock_altitude = g->bedrock_altitude;
+  in.surface_altitude = g->surface_altitude;
+  in.mask = g->mask;
+  in.enthalpy = enthalpy;
+  in.basal_melt_rate = g->basal_melt_rate;
+  in.bed_state_counter = h->bed_ready ? h->bed_counter : 0;
+  int rc = update_impl(h, &in, fast, false);
+  if (rc) return rc;
+  double *dt_out = dt_device ? dt_device : h->dt_dev;
+  launch_dt(h->dp, h->dmax_bits, h->cfl_bits, t_end, t_device, dt_out, h->stream);
+  HIP_TRY(h, hipGetLastError());
+  return mass_step_impl(h, g, dt_out, nullptr);
+}
+
+int sia_profiling(sia_handle *h, int on) {
+  if (!h) return SIA_ERR_ARGUMENT;
+  h->prof_on = on != 0;
+  return SIA_OK;
+}
+
+int sia_profiling_read(sia_handle *h, double *ms, int64_t *count) {
+  if (!h || !ms || !count) return SIA_ERR_ARGUMENT;
+  for (int e = 0; e < SIA_EV_COUNT; e++) {
+    ms[e] = 0.0;
+    count[e] = 0;
+  }
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  for (auto &r : h->recs) {
+    float t = 0.0f;
+    HIP_TRY(h, hipEventElapsedTime(&t, r.a, r.b));
+    ms[r.ev] += (double)t;
+    count[r.ev] += 1;
+    h->event_pool.push_back(r.a);
+    h->event_pool.push_back(r.b);
+  }
+  h->recs.clear();
+  return SIA_OK;
+}
+
+const char *sia_event_name(int event) {
+  static const char *names[SIA_EV_COUNT] = {"SIA bed smoother", "SIA gradient", "SIA flux",
+                                            "SIA 3D hor. vel.", "SB vert. vel.", "mass transport"};
+  return (event >= 0 && event < SIA_EV_COUNT) ? names[event] : "unknown";
+}
+
+const char *sia_last_error(const sia_handle *h) { return h ? h->last_error.c_str() : "NULL handle"; }
+
+}  // extern "C"

@@ -1,0 +1,220 @@
+// sia_device.hpp -- device-side physics of the SIA hot path for gfx950.
+//
+// Flow laws and enthalpy converters of PISM v0.7.3 written for the GPU. The
+// host precomputes every constant into DevParams (passed by value as a kernel
+// argument, so it lives in scalar registers/SGPR-loaded constant memory).
+// The order of floating-point operations follows the reference expressions so
+// that results stay within a few ulp of PISM's; FMA contraction is disabled
+// for the whole library (-ffp-contract=off) and fused multiply-adds appear
+// only where written explicitly.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pism_sia {
+
+enum Law { LAW_ISOGLEN = 0, LAW_PB = 1, LAW_GPBLD = 2, LAW_HOOKE = 3, LAW_ARR = 4, LAW_ARRWARM = 5, LAW_GK = 6 };
+enum EcKind { EC_DEFAULT = 0, EC_COLD = 1, EC_KIRCHHOFF = 2 };
+enum ErrBits { ERR_NEG_THK = 1, ERR_THK_LZ = 2, ERR_OMEGA = 4 };
+
+struct DevParams {
+  int32_t Mx, My, Mz;
+  int32_t law, ec, use_age;
+  int32_t nm1_int;          // n-1 when it is 1, 2 or 3 (integer power), else -1
+  int32_t Nx, Ny;           // bed smoother half-widths, -1 = inactive
+  int32_t grad;             // surface_gradient_method
+  int32_t is_dry, include_bmr;
+  double dx, dy, Lz;
+  // SIA
+  double e, n, nm1, gs;
+  // enthalpy converter (enthalpyConverter.cc:56-68)
+  double p_air, rho_i, g, beta, ci, T_melt, T0, L, cw;
+  // Paterson-Budd (flowlaws.cc:60-86)
+  double A_cold, A_warm, Q_cold, Q_warm, R, T_crit;
+  double pb_coeff;          // m_beta_CC_grad / (m_rho * m_standard_gravity)
+  double A_T0;              // softness_parameter_paterson_budd(water_melting_point_temperature)
+  double wf_coeff, wf_limit;
+  double A_glen;            // ice_softness
+  double hooke_A, hooke_Q, hooke_C, hooke_k, hooke_Tr;
+  double sec_to_ka;
+  // geometry / time stepping
+  double alpha_float, sea_level, icefree_thk, floating_thk, max_dt, adapt_ratio;
+};
+
+__device__ __forceinline__ int wrapi(int i, int M) {
+  return i < 0 ? i + M : (i >= M ? i - M : i);
+}
+
+// IceModelVec2Int::as_int (IceModelVec_inline.hh:94-100)
+__device__ __forceinline__ int as_int(double a) { return (int)floor(a + 0.5); }
+
+// mask predicates (Mask.hh:28-65)
+__device__ __forceinline__ bool m_ocean(int M) { return M >= 3; }
+__device__ __forceinline__ bool m_grounded(int M) { return !m_ocean(M); }
+__device__ __forceinline__ bool m_icy(int M) { return M == 2 || M == 3; }
+__device__ __forceinline__ bool m_grounded_ice(int M) { return m_icy(M) && m_grounded(M); }
+__device__ __forceinline__ bool m_floating_ice(int M) { return m_icy(M) && m_ocean(M); }
+__device__ __forceinline__ bool m_ice_free(int M) { return !m_icy(M); }
+__device__ __forceinline__ bool m_ice_free_ocean(int M) { return m_ocean(M) && m_ice_free(M); }
+__device__ __forceinline__ bool m_ice_free_land(int M) { return m_grounded(M) && m_ice_free(M); }
+
+// ---------------------------------------------------------------- EC
+// EnthalpyConverter::pressure (enthalpyConverter.cc:92-98)
+__device__ __forceinline__ double ec_pressure(const DevParams &P, double depth) {
+  return depth > 0.0 ? P.p_air + P.rho_i * P.g * depth : P.p_air;
+}
+// melting_temperature_impl (enthalpyConverter.cc:128-130; cold: 347-349)
+__device__ __forceinline__ double ec_melting_temperature(const DevParams &P, double p) {
+  return P.ec == EC_COLD ? P.T_melt : P.T_melt - P.beta * p;
+}
+// L_impl (enthalpyConverter.cc:115-118; Kirchhoff: 433-435)
+__device__ __forceinline__ double ec_L(const DevParams &P, double T_pm) {
+  return P.ec == EC_KIRCHHOFF ? P.L + (P.cw - P.ci) * (T_pm - 273.15) : P.L;
+}
+// enthalpy_cts_impl (enthalpyConverter.cc:141-143)
+__device__ __forceinline__ double ec_enthalpy_cts(const DevParams &P, double p) {
+  return P.ci * (ec_melting_temperature(P, p) - P.T0);
+}
+// temperature_impl (enthalpyConverter.cc:175-189; cold: 355-357)
+__device__ __forceinline__ double ec_temperature(const DevParams &P, double E, double p) {
+  if (P.ec == EC_COLD || E < ec_enthalpy_cts(P, p)) {
+    return (E / P.ci) + P.T0;
+  }
+  return ec_melting_temperature(P, p);
+}
+// enthalpyConverter.cc:197-199
+__device__ __forceinline__ double ec_pressure_adjusted_temperature(const DevParams &P, double E, double p) {
+  return ec_temperature(P, E, p) - ec_melting_temperature(P, p) + P.T_melt;
+}
+// water_fraction_impl (enthalpyConverter.cc:219-234; cold: 342-345)
+__device__ __forceinline__ double ec_water_fraction(const DevParams &P, double E, double p) {
+  if (P.ec == EC_COLD) return 0.0;
+  const double E_s = ec_enthalpy_cts(P, p);
+  if (E <= E_s) return 0.0;
+  return (E - E_s) / ec_L(P, ec_melting_temperature(P, p));
+}
+
+// ---------------------------------------------------------------- flow laws
+// FlowLaw::softness_parameter_paterson_budd (flowlaws.cc:90-95)
+__device__ __forceinline__ double softness_pb(const DevParams &P, double T_pa) {
+  const bool cold = T_pa < P.T_crit;
+  const double A = cold ? P.A_cold : P.A_warm;
+  const double Q = cold ? P.Q_cold : P.Q_warm;
+  return A * exp(-Q / (P.R * T_pa));
+}
+
+__device__ __forceinline__ double stress_pow(const DevParams &P, double s) {
+  switch (P.nm1_int) {
+  case 1: return s;
+  case 2: return s * s;               // == correctly rounded pow(s, 2)
+  case 3: return s * s * s;
+  default: return pow(s, P.nm1);
+  }
+}
+
+// GoldsbyKohlstedt::flow_from_temp (flowlaws.cc:274-312 constants, 366-402)
+__device__ inline double gk_flow_from_temp(const DevParams &P, double stress, double temp,
+                                           double pressure, double gs) {
+  const double V_act_vol = -13.e-6;
+  const double disl_crit_temp = 258.0, disl_A_cold = 4.0e-19, disl_A_warm = 6.0e4,
+               disl_n = 4.0, disl_Q_cold = 60.e3, disl_Q_warm = 180.e3;
+  const double gbs_crit_temp = 255.0, gbs_A_cold = 6.1811e-14, gbs_A_warm = 4.7547e15,
+               gbs_n = 1.8, gbs_Q_cold = 49.e3, gbs_Q_warm = 192.e3, p_grain_sz_exp = 1.4;
+  const double basal_A = 2.1896e-7, basal_n = 2.4, basal_Q = 60.e3;
+  const double diff_crit_temp = 258.0, diff_V_m = 1.97e-5, diff_D_0v = 9.10e-4,
+               diff_Q_v = 59.4e3, diff_D_0b = 5.8e-4, diff_Q_b = 49.e3, diff_delta = 9.04e-10;
+  if (fabs(stress) < 1e-10) return 0;
+  const double T = temp + P.pb_coeff * pressure;
+  const double pV = pressure * V_act_vol;
+  const double RT = P.R * T;
+  const double diff_D_v = diff_D_0v * exp(-diff_Q_v / RT);
+  double diff_D_b = diff_D_0b * exp(-diff_Q_b / RT);
+  if (T > diff_crit_temp) diff_D_b *= 1000;
+  const double eps_diff = 14 * diff_V_m * (diff_D_v + M_PI * diff_delta * diff_D_b / gs) / (RT * (gs * gs));
+  double eps_disl;
+  if (T > disl_crit_temp) {
+    eps_disl = disl_A_warm * pow(stress, disl_n - 1) * exp(-(disl_Q_warm + pV) / RT);
+  } else {
+    eps_disl = disl_A_cold * pow(stress, disl_n - 1) * exp(-(disl_Q_cold + pV) / RT);
+  }
+  const double eps_basal = basal_A * pow(stress, basal_n - 1) * exp(-(basal_Q + pV) / RT);
+  double eps_gbs;
+  if (T > gbs_crit_temp) {
+    eps_gbs = gbs_A_warm * (pow(stress, gbs_n - 1) / pow(gs, p_grain_sz_exp)) * exp(-(gbs_Q_warm + pV) / RT);
+  } else {
+    eps_gbs = gbs_A_cold * (pow(stress, gbs_n - 1) / pow(gs, p_grain_sz_exp)) * exp(-(gbs_Q_cold + pV) / RT);
+  }
+  return eps_diff + eps_disl + (eps_basal * eps_gbs) / (eps_basal + eps_gbs);
+}
+
+// FlowLaw::flow and the overrides selected by sia_flow_law
+// (flowlaws.cc:98-101, 209-244, 267-270, 314-318; flowlaws.hh:227-229, 296-326)
+template <int LAW>
+__device__ __forceinline__ double flow(const DevParams &P, double stress, double E, double p, double gs) {
+  if (LAW == LAW_ISOGLEN) {
+    return P.A_glen * stress_pow(P, stress);
+  } else if (LAW == LAW_GPBLD) {
+    const double E_s = ec_enthalpy_cts(P, p);
+    double A;
+    if (E < E_s) {
+      A = softness_pb(P, ec_pressure_adjusted_temperature(P, E, p));
+    } else {
+      double omega = ec_water_fraction(P, E, p);
+      if (P.wf_limit < omega) omega = P.wf_limit;
+      A = P.A_T0 * (1.0 + P.wf_coeff * omega);
+    }
+    return A * stress_pow(P, stress);
+  } else if (LAW == LAW_PB) {
+    const double T_pa = ec_temperature(P, E, p) + P.pb_coeff * p;
+    return softness_pb(P, T_pa) * stress_pow(P, stress);
+  } else if (LAW == LAW_HOOKE) {
+    const double T_pa = ec_temperature(P, E, p) + P.pb_coeff * p;
+    const double A = P.hooke_A * exp(-P.hooke_Q / (P.R * T_pa) + 3.0 * P.hooke_C * pow(P.hooke_Tr - T_pa, -P.hooke_k));
+    return A * stress_pow(P, stress);
+  } else if (LAW == LAW_ARR) {
+    const double temp = ec_temperature(P, E, p);
+    return P.A_cold * exp(-P.Q_cold / (P.R * temp)) * stress_pow(P, stress);
+  } else if (LAW == LAW_ARRWARM) {
+    const double temp = ec_temperature(P, E, p);
+    return P.A_warm * exp(-P.Q_warm / (P.R * temp)) * stress_pow(P, stress);
+  } else {  // LAW_GK
+    return gk_flow_from_temp(P, stress, ec_temperature(P, E, p), p, gs);
+  }
+}
+
+// SIAFD::grainSizeVostok (SIAFD.cc:980-1018)
+__device__ inline double grain_size_vostok(const DevParams &P, double age_seconds) {
+  const int numPoints = 22;
+  const double ageAt[22] = {0.0000e+00, 5.0000e+01, 1.0000e+02, 1.2500e+02, 1.5000e+02, 1.5800e+02,
+                            1.6500e+02, 1.7000e+02, 1.8000e+02, 1.8800e+02, 2.0000e+02, 2.2500e+02,
+                            2.4500e+02, 2.6000e+02, 3.0000e+02, 3.2000e+02, 3.5000e+02, 4.0000e+02,
+                            5.0000e+02, 6.0000e+02, 8.0000e+02, 1.0000e+04};
+  const double gsAt[22] = {1.8000e-03, 2.2000e-03, 3.0000e-03, 4.0000e-03, 4.3000e-03, 3.0000e-03,
+                           3.0000e-03, 4.6000e-03, 3.4000e-03, 3.3000e-03, 5.9000e-03, 6.2000e-03,
+                           5.4000e-03, 6.8000e-03, 3.5000e-03, 6.0000e-03, 8.0000e-03, 8.3000e-03,
+                           3.6000e-03, 3.8000e-03, 9.5000e-03, 1.0000e-02};
+  const double a = age_seconds * P.sec_to_ka;
+  int l = 0, r = numPoints - 1;
+  if (a < ageAt[l]) return gsAt[l];
+  if (a > ageAt[r]) return gsAt[r];
+  while (r > l + 1) {
+    const int j = (r + l) / 2;
+    if (a < ageAt[j]) r = j; else l = j;
+  }
+  return gsAt[l] + (a - ageAt[l]) * (gsAt[r] - gsAt[l]) / (ageAt[r] - ageAt[l]);
+}
+
+// IceGrid::kBelowHeight (IceGrid.cc:379-391): GSL binary search, clamped to Mz-2.
+// Returns -1 when height is outside [-1e-6, Lz + 1e-6].
+__device__ __forceinline__ int kbelow(const double *__restrict__ z, int Mz, double Lz, double height) {
+  if (height < 0.0 - 1.0e-6 || height > Lz + 1.0e-6) return -1;
+  int ilo = 0, ihi = Mz - 1;
+  while (ihi > ilo + 1) {
+    const int i = (ihi + ilo) >> 1;
+    if (z[i] > height) ihi = i; else ilo = i;
+  }
+  return ilo;
+}
+
+}  // namespace pism_sia

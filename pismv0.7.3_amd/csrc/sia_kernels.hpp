@@ -1,0 +1,58 @@
+// sia_kernels.hpp -- launch interface of the SIA kernels (host side).
+#pragma once
+
+#include "sia_device.hpp"
+
+namespace pism_sia {
+
+struct ColumnArgs {
+  const double *z;           // [Mz]
+  const double *E;           // [Mx][My][Mz]
+  const double *age;         // [Mx][My][Mz] or nullptr
+  const double *thks;        // [Mx][My]
+  const double *theta;       // [Mx][My]
+  const double *hx, *hy;     // [Mx][My][2]
+  double *flux;              // [Mx][My][2]
+  double *Dstag;             // [Mx][My][2]
+  double *I0, *I1;           // [Mx][My][Mz] (full updates; levels 0..ks stored)
+  int *ks;                   // [Mx][My][2] kBelowHeight of each staggered column (full updates)
+  unsigned long long *dmax_bits;
+  int *err;
+};
+
+struct VelArgs {
+  const double *z, *mask, *I0, *I1, *hx, *hy, *ub, *vb, *bmr;
+  const int *ks;
+  double *u, *v, *w;
+};
+
+struct MassArgs {
+  const double *dt;          // device scalar
+  const double *H, *flux, *mask, *ub, *vb, *smb, *bmr;
+  double *H_new;
+  double *partials;          // [nblocks][8] or nullptr
+};
+
+void launch_reset(unsigned long long *dmax_bits, int *err, unsigned long long *cfl_bits, double max_dt,
+                  hipStream_t st);
+void launch_fill(double *a, long n, double v, hipStream_t st);
+void launch_bed_smoother(const DevParams &P, double s2, double s3, double s4, const double *b0, double *bs,
+                         double *mt, double *c2, double *c3, double *c4, hipStream_t st);
+void launch_gradient(const DevParams &P, const double *H, const double *b, const double *h,
+                     const double *mask, double *hx, double *hy, hipStream_t st);
+void launch_theta_thk(const DevParams &P, const double *H, const double *b, const double *h,
+                      const double *mask, const double *topgs, const double *maxtl, const double *C2,
+                      const double *C3, const double *C4, double *theta, double *thks, int *err,
+                      hipStream_t st);
+void launch_column(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st);
+void launch_velocity_uvw(const DevParams &P, const VelArgs &A, hipStream_t st);
+void launch_cfl2d(const DevParams &P, const double *mask, const double *ub, const double *vb,
+                  unsigned long long *cfl_bits, hipStream_t st);
+void launch_dt(const DevParams &P, const unsigned long long *dmax_bits, const unsigned long long *cfl_bits,
+               double t_end, double *t, double *dt_out, hipStream_t st);
+long mass_partial_blocks(const DevParams &P);
+void launch_mass_continuity(const DevParams &P, const MassArgs &A, double *acc_out, hipStream_t st);
+void launch_mask_surface(const DevParams &P, const double *H_new, const double *bed, double *H,
+                         double *mask, double *h, int *err, hipStream_t st);
+
+}  // namespace pism_sia

@@ -1,0 +1,272 @@
+"""Python host side of the HIP SIA library (ctypes over include/pism_sia.h).
+
+``SIAFD`` mirrors PISM's SSB_Modifier/SIAFD interface
+(src/base/stressbalance/SSB_Modifier.hh:37-67, sia/SIAFD.hh:51-116):
+``update(fast)``, ``diffusive_flux()``, ``max_diffusivity()``,
+``velocity_u()/velocity_v()`` plus ``velocity_w()`` (StressBalance). Device
+memory and streams come from PyTorch (ROCm build); every computation runs in
+lib/libpism_sia.so. There is no CPU fallback: importing this module on a
+machine without the built library raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+import sia_types as T
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libpism_sia.so")
+
+STATUS = {0: "SIA_OK", 1: "SIA_ERR_ARGUMENT", 2: "SIA_ERR_HIP", 3: "SIA_ERR_NEGATIVE_THICKNESS",
+          4: "SIA_ERR_THICKNESS_ABOVE_LZ", 5: "SIA_ERR_OMEGA_NEGATIVE", 6: "SIA_ERR_CONFIG",
+          7: "SIA_ERR_OUT_OF_MEMORY", 8: "SIA_ERR_STATE"}
+
+_vp = C.c_void_p
+_PP = C.POINTER(C.c_void_p)
+
+
+class SiaWorkloadSpec(C.Structure):
+    _fields_ = [("Mx_full", C.c_int32), ("My_full", C.c_int32), ("i0", C.c_int32), ("j0", C.c_int32),
+                ("n_bumps", C.c_int32), ("reserved", C.c_int32), ("seed", C.c_uint64),
+                ("dx", C.c_double), ("Hmax", C.c_double)]
+
+
+# exported function -> (restype, argtypes); the ABI declared in include/*.h
+SIGNATURES = {
+    "sia_params_default": (None, [C.POINTER(T.SiaParams)]),
+    "sia_create": (C.c_int, [C.POINTER(T.SiaGrid), C.POINTER(T.SiaParams), _PP]),
+    "sia_destroy": (C.c_int, [_vp]),
+    "sia_set_stream": (C.c_int, [_vp, _vp]),
+    "sia_set_sync_check": (C.c_int, [_vp, C.c_int]),
+    "sia_check_errors": (C.c_int, [_vp]),
+    "sia_update": (C.c_int, [_vp, C.POINTER(T.SiaInputs), C.c_int]),
+    "sia_diffusive_flux": (C.c_int, [_vp, _PP]),
+    "sia_max_diffusivity": (C.c_int, [_vp, C.POINTER(C.c_double)]),
+    "sia_max_diffusivity_device": (C.c_int, [_vp, _PP]),
+    "sia_velocity_u": (C.c_int, [_vp, _PP]),
+    "sia_velocity_v": (C.c_int, [_vp, _PP]),
+    "sia_velocity_w": (C.c_int, [_vp, _PP]),
+    "sia_surface_gradient": (C.c_int, [_vp, _PP, _PP]),
+    "sia_diffusivity_staggered": (C.c_int, [_vp, _PP]),
+    "sia_theta_thksmooth": (C.c_int, [_vp, _PP, _PP]),
+    "sia_bed_smoother_fields": (C.c_int, [_vp, _PP, _PP, _PP, _PP, _PP, C.POINTER(C.c_int32),
+                                          C.POINTER(C.c_int32)]),
+    "sia_max_timestep": (C.c_int, [_vp, C.c_double, C.POINTER(C.c_double)]),
+    "sia_mass_continuity_step": (C.c_int, [_vp, C.POINTER(T.SiaGeometry), C.c_double,
+                                           C.POINTER(T.SiaMassAccounting)]),
+    "sia_step": (C.c_int, [_vp, C.POINTER(T.SiaGeometry), _vp, C.c_int, C.c_double, _vp, _vp]),
+    "sia_last_error": (C.c_char_p, [_vp]),
+    "sia_profiling": (C.c_int, [_vp, C.c_int]),
+    "sia_profiling_read": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
+    "sia_event_name": (C.c_char_p, [C.c_int]),
+    "sia_workload_generate": (C.c_int, [C.POINTER(SiaWorkloadSpec), C.POINTER(T.SiaGrid),
+                                        C.POINTER(T.SiaParams), _vp, _vp, _vp, _vp, _vp, _vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load lib/libpism_sia.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("HIP SIA library not built: %s (run __graft_entry__.build())" % LIB_PATH)
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class SIAError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (STATUS.get(code, "?"), code, msg))
+        self.code = code
+
+
+def default_params():
+    p = T.SiaParams()
+    lib().sia_params_default(C.byref(p))
+    return p
+
+
+# ------------------------------------------------------------ device memory
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("no GPU visible to PyTorch; the HIP SIA library needs one")
+    return torch
+
+
+def to_device(a):
+    torch = _torch()
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64).ravel()).to("cuda")
+
+
+def empty_device(n):
+    torch = _torch()
+    return torch.empty(int(n), dtype=torch.float64, device="cuda")
+
+
+def ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def device_to_numpy(p, n):
+    """Copy n doubles from a raw device pointer (returned by the library) to numpy."""
+    torch = _torch()
+    out = np.empty(int(n), dtype=np.float64)
+    torch.cuda.synchronize()
+    rc = _hip().hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(p), C.c_size_t(8 * int(n)), 2)
+    if rc != 0:
+        raise RuntimeError("hipMemcpy failed: %d" % rc)
+    return out
+
+
+_hiplib = None
+
+
+def _hip():
+    global _hiplib
+    if _hiplib is None:
+        _hiplib = C.CDLL("libamdhip64.so")
+        _hiplib.hipMemcpy.restype = C.c_int
+        _hiplib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    return _hiplib
+
+
+# ------------------------------------------------------------ SIAFD mirror
+class SIAFD:
+    """SSB_Modifier/SIAFD over the C ABI; fields are torch CUDA float64 tensors
+    (flat, PISM layout) owned by the caller (inputs) or the library (outputs)."""
+
+    def __init__(self, grid, params, stream=None):
+        self.grid = grid
+        self.params = params
+        h = C.c_void_p()
+        self._check(lib().sia_create(C.byref(grid), C.byref(params), C.byref(h)), None)
+        self.h = h
+        if stream is not None:
+            self._check(lib().sia_set_stream(self.h, C.c_void_p(stream)))
+        self.n2 = grid.Mx * grid.My
+        self.n3 = self.n2 * grid.Mz
+
+    def _check(self, rc, h="self"):
+        if rc != 0:
+            hh = self.h if h == "self" else None
+            msg = lib().sia_last_error(hh).decode() if hh is not None else ""
+            raise SIAError(rc, msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sia_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_sync_check(self, on):
+        self._check(lib().sia_set_sync_check(self.h, 1 if on else 0))
+
+    def update(self, H, b, h, mask, E, fast=False, age=None, ub=None, vb=None, bmr=None,
+               bed_state_counter=0):
+        """SIAFD::update(vel_input, fast) (SIAFD.cc:119-148) + w when not fast."""
+        inp = T.SiaInputs(ptr(H), ptr(b), ptr(h), ptr(mask), ptr(E), ptr(age), ptr(ub), ptr(vb),
+                          ptr(bmr), bed_state_counter)
+        self._keep = (H, b, h, mask, E, age, ub, vb, bmr)
+        self._check(lib().sia_update(self.h, C.byref(inp), 1 if fast else 0))
+
+    def _get(self, fn):
+        p = C.c_void_p()
+        self._check(fn(self.h, C.byref(p)))
+        return p.value
+
+    def diffusive_flux(self):
+        return device_to_numpy(self._get(lib().sia_diffusive_flux), 2 * self.n2)
+
+    def max_diffusivity(self):
+        d = C.c_double()
+        self._check(lib().sia_max_diffusivity(self.h, C.byref(d)))
+        return d.value
+
+    def velocity_u(self):
+        return device_to_numpy(self._get(lib().sia_velocity_u), self.n3)
+
+    def velocity_v(self):
+        return device_to_numpy(self._get(lib().sia_velocity_v), self.n3)
+
+    def velocity_w(self):
+        return device_to_numpy(self._get(lib().sia_velocity_w), self.n3)
+
+    def diffusivity_staggered(self):
+        return device_to_numpy(self._get(lib().sia_diffusivity_staggered), 2 * self.n2)
+
+    def surface_gradient(self):
+        a, b = C.c_void_p(), C.c_void_p()
+        self._check(lib().sia_surface_gradient(self.h, C.byref(a), C.byref(b)))
+        return device_to_numpy(a.value, 2 * self.n2), device_to_numpy(b.value, 2 * self.n2)
+
+    def theta_thksmooth(self):
+        a, b = C.c_void_p(), C.c_void_p()
+        self._check(lib().sia_theta_thksmooth(self.h, C.byref(a), C.byref(b)))
+        return device_to_numpy(a.value, self.n2), device_to_numpy(b.value, self.n2)
+
+    def bed_smoother_fields(self):
+        ps = [C.c_void_p() for _ in range(5)]
+        Nx, Ny = C.c_int32(), C.c_int32()
+        self._check(lib().sia_bed_smoother_fields(self.h, *[C.byref(p) for p in ps], C.byref(Nx), C.byref(Ny)))
+        out = {k: device_to_numpy(p.value, self.n2) for k, p in zip(("topgsmooth", "maxtl", "C2", "C3", "C4"), ps)}
+        out["Nx"], out["Ny"] = Nx.value, Ny.value
+        return out
+
+    def max_timestep(self, time_to_end=0.0):
+        d = C.c_double()
+        self._check(lib().sia_max_timestep(self.h, time_to_end, C.byref(d)))
+        return d.value
+
+    def mass_continuity_step(self, geom, dt, accounting=True):
+        acc = T.SiaMassAccounting()
+        self._check(lib().sia_mass_continuity_step(self.h, C.byref(geom), dt,
+                                                   C.byref(acc) if accounting else None))
+        return np.array([getattr(acc, n) for n, _ in T.SiaMassAccounting._fields_])
+
+    def profiling(self, on):
+        self._check(lib().sia_profiling(self.h, 1 if on else 0))
+
+    def profiling_read(self):
+        """{event name: (total ms, launches)} since the last read."""
+        ms = (C.c_double * 6)()
+        cnt = (C.c_int64 * 6)()
+        self._check(lib().sia_profiling_read(self.h, ms, cnt))
+        return {lib().sia_event_name(e).decode(): (ms[e], cnt[e]) for e in range(6)}
+
+    def step(self, geom, E, fast=False, t_end=0.0, t_dev=None, dt_dev=None):
+        self._check(lib().sia_step(self.h, C.byref(geom), ptr(E), 1 if fast else 0, t_end,
+                                   ptr(t_dev), ptr(dt_dev)))
+
+
+def geometry(H, h, mask, b, smb=None, bmr=None):
+    return T.SiaGeometry(ptr(H), ptr(h), ptr(mask), ptr(b), ptr(smb), ptr(bmr))
+
+
+def generate_greenland_like(grid, params, i0=0, j0=0, Mx_full=1501, My_full=2801, seed=20261203,
+                            dx=1000.0, Hmax=3200.0, n_bumps=200, with_E=True):
+    """Device generator of the polythermal benchmark sheet (include/sia_workload.h)."""
+    torch = _torch()
+    n2 = grid.Mx * grid.My
+    H, b, h, mask = (torch.empty(n2, dtype=torch.float64, device="cuda") for _ in range(4))
+    E = torch.empty(n2 * grid.Mz, dtype=torch.float64, device="cuda") if with_E else None
+    spec = SiaWorkloadSpec(Mx_full, My_full, i0, j0, n_bumps, 0, seed, dx, Hmax)
+    torch.cuda.synchronize()
+    rc = lib().sia_workload_generate(C.byref(spec), C.byref(grid), C.byref(params), ptr(H), ptr(b),
+                                     ptr(h), ptr(mask), ptr(E), None)
+    if rc != 0:
+        raise SIAError(rc, "sia_workload_generate")
+    return dict(H=H, b=b, h=h, mask=mask, E=E)

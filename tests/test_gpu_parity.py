@@ -1,0 +1,208 @@
+"""GPU parity of the HIP SIA path against the CPU oracle, through the C ABI.
+
+Tolerance (stated per field, FP64): 1e-12 relative for the diffusivity D,
+the flux and D_max (all terms of D are non-negative); for u and v 1e-12 of the
+size of their four summands, and for w 1e-12 of its cumulative integrand
+bound (they are sums with cancellation); 1e-12 relative for the 2-D
+intermediates. Determinism: two runs are bitwise identical.
+"""
+import numpy as np
+import pytest
+
+import sia_types as T
+import workloads as W
+from parity_util import check_against_oracle, run_gpu, run_oracle, say
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def S():
+    import pism_sia
+    import torch
+    assert torch.cuda.is_available()
+    return pism_sia
+
+
+def test_dome_full_update(S, oracle):
+    say("[gpu] dome 61x61x31 full update")
+    w = W.dome()
+    g = run_gpu(w)
+    o = run_oracle(oracle, w)
+    errs = check_against_oracle(g, o, w.grid, label="dome")
+    say("  max rel errors:", {k: "%.1e" % v for k, v in errs.items()})
+    assert 1.40 < g["D_max"] < 1.50
+
+
+@pytest.mark.parametrize("range_", [5e3, 0.0])
+def test_greenland_window_full_update(S, oracle, range_):
+    say("[gpu] greenland_like window 64x80x401, bed_smoother_range", range_)
+    w = W.greenland_like(700, 764, 1180, 1260, bed_smoother_range=range_)
+    assert w.fields["H"].max() > 2000.0
+    g = run_gpu(w)
+    o = run_oracle(oracle, w)
+    errs = check_against_oracle(g, o, w.grid, label="greenland")
+    say("  max rel errors:", {k: "%.1e" % v for k, v in errs.items()})
+
+
+def test_greenland_margin_window(S, oracle):
+    """A window across the ice margin: ice-free cells, margin theta, cliffs."""
+    say("[gpu] greenland_like margin window")
+    w = W.greenland_like(60, 140, 300, 360, Mz=201, Lz=4000.0)
+    m = w.fields["mask"]
+    assert (m == 0).any() and (m == 2).any()
+    g = run_gpu(w)
+    o = run_oracle(oracle, w)
+    check_against_oracle(g, o, w.grid, label="margin")
+
+
+LAWS = ["isothermal_glen", "pb", "gpbld", "hooke", "arr", "arrwarm", "gk"]
+
+
+@pytest.mark.parametrize("law", LAWS)
+@pytest.mark.parametrize("method", ["haseloff", "mahaffy", "eta"])
+def test_flow_laws_and_gradient_methods(S, oracle, law, method):
+    say("[gpu] law", law, "gradient", method)
+    over = dict(sia_flow_law=law, surface_gradient_method=method)
+    if law in ("hooke", "arr", "arrwarm", "pb"):
+        over["sia_enhancement_factor"] = 1.0
+    w = W.greenland_like(740, 772, 1380, 1420, Mz=101, Lz=4000.0, **over)
+    g = run_gpu(w, full=(method == "haseloff"))
+    o = run_oracle(oracle, w, full=(method == "haseloff"))
+    check_against_oracle(g, o, w.grid, full=(method == "haseloff"), label=law + "/" + method)
+
+
+@pytest.mark.parametrize("ec", ["cold", "kirchhoff"])
+def test_enthalpy_converters(S, oracle, ec):
+    say("[gpu] enthalpy converter", ec)
+    w = W.greenland_like(740, 772, 1380, 1420, Mz=101, Lz=4000.0, enthalpy_converter=ec)
+    g = run_gpu(w)
+    o = run_oracle(oracle, w)
+    check_against_oracle(g, o, w.grid, label=ec)
+
+
+def test_glen_exponent_4_and_sliding_and_bmr(S, oracle):
+    """n = 4 (integer power path), a sliding velocity u_b (SIAFD.cc:954-956)
+    and a basal melt rate in w (PISMStressBalance.cc:267-271)."""
+    say("[gpu] n=4 with sliding and basal melt")
+    w = W.greenland_like(740, 772, 1380, 1420, Mz=101, Lz=4000.0, sia_Glen_exponent=4.0)
+    n2 = w.grid.Mx * w.grid.My
+    rng = np.random.default_rng(5)
+    ub = rng.uniform(-1e-6, 1e-6, n2)
+    vb = rng.uniform(-1e-6, 1e-6, n2)
+    bmr = rng.uniform(0, 3e-9, n2)
+    g = run_gpu(w, ub=ub, vb=vb, bmr=bmr)
+    o = run_oracle(oracle, w, ub=ub, vb=vb, bmr=bmr)
+    check_against_oracle(g, o, w.grid, ub=ub, vb=vb, bmr=bmr, label="n4")
+
+
+def test_grain_size_from_age(S, oracle):
+    """Goldsby-Kohlstedt with grain size from age (SIAFD.cc:575-577, 666-669)."""
+    say("[gpu] GK with age-dependent grain size")
+    w = W.greenland_like(740, 772, 1380, 1420, Mz=101, Lz=4000.0, sia_flow_law="gk",
+                         compute_grain_size_using_age=1, do_age=1)
+    rng = np.random.default_rng(9)
+    age = rng.uniform(0, 9e5 * T.SECONDS_PER_YEAR, w.fields["E"].size)
+    g = run_gpu(w, age=age)
+    o = run_oracle(oracle, w, age=age)
+    check_against_oracle(g, o, w.grid, label="gk-age")
+
+
+def test_edge_cases(S, oracle):
+    """Tiny grid (3x3), thickness on a level and at Lz (kBelowHeight clamps to
+    Mz-2), ice reaching the domain edge (D = 0 there, SIAFD.cc:694-697), and an
+    ice-free grid (D_max = 0)."""
+    say("[gpu] edge cases")
+    Mx, My, Mz = 3, 4, 11
+    z = T.vertical_levels(1000.0, Mz)
+    P = T.make_params(sia_flow_law="isothermal_glen", bed_smoother_range=0.0, is_dry_simulation=1)
+    H = np.array([[1000.0, 900.0, 300.0, 0.0], [950.0, 1000.0, 550.0, 100.0], [0.0, 400.0, 700.0, 1000.0]])
+    b = np.array([[0.0, 10.0, 20.0, 30.0], [5.0, 15.0, 25.0, 35.0], [0.0, 1.0, 2.0, 3.0]])
+    mask, h = W.geometry_mask_surface(P, b, H)
+    E = np.full((Mx, My, Mz), 60000.0)
+    grid = T.make_grid(Mx, My, 1000.0, 1000.0, z)
+    w = W.Workload("edge", grid, P, dict(H=H.ravel(), b=b.ravel(), h=h.ravel(), mask=mask.ravel(),
+                                         E=E.ravel()), (Mx, My, Mz), (0, Mx, 0, My))
+    g = run_gpu(w)
+    o = run_oracle(oracle, w)
+    check_against_oracle(g, o, grid, label="edge")
+    assert g["D_max"] > 0
+    flux = g["flux"].reshape(Mx, My, 2)
+    assert np.all(flux[Mx - 1, :, :] == 0.0) and np.all(flux[:, My - 1, :] == 0.0)
+    Z = np.zeros_like(H)
+    mask0, h0 = W.geometry_mask_surface(P, b, Z)
+    w0 = W.Workload("empty", grid, P, dict(H=Z.ravel(), b=b.ravel(), h=h0.ravel(), mask=mask0.ravel(),
+                                           E=E.ravel()), (Mx, My, Mz), (0, Mx, 0, My))
+    g0 = run_gpu(w0)
+    assert g0["D_max"] == 0.0 and np.all(g0["flux"] == 0.0) and np.all(g0["u"] == 0.0)
+
+
+def test_error_status_codes(S):
+    """RuntimeError cases of the reference become status codes (no fault):
+    negative thickness (PISMBedSmoother.cc:312-314), H > Lz (IceGrid.cc:386-389),
+    outputs before an update, invalid configuration."""
+    say("[gpu] error status codes")
+    w = W.dome(Mx=11, My=11, Mz=11)
+    f = {k: S.to_device(v) for k, v in w.fields.items()}
+    sia = S.SIAFD(w.grid, w.params)
+    with pytest.raises(S.SIAError) as e:
+        sia.diffusive_flux()
+    assert e.value.code == 8
+    Hn = w.fields["H"].copy()
+    Hn[7] = -2.0
+    with pytest.raises(S.SIAError) as e:
+        sia.update(S.to_device(Hn), f["b"], f["h"], f["mask"], f["E"])
+    assert e.value.code == 3
+    w2 = W.dome(Mx=11, My=11, Mz=11, Lz=3000.0)
+    sia2 = S.SIAFD(w2.grid, w2.params)
+    f2 = {k: S.to_device(v) for k, v in w2.fields.items()}
+    with pytest.raises(S.SIAError) as e:
+        sia2.update(f2["H"], f2["b"], f2["h"], f2["mask"], f2["E"])
+    assert e.value.code == 4
+    bad = T.make_params(sia_flow_law=9)
+    with pytest.raises(S.SIAError) as e:
+        S.SIAFD(w.grid, bad)
+    assert e.value.code == 6
+    gk = T.make_params(sia_flow_law="gk", compute_grain_size_using_age=1, do_age=0)
+    sia3 = S.SIAFD(w.grid, gk)
+    with pytest.raises(S.SIAError) as e:
+        sia3.update(f["H"], f["b"], f["h"], f["mask"], f["E"])
+    assert e.value.code == 6
+    # the handle still works after an error
+    sia.update(f["H"], f["b"], f["h"], f["mask"], f["E"])
+    assert sia.max_diffusivity() > 0
+
+
+def test_bitwise_deterministic(S):
+    say("[gpu] run-to-run bitwise determinism")
+    w = W.greenland_like(700, 764, 1180, 1260)
+    a = run_gpu(w)
+    b = run_gpu(w)
+    for k in ("flux", "D_stag", "u", "v", "w", "h_x", "h_y"):
+        assert np.array_equal(a[k], b[k]), k
+    assert a["D_max"] == b["D_max"]
+
+
+def test_bed_state_counter_triggers_smoother(S, oracle):
+    """SIAFD.cc:128-133: the bed is re-smoothed only when its counter increases."""
+    say("[gpu] bed state counter")
+    w = W.greenland_like(700, 732, 1180, 1212, Mz=101)
+    f = {k: S.to_device(v) for k, v in w.fields.items()}
+    sia = S.SIAFD(w.grid, w.params)
+    sia.update(f["H"], f["b"], f["h"], f["mask"], f["E"], fast=True, bed_state_counter=0)
+    first = sia.bed_smoother_fields()["topgsmooth"]
+    b2 = w.fields["b"] + 100.0
+    sia.update(f["H"], S.to_device(b2), f["h"], f["mask"], f["E"], fast=True, bed_state_counter=0)
+    assert np.array_equal(sia.bed_smoother_fields()["topgsmooth"], first)
+    sia.update(f["H"], S.to_device(b2), f["h"], f["mask"], f["E"], fast=True, bed_state_counter=1)
+    np.testing.assert_allclose(sia.bed_smoother_fields()["topgsmooth"], first + 100.0, rtol=1e-12)
+
+
+def test_device_generator_matches_numpy(S):
+    say("[gpu] device workload generator vs numpy")
+    import pism_sia
+    w = W.greenland_like(500, 540, 900, 950, Mz=401)
+    dev = pism_sia.generate_greenland_like(w.grid, w.params, i0=500, j0=900)
+    for k in ("H", "b", "h", "mask", "E"):
+        a = dev[k].cpu().numpy()
+        np.testing.assert_allclose(a, w.fields[k], rtol=1e-12, atol=1e-9, err_msg=k)
