@@ -131,11 +131,21 @@ def params_dict(p):
     return {name: getattr(p, name) for name, _ in SiaParams._fields_}
 
 
-def vertical_levels(Lz, Mz):
-    """IceGrid::compute_vertical_levels, EQUAL spacing (IceGrid.cc:352-360)."""
-    dz = Lz / (float(Mz) - 1)
-    z = np.array([dz * float(k) for k in range(Mz - 1)] + [Lz], dtype=np.float64)
-    return z
+def vertical_levels(Lz, Mz, spacing="equal", lam=4.0):
+    """IceGrid::compute_vertical_levels (IceGrid.cc:333-376): EQUAL spacing
+    (352-360) or QUADRATIC with slope lambda at the base (362-369); the top
+    level is exactly Lz."""
+    if spacing == "equal":
+        dz = Lz / (float(Mz) - 1)
+        z = [dz * float(k) for k in range(Mz - 1)]
+    elif spacing == "quadratic":
+        z = []
+        for k in range(Mz - 1):
+            zeta = float(k) / (float(Mz) - 1)
+            z.append(Lz * ((zeta / lam) * (1.0 + (lam - 1.0) * zeta)))
+    else:
+        raise ValueError("spacing must be 'equal' or 'quadratic'")
+    return np.array(z + [Lz], dtype=np.float64)
 
 
 def horizontal_spacing(half_width, M):

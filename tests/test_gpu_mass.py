@@ -137,3 +137,17 @@ def test_cpp_host_driver():
     out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     say(out.stdout)
     assert out.returncode == 0 and "PASSED" in out.stdout
+
+
+@pytest.mark.parametrize("Mx,key", [(31, "Mx31"), (41, "Mx41")])
+def test_gpu_reproduces_test_15_table(S, Mx, key):
+    """The HIP path reproduces every printed digit of test_15's error table
+    (test/regression/test_15.sh:19-28): 5000 years of verification test C."""
+    import json
+    import verification as V
+    say("[gpu] verification test C", Mx)
+    gold = json.load(open(os.path.join(REPO, "tests", "golden", "regression_outputs.json")))["test_C"]
+    errs, steps, _ = V.run_test_c_gpu(Mx, Mx)
+    say("  errors", ["%.6f" % e for e in errs], "golden", gold[key])
+    assert ["%.6f" % e for e in errs] == gold[key]
+    assert steps == 84
