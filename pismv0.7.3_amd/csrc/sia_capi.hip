@@ -37,6 +37,7 @@ struct sia_handle {
   unsigned long long *dmax_bits = nullptr, *cfl_bits = nullptr;
   int *err = nullptr;
   double *dt_dev = nullptr, *acc_dev = nullptr, *partials = nullptr, *H_tmp = nullptr;
+  double *dmax_part = nullptr;
   const double *last_ub = nullptr, *last_vb = nullptr, *last_mask = nullptr;
   bool have_flux = false, have_3d = false;
   // profiling events (Profiling::begin/end of the reference)
@@ -285,6 +286,7 @@ int update_impl(sia_handle *h, const sia_inputs *in, int fast, bool allow_sync) 
   A.I1 = h->I1;
   A.ks = h->ks;
   A.dmax_bits = h->dmax_bits;
+  A.dmax_part = h->dmax_part;
   A.err = h->err;
   {
     Stage stage(h, SIA_EV_FLUX);
@@ -377,6 +379,8 @@ int sia_create(const sia_grid *grid, const sia_params *params, sia_handle **out)
   if (!grid || !params || !out || !grid->z) return SIA_ERR_ARGUMENT;
   *out = nullptr;
   if (grid->Mx < 3 || grid->My < 3 || grid->Mz < 2 || !(grid->dx > 0) || !(grid->dy > 0)) return SIA_ERR_ARGUMENT;
+  // indices of staggered columns are 32-bit; z levels are staged in LDS
+  if (2.0 * grid->Mx * grid->My >= 2147483647.0 || grid->Mz > pism_sia::MAX_MZ) return SIA_ERR_ARGUMENT;
   if (params->sia_flow_law < 0 || params->sia_flow_law > SIA_FLOW_GK) return SIA_ERR_CONFIG;
   if (params->surface_gradient_method < 0 || params->surface_gradient_method > SIA_GRADIENT_ETA) return SIA_ERR_CONFIG;
   if (params->enthalpy_converter < 0 || params->enthalpy_converter > SIA_EC_KIRCHHOFF) return SIA_ERR_CONFIG;
@@ -403,7 +407,8 @@ int sia_create(const sia_grid *grid, const sia_params *params, sia_handle **out)
       (rc = dalloc(h, &h->Dstag, 2 * n2)) || (rc = dalloc(h, &h->dmax_bits, 1)) ||
       (rc = dalloc(h, &h->cfl_bits, 1)) || (rc = dalloc(h, &h->err, 1)) || (rc = dalloc(h, &h->dt_dev, 1)) ||
       (rc = dalloc(h, &h->acc_dev, 8)) || (rc = dalloc(h, &h->H_tmp, n2)) ||
-      (rc = dalloc(h, &h->partials, (size_t)mass_partial_blocks(h->dp) * 8))) {
+      (rc = dalloc(h, &h->partials, (size_t)mass_partial_blocks(h->dp) * 8)) ||
+      (rc = dalloc(h, &h->dmax_part, (size_t)column_blocks(h->dp)))) {
     sia_destroy(h);
     return rc;
   }
@@ -421,7 +426,7 @@ int sia_destroy(sia_handle *h) {
   else (void)hipDeviceSynchronize();
   void *bufs[] = {h->d_z, h->topgs, h->maxtl, h->C2, h->C3, h->C4, h->hx, h->hy, h->theta, h->thks,
                   h->flux, h->Dstag, h->I0, h->I1, h->u, h->v, h->w, h->ks, h->dmax_bits, h->cfl_bits, h->err,
-                  h->dt_dev, h->acc_dev, h->partials, h->H_tmp};
+                  h->dt_dev, h->acc_dev, h->partials, h->H_tmp, h->dmax_part};
   for (void *b : bufs) dfree(b);
   for (auto &r : h->recs) {
     (void)hipEventDestroy(r.a);

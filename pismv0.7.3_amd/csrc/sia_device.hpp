@@ -18,6 +18,9 @@ enum Law { LAW_ISOGLEN = 0, LAW_PB = 1, LAW_GPBLD = 2, LAW_HOOKE = 3, LAW_ARR = 
 enum EcKind { EC_DEFAULT = 0, EC_COLD = 1, EC_KIRCHHOFF = 2 };
 enum ErrBits { ERR_NEG_THK = 1, ERR_THK_LZ = 2, ERR_OMEGA = 4 };
 
+// largest Mz the kernels stage in LDS (z levels)
+constexpr int MAX_MZ = 4096;
+
 struct DevParams {
   int32_t Mx, My, Mz;
   int32_t law, ec, use_age;

@@ -20,18 +20,86 @@
 namespace pism_sia {
 
 // ---------------------------------------------------------------- wave helpers
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+// Cross-lane moves of doubles by DPP (one 32-bit move per half). With
+// BOUND_ZERO a lane whose source lies outside its row reads 0; rows outside
+// ROW_MASK keep the old value 0. Callers keep all 64 lanes active.
+template <int CTRL, int ROW_MASK, bool BOUND_ZERO>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xf, BOUND_ZERO);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xf, BOUND_ZERO);
+  return __hiloint2double(hi, lo);
+}
+
+// value of lane l-1, 0 in lane 0 (wave_shr:1)
+__device__ __forceinline__ double wave_shift_up1(double v) { return dpp_f64<0x138, 0xf, true>(v); }
+
+// inclusive prefix sum over the 64 lanes: Hillis-Steele with row_shr 1, 2,
+// 4, 8 inside each row of 16 lanes, then row_bcast:15 adds the total of row
+// 0 (2) to row 1 (3), and row_bcast:31 the total of lanes 0-31 to rows 2, 3.
+__device__ __forceinline__ double wave_inclusive_scan(double v) {
+  v += dpp_f64<0x111, 0xf, true>(v);
+  v += dpp_f64<0x112, 0xf, true>(v);
+  v += dpp_f64<0x114, 0xf, true>(v);
+  v += dpp_f64<0x118, 0xf, true>(v);
+  v += dpp_f64<0x142, 0xa, false>(v);
+  v += dpp_f64<0x143, 0xc, false>(v);
   return v;
 }
-__device__ __forceinline__ double wave_inclusive_scan(double v, int lane) {
+
+// value held by lane l (l wave-uniform), as a scalar
+__device__ __forceinline__ double lane_value(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ int lane_value(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+__device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const double t = __shfl_up(v, off, 64);
-    if (lane >= off) v += t;
-  }
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
   return v;
+}
+
+constexpr int WAVES_PER_BLOCK = 4;
+
+// ---------------------------------------------------------------- work ordering
+// MI355X dispatches workgroups round-robin over its 8 XCDs, each with its own
+// 4 MB L2. k_velocity_uvw reads the I columns of rows i-2 .. i+1, which at
+// full occupancy do not fit one L2 if each XCD walks whole rows. The grid is
+// cut into strips of STRIP columns (j); XCD x walks strips x, x+8, x+16, ...
+// down the rows (i), so the neighbours of a column were touched a few strip
+// widths earlier by the same XCD and are still in its L2, and the thick
+// interior of the ice sheet is shared evenly by the XCDs.
+constexpr int N_XCD = 8;
+constexpr int STRIP = 32;
+
+struct StripMap {
+  int Mx, My, nstrips, per_xcd;  // per_xcd: work items reserved per XCD
+};
+
+__host__ __device__ inline StripMap strip_map(int Mx, int My, int per_item) {
+  StripMap m;
+  m.Mx = Mx;
+  m.My = My;
+  m.nstrips = (My + STRIP - 1) / STRIP;
+  const int strips_per_xcd = (m.nstrips + N_XCD - 1) / N_XCD;
+  const long cols = (long)strips_per_xcd * Mx * STRIP;
+  m.per_xcd = (int)((cols + per_item - 1) / per_item);
+  return m;
+}
+
+// column of local index Lp (0, 1, ...) of XCD x; false past the end
+__device__ __forceinline__ bool strip_column(const StripMap &m, int x, int Lp, int &i, int &j) {
+  const int span = m.Mx * STRIP;
+  const int ss = Lp / span;
+  const int st = x + N_XCD * ss;
+  if (st >= m.nstrips) return false;
+  const int r = Lp - ss * span;
+  const int width = min(STRIP, m.My - st * STRIP);
+  if (r >= m.Mx * width) return false;
+  i = r / width;
+  j = st * STRIP + (r - i * width);
+  return true;
 }
 
 // ---------------------------------------------------------------- resets
@@ -276,98 +344,168 @@ __global__ void k_theta_thk(DevParams P, Fields2D F, const double *__restrict__ 
 
 // ---------------------------------------------------------------- column kernel
 // compute_diffusive_flux (SIAFD.cc:620-721) and, when FULL, compute_I
-// (SIAFD.cc:845-878). One wavefront per staggered column s = 2*(i*My+j)+o.
+// (SIAFD.cc:845-878). Each wavefront owns COLS_PER_WAVE consecutive
+// staggered columns s = 2*(i*My+j)+o. Lane c first prepares column c (2-D
+// inputs, kBelowHeight), so the 2-D loads of all its columns are in flight
+// together; the wave then walks the columns, one lane per level, with the
+// first 64 levels of E of the next column loaded one column ahead. D, flux
+// and ks leave as one coalesced store per wave. D_max is reduced per block
+// into dmax_part[block]; a one-block kernel takes the max of the partials.
+constexpr int COLS_PER_WAVE = 16;
+constexpr int COLS_PER_BLOCK = WAVES_PER_BLOCK * COLS_PER_WAVE;
+
 template <int LAW, bool FULL>
 __global__ __launch_bounds__(256) void k_column(DevParams P, ColumnArgs A) {
-  const int lane = threadIdx.x & 63;
-  const long s = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  extern __shared__ double zs[];  // Mz levels, staged once per block
+  __shared__ double wmax[WAVES_PER_BLOCK];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int Mx = P.Mx, My = P.My, Mz = P.Mz;
-  if (s >= 2L * Mx * My) return;
-  const int o = (int)(s & 1);
-  const long q = s >> 1;
-  const int i = (int)(q / My), j = (int)(q % My);
-  const long q2 = (o == 0) ? (long)wrapi(i + 1, Mx) * My + j : (long)i * My + wrapi(j + 1, My);
-  const double *__restrict__ z = A.z;
+  for (int k = threadIdx.x; k < Mz; k += blockDim.x) zs[k] = A.z[k];
+  __syncthreads();
+  const int nstag = 2 * Mx * My;
+  const int s0 = (blockIdx.x * WAVES_PER_BLOCK + wv) * COLS_PER_WAVE;
+  const int s = s0 + lane;
 
-  const double thk = 0.5 * (A.thks[q] + A.thks[q2]);
-  double *Icol = FULL ? ((o == 0) ? A.I0 : A.I1) + q * Mz : nullptr;
-
-  // zero thickness (SIAFD.cc:635-641): delta = 0, so I = 0 at every level.
-  // Above ks, I is constant (SIAFD.cc:870-872); only I[0..ks] is stored and
-  // readers use I[min(k, ks)].
-  if (thk == 0.0) {
-    if (lane == 0) {
-      A.flux[s] = 0.0;
-      A.Dstag[s] = 0.0;
-      if (FULL) {
-        Icol[0] = 0.0;
-        A.ks[s] = 0;
+  // lane c < COLS_PER_WAVE: column s0 + c. c_ks: -3 no column, -2 zero
+  // thickness (delta = 0, SIAFD.cc:635-641), -1 thickness above Lz, else kBelowHeight
+  int c_ks = -3, c_qa = 0, c_qb = 0;
+  double c_thk = 0.0, c_alpha = 0.0, c_c0 = 0.0, c_slope = 0.0;
+  bool c_edge = false;
+  if (lane < COLS_PER_WAVE && s < nstag) {
+    const int q = s >> 1, o = s & 1;
+    const int i = q / My, j = q - i * My;
+    const int q2 = (o == 0) ? wrapi(i + 1, Mx) * My + j : i * My + wrapi(j + 1, My);
+    c_qa = q;
+    c_qb = q2;
+    c_thk = 0.5 * (A.thks[q] + A.thks[q2]);
+    c_edge = (i >= Mx - 1 || j >= My - 1);  // SIAFD.cc:694-697
+    if (c_thk == 0.0) {
+      c_ks = -2;
+    } else {
+      c_ks = kbelow(zs, Mz, P.Lz, c_thk);
+      if (c_ks < 0) {
+        atomicOr(A.err, ERR_THK_LZ);
+        c_ks = -1;
       }
     }
-    return;
+    if (c_ks >= 0) {
+      const double hxs = A.hx[s], hys = A.hy[s];
+      c_slope = (o == 0) ? hxs : hys;
+      c_alpha = sqrt(hxs * hxs + hys * hys);
+      const double theta_local = 0.5 * (A.theta[q] + A.theta[q2]);
+      c_c0 = P.e * theta_local * 2.0;
+    }
   }
-  const int ks = kbelow(z, Mz, P.Lz, thk);
-  if (ks < 0) {
-    if (lane == 0) {
-      atomicOr(A.err, ERR_THK_LZ);
-      A.flux[s] = 0.0;
-      A.Dstag[s] = 0.0;
-      if (FULL) {
-        Icol[0] = 0.0;
-        A.ks[s] = 0;
+
+  // first chunk of E (and age) of column c, levels 0..min(ks, 63)
+  double nEa = 0.0, nEb = 0.0, nAa = 0.0, nAb = 0.0;
+  auto prefetch = [&](int c) {
+    const int ks = lane_value(c_ks, c);
+    if (ks >= 0 && lane <= ks) {
+      const long qa = lane_value(c_qa, c), qb = lane_value(c_qb, c);
+      nEa = A.E[qa * Mz + lane];
+      nEb = A.E[qb * Mz + lane];
+      if (P.use_age) {
+        nAa = A.age[qa * Mz + lane];
+        nAb = A.age[qb * Mz + lane];
       }
     }
-    return;
-  }
-  const double hxs = A.hx[s], hys = A.hy[s];
-  const double slope = (o == 0) ? hxs : hys;
-  const double alpha = sqrt(hxs * hxs + hys * hys);
-  const double theta_local = 0.5 * (A.theta[q] + A.theta[q2]);
-  const double *__restrict__ Ea = A.E + q * Mz;
-  const double *__restrict__ Eb = A.E + q2 * Mz;
-  const double *__restrict__ aa = P.use_age ? A.age + q * Mz : nullptr;
-  const double *__restrict__ ab = P.use_age ? A.age + q2 * Mz : nullptr;
-  const double c0 = P.e * theta_local * 2.0;
+  };
+  prefetch(0);
 
-  double Dpart = 0.0, dprev_carry = 0.0, Icarry = 0.0;
-  for (int k0 = 0; k0 <= ks; k0 += 64) {
-    const int k = k0 + lane;
-    const bool act = k <= ks;
-    double delta = 0.0, depth = 0.0, dz = 0.0;
-    if (act) {
-      depth = thk - z[k];
-      const double pressure = ec_pressure(P, depth);
-      const double gs = P.use_age ? grain_size_vostok(P, 0.5 * (aa[k] + ab[k])) : P.gs;
-      const double Ek = 0.5 * (Ea[k] + Eb[k]);
-      const double fl = flow<LAW>(P, alpha * pressure, Ek, pressure, gs);
-      delta = c0 * pressure * fl;
-      if (k > 0) dz = z[k] - z[k - 1];
+  double myD = 0.0;  // lane c: D of column c before the edge override
+  for (int c = 0; c < COLS_PER_WAVE; c++) {
+    const int ks = lane_value(c_ks, c);
+    if (ks == -3) break;
+    const double Ea0 = nEa, Eb0 = nEb, Aa0 = nAa, Ab0 = nAb;
+    if (c + 1 < COLS_PER_WAVE) prefetch(c + 1);
+    const long qa = lane_value(c_qa, c), qb = lane_value(c_qb, c);
+    const int s_c = s0 + c;
+    double *Icol = FULL ? ((s_c & 1) ? A.I1 : A.I0) + (long)(s_c >> 1) * Mz : nullptr;
+    if (ks < 0) {
+      // I = 0 at every level; above ks readers use I[ks] with ks stored as 0
+      if (FULL && lane == 0) Icol[0] = 0.0;
+      continue;
     }
-    double dprev = __shfl_up(delta, 1, 64);
-    if (lane == 0) dprev = dprev_carry;
-    if (act && k > 0) {
-      Dpart += 0.5 * dz * ((depth + dz) * dprev + depth * delta);
+    const double thk = lane_value(c_thk, c), alpha = lane_value(c_alpha, c), c0 = lane_value(c_c0, c);
+    const double *__restrict__ Ea = A.E + qa * Mz;
+    const double *__restrict__ Eb = A.E + qb * Mz;
+    const double *__restrict__ aa = P.use_age ? A.age + qa * Mz : nullptr;
+    const double *__restrict__ ab = P.use_age ? A.age + qb * Mz : nullptr;
+    double Dpart = 0.0, dprev_carry = 0.0, Icarry = 0.0;
+    for (int k0 = 0; k0 <= ks; k0 += 64) {
+      const int k = k0 + lane;
+      const bool act = k <= ks;
+      double ea = Ea0, eb = Eb0, ga = Aa0, gb = Ab0;
+      if (k0 > 0 && act) {
+        ea = Ea[k];
+        eb = Eb[k];
+        if (P.use_age) {
+          ga = aa[k];
+          gb = ab[k];
+        }
+      }
+      double delta = 0.0, depth = 0.0, dz = 0.0;
+      if (act) {
+        depth = thk - zs[k];
+        const double pressure = ec_pressure(P, depth);
+        const double gs = P.use_age ? grain_size_vostok(P, 0.5 * (ga + gb)) : P.gs;
+        const double Ek = 0.5 * (ea + eb);
+        const double fl = flow<LAW>(P, alpha * pressure, Ek, pressure, gs);
+        delta = c0 * pressure * fl;
+        if (k > 0) dz = zs[k] - zs[k - 1];
+      }
+      double dprev = wave_shift_up1(delta);
+      if (lane == 0) dprev = dprev_carry;
+      if (act && k > 0) {
+        Dpart += 0.5 * dz * ((depth + dz) * dprev + depth * delta);  // SIAFD.cc:677-680
+      }
+      if (k == ks) {
+        const double dzt = thk - zs[ks];  // SIAFD.cc:682-684
+        Dpart += 0.5 * dzt * dzt * delta;
+      }
+      if (FULL) {
+        const double t = (act && k > 0) ? 0.5 * dz * (dprev + delta) : 0.0;  // SIAFD.cc:863-867
+        const double Ik = Icarry + wave_inclusive_scan(t);
+        if (act) Icol[k] = Ik;
+        Icarry = lane_value(Ik, 63);
+      }
+      dprev_carry = lane_value(delta, 63);
     }
-    if (k == ks) {
-      const double dzt = thk - z[ks];
-      Dpart += 0.5 * dzt * dzt * delta;
-    }
-    if (FULL) {
-      const double t = (act && k > 0) ? 0.5 * dz * (dprev + delta) : 0.0;
-      const double Ik = Icarry + wave_inclusive_scan(t, lane);
-      if (act) Icol[k] = Ik;
-      Icarry = __shfl(Ik, 63, 64);
-    }
-    dprev_carry = __shfl(delta, 63, 64);
+    const double D = lane_value(wave_inclusive_scan(Dpart), 63);
+    if (lane == c) myD = D;
   }
-  if (FULL && lane == 0) A.ks[s] = ks;
-  double D = wave_sum(Dpart);
-  if (lane == 0) {
-    A.Dstag[s] = D;
-    if (i >= Mx - 1 || j >= My - 1) D = 0.0;  // SIAFD.cc:694-697
-    atomicMax(A.dmax_bits, (unsigned long long)__double_as_longlong(D));
-    A.flux[s] = -D * slope;
+
+  double dmax = 0.0;
+  if (lane < COLS_PER_WAVE && c_ks != -3) {
+    const double Dout = c_edge ? 0.0 : myD;
+    A.Dstag[s] = myD;
+    A.flux[s] = (c_ks < 0) ? 0.0 : -Dout * c_slope;
+    if (FULL) A.ks[s] = (c_ks < 0) ? 0 : c_ks;
+    dmax = Dout;
   }
+  dmax = wave_max(dmax);  // D >= 0: +0.0 is the identity (std::max, SIAFD.cc:699)
+  if (lane == 0) wmax[wv] = dmax;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = wmax[0];
+    for (int w = 1; w < WAVES_PER_BLOCK; w++) m = (m < wmax[w]) ? wmax[w] : m;
+    A.dmax_part[blockIdx.x] = m;
+  }
+}
+
+// max of the per-block partials (one block); D >= 0 so +0.0 is the identity
+__global__ void k_reduce_max(const double *__restrict__ part, int n, unsigned long long *dmax_bits) {
+  __shared__ double red[1024];
+  double m = 0.0;
+  for (int b = threadIdx.x; b < n; b += blockDim.x) m = (m < part[b]) ? part[b] : m;
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = blockDim.x >> 1; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] = (red[threadIdx.x] < red[threadIdx.x + off]) ? red[threadIdx.x + off] : red[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *dmax_bits = (unsigned long long)__double_as_longlong(red[0]);
 }
 
 // ---------------------------------------------------------------- u, v, w
@@ -379,50 +517,96 @@ __global__ __launch_bounds__(256) void k_column(DevParams P, ColumnArgs A) {
 // each neighbour value is bitwise the one that neighbour writes), then w by a
 // trapezoid prefix scan over the levels. u, v, w are written once; u, v are
 // never read back from HBM.
-struct UV4 {
-  long e, w, n, s;   // staggered column indices (2 q + o) of I_e, I_w, I_n, I_s
-  double hx_e, hx_w, hx_n, hx_s, hy_e, hy_w, hy_n, hy_s;
-  double ub, vb;
+//
+// Lanes 0..15 put the 16 staggered neighbours into a per-wave LDS table
+// (I column, its kBelowHeight, h_x, h_y at that staggered point); each level
+// reads I[min(k, ks)] (I is constant above ks, SIAFD.cc:870-872). Above the
+// largest ks of the 16 every input is constant, so u, v and the w integrand
+// are those of level kmax.
+struct VelSlot {
+  const double *col;
+  int ks, pad;
+  double hx, hy;
 };
 
-__device__ __forceinline__ UV4 uv_setup(const DevParams &P, const double *__restrict__ hx,
-                                        const double *__restrict__ hy, const double *__restrict__ ub,
-                                        const double *__restrict__ vb, int i, int j) {
-  const int Mx = P.Mx, My = P.My;
-  const long q = (long)i * My + j;
-  const long qw = (long)wrapi(i - 1, Mx) * My + j, qs = (long)i * My + wrapi(j - 1, My);
-  UV4 c;
-  c.e = 2 * q; c.w = 2 * qw; c.n = 2 * q + 1; c.s = 2 * qs + 1;
-  c.hx_w = hx[c.w]; c.hx_e = hx[c.e]; c.hx_n = hx[c.n]; c.hx_s = hx[c.s];
-  c.hy_w = hy[c.w]; c.hy_e = hy[c.e]; c.hy_n = hy[c.n]; c.hy_s = hy[c.s];
-  c.ub = ub ? ub[q] : 0.0;
-  c.vb = vb ? vb[q] : 0.0;
-  return c;
-}
-
-__device__ __forceinline__ double Iat(const double *__restrict__ I, const int *__restrict__ ks, long sidx,
-                                      int Mz, int k) {
-  const int kk = min(k, ks[sidx]);
-  return I[(sidx >> 1) * (long)Mz + kk];
-}
-
-__global__ __launch_bounds__(256) void k_velocity_uvw(DevParams P, VelArgs A) {
-  const int lane = threadIdx.x & 63;
-  const long q = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int Mx = P.Mx, My = P.My, Mz = P.Mz;
-  if (q >= (long)Mx * My) return;
-  const int i = (int)(q / My), j = (int)(q % My);
+// staggered index of neighbour slot t of column (i,j)
+//  u(i-1,j): e=I0(i-1,j) w=I0(i-2,j) n=I1(i-1,j) s=I1(i-1,j-1)
+//  u(i,j)  : e=I0(i,j)   w=I0(i-1,j) n=I1(i,j)   s=I1(i,j-1)
+//  u(i+1,j): e=I0(i+1,j) w=I0(i,j)   n=I1(i+1,j) s=I1(i+1,j-1)
+//  v(i,j-1): e=I0(i,j-1) w=I0(i-1,j-1) n=I1(i,j-1) s=I1(i,j-2)
+//  v(i,j+1): e=I0(i,j+1) w=I0(i-1,j+1) n=I1(i,j+1) s=I1(i,j)
+__device__ __forceinline__ int vel_slot(int t, int i, int j, int Mx, int My) {
   const int im = wrapi(i - 1, Mx), ip = wrapi(i + 1, Mx), jm = wrapi(j - 1, My), jp = wrapi(j + 1, My);
-  const UV4 cc = uv_setup(P, A.hx, A.hy, A.ub, A.vb, i, j);
-  const UV4 cw = uv_setup(P, A.hx, A.hy, A.ub, A.vb, im, j);
-  const UV4 ce = uv_setup(P, A.hx, A.hy, A.ub, A.vb, ip, j);
-  const UV4 cs = uv_setup(P, A.hx, A.hy, A.ub, A.vb, i, jm);
-  const UV4 cn = uv_setup(P, A.hx, A.hy, A.ub, A.vb, i, jp);
+  const int im2 = wrapi(i - 2, Mx), jm2 = wrapi(j - 2, My);
+  switch (t) {
+  case 0: return 2 * (i * My + j);          // I0(i,j)
+  case 1: return 2 * (im * My + j);         // I0(i-1,j)
+  case 2: return 2 * (i * My + j) + 1;      // I1(i,j)
+  case 3: return 2 * (i * My + jm) + 1;     // I1(i,j-1)
+  case 4: return 2 * (im2 * My + j);        // I0(i-2,j)
+  case 5: return 2 * (im * My + j) + 1;     // I1(i-1,j)
+  case 6: return 2 * (im * My + jm) + 1;    // I1(i-1,j-1)
+  case 7: return 2 * (ip * My + j);         // I0(i+1,j)
+  case 8: return 2 * (ip * My + j) + 1;     // I1(i+1,j)
+  case 9: return 2 * (ip * My + jm) + 1;    // I1(i+1,j-1)
+  case 10: return 2 * (i * My + jm);        // I0(i,j-1)
+  case 11: return 2 * (im * My + jm);       // I0(i-1,j-1)
+  case 12: return 2 * (i * My + jm2) + 1;   // I1(i,j-2)
+  case 13: return 2 * (i * My + jp);        // I0(i,j+1)
+  case 14: return 2 * (im * My + jp);       // I0(i-1,j+1)
+  default: return 2 * (i * My + jp) + 1;    // I1(i,j+1)
+  }
+}
+
+// -0.25 * (I_e h_e + I_w h_w + I_n h_n + I_s h_s) + basal (SIAFD.cc:949-956)
+__device__ __forceinline__ double uv_formula(double Ie, double he, double Iw, double hw, double In, double hn,
+                                             double Is, double hs, double basal) {
+  return -0.25 * (Ie * he + Iw * hw + In * hn + Is * hs) + basal;
+}
+
+__global__ __launch_bounds__(256) void k_velocity_uvw(DevParams P, VelArgs A, StripMap sm) {
+  extern __shared__ double zs[];
+  __shared__ VelSlot tbl[WAVES_PER_BLOCK][16];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int Mx = P.Mx, My = P.My, Mz = P.Mz;
+  for (int k = threadIdx.x; k < Mz; k += blockDim.x) zs[k] = A.z[k];
+  __syncthreads();
+  // block b runs on XCD b % 8 as the (b / 8)-th block of that XCD
+  const int x = blockIdx.x % N_XCD;
+  const int Lp = __builtin_amdgcn_readfirstlane((blockIdx.x / N_XCD) * WAVES_PER_BLOCK + wv);
+  int i, j;
+  if (!strip_column(sm, x, Lp, i, j)) return;
+  int myks = 0;
+  if (lane < 16) {
+    const int sid = vel_slot(lane, i, j, Mx, My);
+    VelSlot e;
+    e.ks = A.ks[sid];
+    e.pad = 0;
+    e.col = ((sid & 1) ? A.I1 : A.I0) + (long)(sid >> 1) * Mz;
+    e.hx = A.hx[sid];
+    e.hy = A.hy[sid];
+    tbl[wv][lane] = e;
+    myks = e.ks;
+  }
+  // the table is private to this wavefront: a wavefront-scope fence orders
+  // its LDS writes before the reads below
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  int kmax = 0;
+#pragma unroll
+  for (int t = 0; t < 16; t++) kmax = max(kmax, lane_value(myks, t));
+  const int q = i * My + j;
+  const int im = wrapi(i - 1, Mx), ip = wrapi(i + 1, Mx), jm = wrapi(j - 1, My), jp = wrapi(j + 1, My);
+  const double ub_c = A.ub ? A.ub[q] : 0.0, ub_w = A.ub ? A.ub[im * My + j] : 0.0,
+               ub_e = A.ub ? A.ub[ip * My + j] : 0.0;
+  const double vb_c = A.vb ? A.vb[q] : 0.0, vb_s = A.vb ? A.vb[i * My + jm] : 0.0,
+               vb_n = A.vb ? A.vb[i * My + jp] : 0.0;
   // margin-aware differences (PISMStressBalance.cc:223-260)
   const double *mask = A.mask;
   const int m = as_int(mask[q]);
-  const int me = as_int(mask[(long)ip * My + j]), mw = as_int(mask[(long)im * My + j]);
-  const int mn = as_int(mask[(long)i * My + jp]), ms = as_int(mask[(long)i * My + jm]);
+  const int me = as_int(mask[ip * My + j]), mw = as_int(mask[im * My + j]);
+  const int mn = as_int(mask[i * My + jp]), ms = as_int(mask[i * My + jm]);
   double west = 1, east = 1, south = 1, north = 1;
   if ((m_icy(m) && m_ice_free(me)) || (m_ice_free(m) && m_icy(me))) east = 0;
   if ((m_icy(m) && m_ice_free(mw)) || (m_ice_free(m) && m_icy(mw))) west = 0;
@@ -431,64 +615,62 @@ __global__ __launch_bounds__(256) void k_velocity_uvw(DevParams P, VelArgs A) {
   if ((m_icy(m) && m_ice_free(ms)) || (m_ice_free(m) && m_icy(ms))) south = 0;
   const double D_y = (north + south > 0) ? 1.0 / (P.dy * (north + south)) : 0.0;
   const double w0 = A.bmr ? -A.bmr[q] : 0.0;
-  const double *I0 = A.I0, *I1 = A.I1;
-  const int *ks = A.ks;
-  const double *__restrict__ z = A.z;
+  double *__restrict__ uo = A.u + (long)q * Mz;
+  double *__restrict__ vo = A.v + (long)q * Mz;
+  double *__restrict__ wo = A.w + (long)q * Mz;
+  const VelSlot *T = tbl[wv];
+  double uc_t = 0.0, vc_t = 0.0, int_t = 0.0;  // values at kmax, constant above it
   double carry_int = 0.0, carry_sum = 0.0;
   for (int k0 = 0; k0 < Mz; k0 += 64) {
     const int k = k0 + lane;
     const bool act = k < Mz;
-    const int kc = act ? k : Mz - 1;
-    double uc, vc, integrand = 0.0, dz = 0.0;
-    {
-      // u, v at (i,j): SIAFD.cc:949-956
-      const double ie = Iat(I0, ks, cc.e, Mz, kc), iw = Iat(I0, ks, cc.w, Mz, kc);
-      const double in = Iat(I1, ks, cc.n, Mz, kc), is = Iat(I1, ks, cc.s, Mz, kc);
-      uc = -0.25 * (ie * cc.hx_e + iw * cc.hx_w + in * cc.hx_n + is * cc.hx_s) + cc.ub;
-      vc = -0.25 * (ie * cc.hy_e + iw * cc.hy_w + in * cc.hy_n + is * cc.hy_s) + cc.vb;
-    }
-    double ue = 0.0, uw = 0.0, vn = 0.0, vs = 0.0;
-    if (D_x != 0.0) {
-      if (east != 0.0) {
-        const double ie = Iat(I0, ks, ce.e, Mz, kc), iw = Iat(I0, ks, ce.w, Mz, kc);
-        const double in = Iat(I1, ks, ce.n, Mz, kc), is = Iat(I1, ks, ce.s, Mz, kc);
-        ue = -0.25 * (ie * ce.hx_e + iw * ce.hx_w + in * ce.hx_n + is * ce.hx_s) + ce.ub;
+    double uc, vc, integrand;
+    if (k0 <= kmax) {
+      const int kk = act ? k : Mz - 1;
+      double Iv[16];
+      if (kmax == 0) {
+        // every neighbour is ice-free or thinner than the first level: I = I[0] = 0
+#pragma unroll
+        for (int t = 0; t < 16; t++) Iv[t] = 0.0;
+      } else {
+#pragma unroll
+        for (int t = 0; t < 16; t++) Iv[t] = T[t].col[min(kk, T[t].ks)];
       }
-      if (west != 0.0) {
-        const double ie = Iat(I0, ks, cw.e, Mz, kc), iw = Iat(I0, ks, cw.w, Mz, kc);
-        const double in = Iat(I1, ks, cw.n, Mz, kc), is = Iat(I1, ks, cw.s, Mz, kc);
-        uw = -0.25 * (ie * cw.hx_e + iw * cw.hx_w + in * cw.hx_n + is * cw.hx_s) + cw.ub;
-      }
-    }
-    if (D_y != 0.0) {
-      if (north != 0.0) {
-        const double ie = Iat(I0, ks, cn.e, Mz, kc), iw = Iat(I0, ks, cn.w, Mz, kc);
-        const double in = Iat(I1, ks, cn.n, Mz, kc), is = Iat(I1, ks, cn.s, Mz, kc);
-        vn = -0.25 * (ie * cn.hy_e + iw * cn.hy_w + in * cn.hy_n + is * cn.hy_s) + cn.vb;
-      }
-      if (south != 0.0) {
-        const double ie = Iat(I0, ks, cs.e, Mz, kc), iw = Iat(I0, ks, cs.w, Mz, kc);
-        const double in = Iat(I1, ks, cs.n, Mz, kc), is = Iat(I1, ks, cs.s, Mz, kc);
-        vs = -0.25 * (ie * cs.hy_e + iw * cs.hy_w + in * cs.hy_n + is * cs.hy_s) + cs.vb;
-      }
-    }
-    if (act) {
-      A.u[q * Mz + k] = uc;
-      A.v[q * Mz + k] = vc;
-      // PISMStressBalance.cc:276-278; a zero weight multiplies a zero here,
-      // as in the reference where west*(...) is 0 * finite
+      uc = uv_formula(Iv[0], T[0].hx, Iv[1], T[1].hx, Iv[2], T[2].hx, Iv[3], T[3].hx, ub_c);
+      vc = uv_formula(Iv[0], T[0].hy, Iv[1], T[1].hy, Iv[2], T[2].hy, Iv[3], T[3].hy, vb_c);
+      const double uw = uv_formula(Iv[1], T[1].hx, Iv[4], T[4].hx, Iv[5], T[5].hx, Iv[6], T[6].hx, ub_w);
+      const double ue = uv_formula(Iv[7], T[7].hx, Iv[0], T[0].hx, Iv[8], T[8].hx, Iv[9], T[9].hx, ub_e);
+      const double vs = uv_formula(Iv[10], T[10].hy, Iv[11], T[11].hy, Iv[3], T[3].hy, Iv[12], T[12].hy, vb_s);
+      const double vn = uv_formula(Iv[13], T[13].hy, Iv[14], T[14].hy, Iv[15], T[15].hy, Iv[2], T[2].hy, vb_n);
+      // PISMStressBalance.cc:276-278
       const double u_x = D_x * (west * (uc - uw) + east * (ue - uc));
       const double v_y = D_y * (south * (vc - vs) + north * (vn - vc));
       integrand = u_x + v_y;
-      if (k > 0) dz = z[k] - z[k - 1];
+      if (kmax < k0 + 64) {
+        uc_t = lane_value(uc, kmax - k0);
+        vc_t = lane_value(vc, kmax - k0);
+        int_t = lane_value(integrand, kmax - k0);
+      }
+    } else {
+      uc = uc_t;
+      vc = vc_t;
+      integrand = int_t;
     }
-    double iprev = __shfl_up(integrand, 1, 64);
+    double dz = 0.0;
+    if (act) {
+      uo[k] = uc;
+      vo[k] = vc;
+      if (k > 0) dz = zs[k] - zs[k - 1];
+    } else {
+      integrand = 0.0;
+    }
+    double iprev = wave_shift_up1(integrand);
     if (lane == 0) iprev = carry_int;
-    const double t = (act && k > 0) ? 0.5 * (integrand + iprev) * dz : 0.0;
-    const double S = carry_sum + wave_inclusive_scan(t, lane);
-    if (act) A.w[q * Mz + k] = w0 - S;
-    carry_sum = __shfl(S, 63, 64);
-    carry_int = __shfl(integrand, 63, 64);
+    const double t = (act && k > 0) ? 0.5 * (integrand + iprev) * dz : 0.0;  // PISMStressBalance.cc:280-282
+    const double S = carry_sum + wave_inclusive_scan(t);
+    if (act) wo[k] = w0 - S;
+    carry_sum = lane_value(S, 63);
+    carry_int = lane_value(integrand, 63);
   }
 }
 
@@ -678,14 +860,14 @@ __global__ void k_mask_surface(DevParams P, const double *__restrict__ H_new, co
 // ---------------------------------------------------------------- launchers
 template <int LAW>
 static void launch_column_law(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st) {
-  const long nstag = 2L * P.Mx * P.My;
-  const dim3 block(256);
-  const dim3 grid((unsigned)((nstag + 3) / 4));
+  const int nb = column_blocks(P);
+  const size_t lds = sizeof(double) * (size_t)P.Mz;
   if (full) {
-    hipLaunchKernelGGL((k_column<LAW, true>), grid, block, 0, st, P, A);
+    hipLaunchKernelGGL((k_column<LAW, true>), dim3(nb), dim3(256), lds, st, P, A);
   } else {
-    hipLaunchKernelGGL((k_column<LAW, false>), grid, block, 0, st, P, A);
+    hipLaunchKernelGGL((k_column<LAW, false>), dim3(nb), dim3(256), lds, st, P, A);
   }
+  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, st, A.dmax_part, nb, A.dmax_bits);
 }
 
 void launch_column(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st) {
@@ -701,6 +883,8 @@ void launch_column(const DevParams &P, const ColumnArgs &A, bool full, hipStream
 }
 
 static inline unsigned blocks_for(long n, int per_block) { return (unsigned)((n + per_block - 1) / per_block); }
+
+int column_blocks(const DevParams &P) { return (int)blocks_for(2L * P.Mx * P.My, COLS_PER_BLOCK); }
 
 void launch_reset(unsigned long long *dmax_bits, int *err, unsigned long long *cfl_bits, double max_dt,
                   hipStream_t st) {
@@ -737,8 +921,9 @@ void launch_theta_thk(const DevParams &P, const double *H, const double *b, cons
 }
 
 void launch_velocity_uvw(const DevParams &P, const VelArgs &A, hipStream_t st) {
-  const long n = (long)P.Mx * P.My;
-  hipLaunchKernelGGL(k_velocity_uvw, dim3(blocks_for(n, 4)), dim3(256), 0, st, P, A);
+  const StripMap sm = strip_map(P.Mx, P.My, WAVES_PER_BLOCK);
+  hipLaunchKernelGGL(k_velocity_uvw, dim3(sm.per_xcd * N_XCD), dim3(256), sizeof(double) * (size_t)P.Mz, st, P,
+                     A, sm);
 }
 
 void launch_cfl2d(const DevParams &P, const double *mask, const double *ub, const double *vb,

@@ -17,6 +17,7 @@ struct ColumnArgs {
   double *I0, *I1;           // [Mx][My][Mz] (full updates; levels 0..ks stored)
   int *ks;                   // [Mx][My][2] kBelowHeight of each staggered column (full updates)
   unsigned long long *dmax_bits;
+  double *dmax_part;         // [column_blocks(P)] per-block maxima of D
   int *err;
 };
 
@@ -44,6 +45,7 @@ void launch_theta_thk(const DevParams &P, const double *H, const double *b, cons
                       const double *mask, const double *topgs, const double *maxtl, const double *C2,
                       const double *C3, const double *C4, double *theta, double *thks, int *err,
                       hipStream_t st);
+int column_blocks(const DevParams &P);
 void launch_column(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st);
 void launch_velocity_uvw(const DevParams &P, const VelArgs &A, hipStream_t st);
 void launch_cfl2d(const DevParams &P, const double *mask, const double *ub, const double *vb,

@@ -48,7 +48,7 @@ def test_greenland_window_full_update(S, oracle, range_):
 def test_greenland_margin_window(S, oracle):
     """A window across the ice margin: ice-free cells, margin theta, cliffs."""
     say("[gpu] greenland_like margin window")
-    w = W.greenland_like(60, 140, 300, 360, Mz=201, Lz=4000.0)
+    w = W.greenland_like(150, 190, 1380, 1430, Mz=201, Lz=4000.0)
     m = w.fields["mask"]
     assert (m == 0).any() and (m == 2).any()
     g = run_gpu(w)
