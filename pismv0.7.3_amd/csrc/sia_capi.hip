@@ -2,8 +2,11 @@
 // orchestration of SIAFD::update, vertical velocity, time step and mass
 // continuity on one HIP stream.
 #include "../../include/pism_sia.h"
+#include "../../include/sia_surrogate.h"
 #include "sia_kernels.hpp"
+#include "sia_surrogate.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -40,6 +43,12 @@ struct sia_handle {
   double *dmax_part = nullptr;
   const double *last_ub = nullptr, *last_vb = nullptr, *last_mask = nullptr;
   bool have_flux = false, have_3d = false;
+  // learned surrogate for fast updates (borrowed) and its per-update buffers
+  sia_surrogate *sg = nullptr;
+  unsigned char *sg_status = nullptr;
+  int *sg_block_count = nullptr, *sg_list = nullptr, *sg_nlist = nullptr, *sg_counts = nullptr;
+  float *sg_F = nullptr, *sg_CQb = nullptr;
+  double *sg_CSt = nullptr;
   // profiling events (Profiling::begin/end of the reference)
   struct Rec {
     int ev;
@@ -48,6 +57,27 @@ struct sia_handle {
   bool prof_on = false;
   std::vector<Rec> recs;
   std::vector<hipEvent_t> event_pool;
+};
+
+// learned column surrogate: FP32 master weights, Adam state, packed inference
+// weights, normalization and the training samples (all on one device)
+struct sia_surrogate {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  float *master = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
+  unsigned short *frag = nullptr;
+  SgNorm norm{};
+  bool have_norm = false;
+  double glen_n = 0.0;  // Glen exponent of the training samples (0: none yet)
+  SgSample *samples = nullptr;
+  long long *d_n = nullptr;
+  int64_t cap = 0, n = 0;
+  long long adam_t = 0;
+  // training buffers for batch size tb.B
+  SgTrainBuffers tb{};
+  std::vector<void *> tb_mem;
+  double *d_loss = nullptr;
+  int64_t d_loss_len = 0;
 };
 
 namespace {
@@ -232,6 +262,8 @@ int preprocess_bed(sia_handle *h, const double *bed) {
   return SIA_OK;
 }
 
+int surrogate_flux(sia_handle *h, const sia_inputs *in, ColumnArgs A);
+
 int update_impl(sia_handle *h, const sia_inputs *in, int fast, bool allow_sync) {
   if (!h || !in) return set_err(h, SIA_ERR_ARGUMENT, "NULL argument");
   if (!in->land_ice_thickness || !in->bedrock_altitude || !in->surface_altitude || !in->mask || !in->enthalpy) {
@@ -290,7 +322,11 @@ int update_impl(sia_handle *h, const sia_inputs *in, int fast, bool allow_sync) 
   A.err = h->err;
   {
     Stage stage(h, SIA_EV_FLUX);
-    launch_column(P, A, full, st);
+    if (!full && h->sg) {
+      if ((rc = surrogate_flux(h, in, A))) return rc;
+    } else {
+      launch_column(P, A, full, st);
+    }
   }
   if (full) {
     // "SIA 3D hor. vel." and "SB vert. vel." run as one fused kernel
@@ -323,6 +359,147 @@ int update_impl(sia_handle *h, const sia_inputs *in, int fast, bool allow_sync) 
   if (allow_sync && h->sync_check) return check_flags(h);
   return SIA_OK;
 }
+
+int ensure_sg_buffers(sia_handle *h) {
+  const size_t n2 = (size_t)h->grid.Mx * h->grid.My, ns = 2 * n2;
+  int rc;
+  if ((rc = dalloc(h, &h->sg_status, ns)) || (rc = dalloc(h, &h->sg_block_count, 3 * (size_t)sg_classify_blocks(h->dp))) ||
+      (rc = dalloc(h, &h->sg_list, ns)) || (rc = dalloc(h, &h->sg_nlist, 1)) || (rc = dalloc(h, &h->sg_counts, 3)) ||
+      (rc = dalloc(h, &h->sg_F, ns * SG_FSTRIDE)) || (rc = dalloc(h, &h->sg_CQb, ns * SG_NCELL)) ||
+      (rc = dalloc(h, &h->sg_CSt, ns))) {
+    return rc;
+  }
+  return SIA_OK;
+}
+
+SgColumnArgs sg_column_args(sia_handle *h, const sia_inputs *in) {
+  return SgColumnArgs{h->d_z, in->enthalpy, h->thks, h->theta, h->hx, h->hy, h->err};
+}
+
+SgListArgs sg_list_args(sia_handle *h) {
+  return SgListArgs{h->sg_status, h->sg_block_count, h->sg_list, h->sg_nlist, h->sg_counts, h->flux, h->Dstag};
+}
+
+// flux, D_stag and D_max of a fast update through the surrogate: classify and
+// list the staggered columns, one feature pass over their levels, the fused
+// MLP, then the FP64 column kernel on the fallback columns only
+int surrogate_flux(sia_handle *h, const sia_inputs *in, ColumnArgs A) {
+  int rc;
+  if ((rc = ensure_sg_buffers(h))) return rc;
+  const DevParams &P = h->dp;
+  hipStream_t st = h->stream;
+  const SgColumnArgs C = sg_column_args(h, in);
+  const SgListArgs L = sg_list_args(h);
+  launch_sg_list(P, C, L, st);
+  SgFeatureArgs F{};
+  F.list = h->sg_list;
+  F.n_list = h->sg_nlist;
+  F.F = h->sg_F;
+  F.CQb = h->sg_CQb;
+  F.CSt = h->sg_CSt;
+  F.status = h->sg_status;
+  launch_sg_features(P, C, F, false, st);
+  SgMlpArgs M{};
+  M.list = h->sg_list;
+  M.n_list = h->sg_nlist;
+  M.F = h->sg_F;
+  M.CQb = h->sg_CQb;
+  M.CSt = h->sg_CSt;
+  M.frag = h->sg->frag;
+  M.bias = h->sg->master;
+  M.norm = h->sg->norm;
+  M.hx = h->hx;
+  M.hy = h->hy;
+  M.status = h->sg_status;
+  M.flux = h->flux;
+  M.Dstag = h->Dstag;
+  M.dmax_part = h->dmax_part + column_blocks(P);
+  M.counts = h->sg_counts;
+  launch_sg_mlp(P, M, st);
+  A.only = h->sg_status;
+  launch_column(P, A, false, st, sg_mlp_blocks());
+  HIP_TRY(h, hipGetLastError());
+  return SIA_OK;
+}
+
+// Xavier-uniform initial weights, zero biases (host, seeded)
+void sg_init_params(std::vector<float> &w, uint64_t seed) {
+  w.assign(SG_NPARAM, 0.0f);
+  uint64_t x = seed;
+  auto next = [&x]() {
+    x += 0x9E3779B97F4A7C15ull;
+    uint64_t z = x;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  const int off[5] = {SG_W1, SG_W2, SG_W3, SG_W4, SG_W5};
+  const int fin[5] = {SG_NIN, SG_WIDTH, SG_WIDTH, SG_WIDTH, SG_WIDTH};
+  const int fout[5] = {SG_WIDTH, SG_WIDTH, SG_WIDTH, SG_WIDTH, SG_NOUT};
+  for (int l = 0; l < 5; l++) {
+    const double a = std::sqrt(6.0 / (fin[l] + fout[l]));
+    for (int e = 0; e < fin[l] * fout[l]; e++) w[off[l] + e] = (float)((2.0 * next() - 1.0) * a);
+  }
+}
+
+// (re)allocates the training buffers for batch size B
+int sg_train_buffers(sia_surrogate *sg, int B) {
+  if (sg->tb.B == B) return SIA_OK;
+  for (void *p : sg->tb_mem) dfree(p);
+  sg->tb_mem.clear();
+  sg->tb = SgTrainBuffers{};
+  auto alloc = [sg](size_t bytes, void **p) {
+    if (hipMalloc(p, bytes) != hipSuccess) {
+      *p = nullptr;
+      return false;
+    }
+    sg->tb_mem.push_back(*p);
+    return true;
+  };
+  SgTrainBuffers &T = sg->tb;
+  const size_t b = (size_t)B;
+  bool ok = alloc(b * SG_KIN * 2, (void **)&T.X0);
+  const int N[SG_NLAYER] = {SG_WIDTH, SG_WIDTH, SG_WIDTH, SG_WIDTH, SG_NOUT};
+  for (int l = 0; l < SG_NLAYER && ok; l++) ok = alloc(b * N[l] * 4, (void **)&T.Z[l]);
+  T.Abf[0] = T.X0;
+  for (int l = 1; l < SG_NLAYER && ok; l++) ok = alloc(b * SG_WIDTH * 2, (void **)&T.Abf[l]);
+  ok = ok && alloc(b * SG_NCELL * 4, (void **)&T.T) && alloc(b * SG_NCELL * 4, (void **)&T.S) &&
+       alloc(b * SG_NCELL * 4, (void **)&T.Hm) && alloc(b * SG_WIDTH * 4, (void **)&T.dZ) &&
+       alloc(b * SG_WIDTH * 2, (void **)&T.dZbf) && alloc(b * SG_WIDTH * 4, (void **)&T.dA) &&
+       alloc((size_t)SG_WIDTH * SG_KIN * 4, (void **)&T.dWpad) &&
+       alloc((size_t)(SG_WIDTH * SG_KIN + 3 * SG_WIDTH * SG_WIDTH + SG_NOUT * SG_WIDTH) * 2, (void **)&T.Wbf);
+  if (!ok) {
+    for (void *p : sg->tb_mem) dfree(p);
+    sg->tb_mem.clear();
+    sg->tb = SgTrainBuffers{};
+    return SIA_ERR_OUT_OF_MEMORY;
+  }
+  T.grad = sg->grad;
+  T.B = B;
+  return SIA_OK;
+}
+
+int sg_loss_buffer(sia_surrogate *sg, int64_t n) {
+  if (sg->d_loss_len >= n) return SIA_OK;
+  dfree(sg->d_loss);
+  sg->d_loss = nullptr;
+  sg->d_loss_len = 0;
+  if (hipMalloc((void **)&sg->d_loss, sizeof(double) * (size_t)n) != hipSuccess) {
+    sg->d_loss = nullptr;
+    return SIA_ERR_OUT_OF_MEMORY;
+  }
+  sg->d_loss_len = n;
+  return SIA_OK;
+}
+
+constexpr char SG_MAGIC[8] = {'P', 'S', 'I', 'A', 'S', 'G', '0', '1'};
+struct SgBlobHeader {
+  char magic[8];
+  int32_t nparam, nin;
+  double glen_n;
+  int32_t have_norm, pad;
+};
 
 }  // namespace
 
@@ -408,7 +585,7 @@ int sia_create(const sia_grid *grid, const sia_params *params, sia_handle **out)
       (rc = dalloc(h, &h->cfl_bits, 1)) || (rc = dalloc(h, &h->err, 1)) || (rc = dalloc(h, &h->dt_dev, 1)) ||
       (rc = dalloc(h, &h->acc_dev, 8)) || (rc = dalloc(h, &h->H_tmp, n2)) ||
       (rc = dalloc(h, &h->partials, (size_t)mass_partial_blocks(h->dp) * 8)) ||
-      (rc = dalloc(h, &h->dmax_part, (size_t)column_blocks(h->dp)))) {
+      (rc = dalloc(h, &h->dmax_part, (size_t)(column_blocks(h->dp) + sg_mlp_blocks())))) {
     sia_destroy(h);
     return rc;
   }
@@ -426,7 +603,8 @@ int sia_destroy(sia_handle *h) {
   else (void)hipDeviceSynchronize();
   void *bufs[] = {h->d_z, h->topgs, h->maxtl, h->C2, h->C3, h->C4, h->hx, h->hy, h->theta, h->thks,
                   h->flux, h->Dstag, h->I0, h->I1, h->u, h->v, h->w, h->ks, h->dmax_bits, h->cfl_bits, h->err,
-                  h->dt_dev, h->acc_dev, h->partials, h->H_tmp, h->dmax_part};
+                  h->dt_dev, h->acc_dev, h->partials, h->H_tmp, h->dmax_part, h->sg_status,
+                  h->sg_block_count, h->sg_list, h->sg_nlist, h->sg_counts, h->sg_F, h->sg_CQb, h->sg_CSt};
   for (void *b : bufs) dfree(b);
   for (auto &r : h->recs) {
     (void)hipEventDestroy(r.a);
@@ -644,5 +822,282 @@ const char *sia_event_name(int event) {
 }
 
 const char *sia_last_error(const sia_handle *h) { return h ? h->last_error.c_str() : "NULL handle"; }
+
+
+// ---------------------------------------------------------------- learned surrogate
+int sia_surrogate_create(uint64_t seed, int64_t capacity, sia_surrogate **out) {
+  if (!out || capacity < 0) return SIA_ERR_ARGUMENT;
+  *out = nullptr;
+  sia_surrogate *sg = new (std::nothrow) sia_surrogate();
+  if (!sg) return SIA_ERR_OUT_OF_MEMORY;
+  if (hipGetDevice(&sg->device) != hipSuccess) {
+    delete sg;
+    return SIA_ERR_HIP;
+  }
+  std::vector<float> w;
+  sg_init_params(w, seed);
+  bool ok = hipMalloc((void **)&sg->master, sizeof(float) * SG_NPARAM) == hipSuccess &&
+            hipMalloc((void **)&sg->m, sizeof(float) * SG_NPARAM) == hipSuccess &&
+            hipMalloc((void **)&sg->v, sizeof(float) * SG_NPARAM) == hipSuccess &&
+            hipMalloc((void **)&sg->grad, sizeof(float) * SG_NPARAM) == hipSuccess &&
+            hipMalloc((void **)&sg->frag, sizeof(unsigned short) * SG_NFRAG * 512) == hipSuccess &&
+            hipMalloc((void **)&sg->d_n, sizeof(long long)) == hipSuccess &&
+            (capacity == 0 || hipMalloc((void **)&sg->samples, sizeof(SgSample) * (size_t)capacity) == hipSuccess);
+  if (ok) {
+    const long long zero = 0;
+    ok = hipMemcpy(sg->master, w.data(), sizeof(float) * SG_NPARAM, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemset(sg->m, 0, sizeof(float) * SG_NPARAM) == hipSuccess &&
+         hipMemset(sg->v, 0, sizeof(float) * SG_NPARAM) == hipSuccess &&
+         hipMemcpy(sg->d_n, &zero, sizeof(zero), hipMemcpyHostToDevice) == hipSuccess;
+  }
+  if (!ok) {
+    sia_surrogate_destroy(sg);
+    return SIA_ERR_OUT_OF_MEMORY;
+  }
+  launch_sg_pack(sg->master, sg->frag, nullptr);
+  if (hipDeviceSynchronize() != hipSuccess) {
+    sia_surrogate_destroy(sg);
+    return SIA_ERR_HIP;
+  }
+  sg->cap = capacity;
+  *out = sg;
+  return SIA_OK;
+}
+
+int sia_surrogate_destroy(sia_surrogate *sg) {
+  if (!sg) return SIA_OK;
+  (void)hipDeviceSynchronize();
+  void *bufs[] = {sg->master, sg->m, sg->v, sg->grad, sg->frag, sg->samples, sg->d_n, sg->d_loss};
+  for (void *b : bufs) dfree(b);
+  for (void *p : sg->tb_mem) dfree(p);
+  delete sg;
+  return SIA_OK;
+}
+
+int sia_surrogate_set_stream(sia_surrogate *sg, void *hip_stream) {
+  if (!sg) return SIA_ERR_ARGUMENT;
+  sg->stream = (hipStream_t)hip_stream;
+  return SIA_OK;
+}
+
+int sia_surrogate_collect(sia_surrogate *sg, sia_handle *h, const sia_inputs *in, int stride, int64_t *n_total) {
+  if (!sg || !h || !in || stride < 1) return set_err(h, SIA_ERR_ARGUMENT, "sia_surrogate_collect: bad argument");
+  const sia_params &p = h->params;
+  if (p.sia_flow_law != SIA_FLOW_GPBLD || p.enthalpy_converter != SIA_EC_DEFAULT) {
+    return set_err(h, SIA_ERR_CONFIG, "the surrogate models GPBLD with the default enthalpy converter");
+  }
+  if (sg->glen_n != 0.0 && sg->glen_n != p.sia_Glen_exponent) {
+    return set_err(h, SIA_ERR_CONFIG, "samples with another Glen exponent are already held");
+  }
+  if (!sg->samples) return set_err(h, SIA_ERR_CONFIG, "the surrogate was created without sample capacity");
+  // the FP64 path provides thksmooth, theta and the surface gradient
+  sia_surrogate *saved = h->sg;
+  h->sg = nullptr;
+  int rc = update_impl(h, in, 1, true);
+  h->sg = saved;
+  if (rc) return rc;
+  if ((rc = ensure_sg_buffers(h))) return rc;
+  const SgColumnArgs C = sg_column_args(h, in);
+  const SgListArgs L = sg_list_args(h);
+  launch_sg_list(h->dp, C, L, h->stream);
+  SgFeatureArgs F{};
+  F.list = h->sg_list;
+  F.n_list = h->sg_nlist;
+  F.status = h->sg_status;
+  F.samples = sg->samples;
+  F.n_samples = sg->d_n;
+  F.cap = sg->cap;
+  F.stride = stride;
+  launch_sg_features(h->dp, C, F, true, h->stream);
+  HIP_TRY(h, hipGetLastError());
+  int nlist = 0;
+  HIP_TRY(h, hipMemcpyAsync(&nlist, h->sg_nlist, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  const int64_t nsel = (nlist + stride - 1) / stride;
+  sg->n = std::min<int64_t>(sg->cap, sg->n + nsel);
+  const long long nn = sg->n;
+  HIP_TRY(h, hipMemcpy(sg->d_n, &nn, sizeof(nn), hipMemcpyHostToDevice));
+  sg->glen_n = p.sia_Glen_exponent;
+  if (n_total) *n_total = sg->n;
+  return SIA_OK;
+}
+
+int sia_surrogate_fit_normalization(sia_surrogate *sg) {
+  if (!sg) return SIA_ERR_ARGUMENT;
+  if (sg->n < 2) return SIA_ERR_STATE;
+  const size_t np = (size_t)SG_STAT_WAVES * SG_NIN * 4 + SG_STAT_WAVES;
+  double *d = nullptr;
+  if (hipMalloc((void **)&d, sizeof(double) * np) != hipSuccess) return SIA_ERR_OUT_OF_MEMORY;
+  launch_sg_norm_stats(sg->samples, sg->n, d, sg->stream);
+  std::vector<double> hbuf(np);
+  const bool ok = hipMemcpyAsync(hbuf.data(), d, sizeof(double) * np, hipMemcpyDeviceToHost, sg->stream) == hipSuccess &&
+                  hipStreamSynchronize(sg->stream) == hipSuccess;
+  dfree(d);
+  if (!ok) return SIA_ERR_HIP;
+  const int nw = std::min<int64_t>(SG_STAT_WAVES, sg->n);  // waves past n hold no sample
+  for (int f = 0; f < SG_NIN; f++) {
+    double s = 0, s2 = 0, lo = 1e300, hi = -1e300;
+    for (int w = 0; w < nw; w++) {
+      const double *o = &hbuf[((size_t)w * SG_NIN + f) * 4];
+      s += o[0];
+      s2 += o[1];
+      lo = std::min(lo, o[2]);
+      hi = std::max(hi, o[3]);
+    }
+    const double mean = s / (double)sg->n;
+    const double var = std::max(s2 / (double)sg->n - mean * mean, 0.0);
+    const double sd = std::max(std::sqrt(var), 1e-6);
+    const double margin = 0.02 * (hi - lo) + 1e-6;
+    sg->norm.mean[f] = (float)mean;
+    sg->norm.inv_std[f] = (float)(1.0 / sd);
+    sg->norm.lo[f] = (float)(lo - margin);
+    sg->norm.hi[f] = (float)(hi + margin);
+  }
+  double rmax = 0.0;
+  for (int w = 0; w < nw; w++) rmax = std::max(rmax, hbuf[(size_t)SG_STAT_WAVES * SG_NIN * 4 + w]);
+  sg->norm.rscale = (float)(rmax > 0.0 ? rmax : 1.0);
+  sg->have_norm = true;
+  return SIA_OK;
+}
+
+int sia_surrogate_train(sia_surrogate *sg, int64_t steps, int batch, double lr_start, double lr_end, uint64_t seed,
+                        double *loss_out) {
+  if (!sg || steps < 1 || batch < 32 || batch % 32 != 0) return SIA_ERR_ARGUMENT;
+  if (!sg->have_norm || sg->n < 1) return SIA_ERR_STATE;
+  int rc;
+  if ((rc = sg_train_buffers(sg, batch)) || (rc = sg_loss_buffer(sg, steps))) return rc;
+  if (hipMemsetAsync(sg->d_loss, 0, sizeof(double) * (size_t)steps, sg->stream) != hipSuccess) return SIA_ERR_HIP;
+  const double pi = 3.14159265358979323846;
+  for (int64_t k = 0; k < steps; k++) {
+    const double lr = lr_end + 0.5 * (lr_start - lr_end) * (1.0 + std::cos(pi * (double)k / (double)steps));
+    launch_sg_gather(sg->samples, sg->n, sg->norm, seed, k, sg->tb, sg->stream);
+    SgTrainBuffers T = sg->tb;
+    T.loss = sg->d_loss + k;
+    launch_sg_forward_backward(sg->master, T, sg->norm.rscale, 1e-3f, sg->stream);
+    launch_sg_adam(sg->master, sg->m, sg->v, sg->grad, (float)lr, 0.9f, 0.999f, 1e-8f, ++sg->adam_t, sg->stream);
+  }
+  launch_sg_pack(sg->master, sg->frag, sg->stream);
+  if (hipGetLastError() != hipSuccess) return SIA_ERR_HIP;
+  if (loss_out) {
+    if (hipMemcpyAsync(loss_out, sg->d_loss, sizeof(double) * (size_t)steps, hipMemcpyDeviceToHost, sg->stream) !=
+        hipSuccess)
+      return SIA_ERR_HIP;
+  }
+  if (hipStreamSynchronize(sg->stream) != hipSuccess) return SIA_ERR_HIP;
+  return SIA_OK;
+}
+
+int sia_surrogate_get_params(const sia_surrogate *sg, float *host, size_t n) {
+  if (!sg || !host || n != (size_t)SG_NPARAM) return SIA_ERR_ARGUMENT;
+  if (hipMemcpy(host, sg->master, sizeof(float) * SG_NPARAM, hipMemcpyDeviceToHost) != hipSuccess) return SIA_ERR_HIP;
+  return SIA_OK;
+}
+
+int sia_surrogate_set_params(sia_surrogate *sg, const float *host, size_t n) {
+  if (!sg || !host || n != (size_t)SG_NPARAM) return SIA_ERR_ARGUMENT;
+  if (hipMemcpy(sg->master, host, sizeof(float) * SG_NPARAM, hipMemcpyHostToDevice) != hipSuccess) return SIA_ERR_HIP;
+  launch_sg_pack(sg->master, sg->frag, nullptr);
+  if (hipDeviceSynchronize() != hipSuccess) return SIA_ERR_HIP;
+  return SIA_OK;
+}
+
+int sia_surrogate_grad(sia_surrogate *sg, int batch, const float *x, const float *r, const float *share,
+                       const float *has, double *loss_host, float *grad_host) {
+  if (!sg || !x || !r || !share || !has || batch < 32 || batch % 32 != 0) return SIA_ERR_ARGUMENT;
+  int rc;
+  if ((rc = sg_train_buffers(sg, batch)) || (rc = sg_loss_buffer(sg, 1))) return rc;
+  const float rscale = sg->have_norm ? sg->norm.rscale : 1.0f;
+  if (hipMemsetAsync(sg->d_loss, 0, sizeof(double), sg->stream) != hipSuccess) return SIA_ERR_HIP;
+  launch_sg_set_inputs(x, r, share, has, batch, sg->tb, sg->stream);
+  SgTrainBuffers T = sg->tb;
+  T.loss = sg->d_loss;
+  launch_sg_forward_backward(sg->master, T, rscale, 1e-3f, sg->stream);
+  if (hipGetLastError() != hipSuccess) return SIA_ERR_HIP;
+  if (loss_host && hipMemcpyAsync(loss_host, sg->d_loss, sizeof(double), hipMemcpyDeviceToHost, sg->stream) != hipSuccess)
+    return SIA_ERR_HIP;
+  if (grad_host &&
+      hipMemcpyAsync(grad_host, sg->grad, sizeof(float) * SG_NPARAM, hipMemcpyDeviceToHost, sg->stream) != hipSuccess)
+    return SIA_ERR_HIP;
+  if (hipStreamSynchronize(sg->stream) != hipSuccess) return SIA_ERR_HIP;
+  return SIA_OK;
+}
+
+int sia_surrogate_save(const sia_surrogate *sg, void *blob, size_t *size) {
+  if (!sg || !size) return SIA_ERR_ARGUMENT;
+  const size_t need = sizeof(SgBlobHeader) + sizeof(SgNorm) + sizeof(float) * SG_NPARAM;
+  if (!blob) {
+    *size = need;
+    return SIA_OK;
+  }
+  if (*size < need) return SIA_ERR_ARGUMENT;
+  SgBlobHeader hdr{};
+  std::memcpy(hdr.magic, SG_MAGIC, 8);
+  hdr.nparam = SG_NPARAM;
+  hdr.nin = SG_NIN;
+  hdr.glen_n = sg->glen_n;
+  hdr.have_norm = sg->have_norm ? 1 : 0;
+  char *b = (char *)blob;
+  std::memcpy(b, &hdr, sizeof(hdr));
+  std::memcpy(b + sizeof(hdr), &sg->norm, sizeof(SgNorm));
+  if (hipMemcpy(b + sizeof(hdr) + sizeof(SgNorm), sg->master, sizeof(float) * SG_NPARAM, hipMemcpyDeviceToHost) !=
+      hipSuccess)
+    return SIA_ERR_HIP;
+  *size = need;
+  return SIA_OK;
+}
+
+int sia_surrogate_load(sia_surrogate *sg, const void *blob, size_t size) {
+  if (!sg || !blob) return SIA_ERR_ARGUMENT;
+  const size_t need = sizeof(SgBlobHeader) + sizeof(SgNorm) + sizeof(float) * SG_NPARAM;
+  if (size < need) return SIA_ERR_ARGUMENT;
+  SgBlobHeader hdr;
+  std::memcpy(&hdr, blob, sizeof(hdr));
+  if (std::memcmp(hdr.magic, SG_MAGIC, 8) != 0 || hdr.nparam != SG_NPARAM || hdr.nin != SG_NIN) {
+    return SIA_ERR_CONFIG;
+  }
+  const char *b = (const char *)blob;
+  std::memcpy(&sg->norm, b + sizeof(hdr), sizeof(SgNorm));
+  sg->have_norm = hdr.have_norm != 0;
+  sg->glen_n = hdr.glen_n;
+  return sia_surrogate_set_params(sg, (const float *)(b + sizeof(hdr) + sizeof(SgNorm)), SG_NPARAM);
+}
+
+int sia_surrogate_samples(const sia_surrogate *sg, const void **samples, int64_t *n) {
+  if (!sg || !samples || !n) return SIA_ERR_ARGUMENT;
+  *samples = sg->samples;
+  *n = sg->n;
+  return SIA_OK;
+}
+
+int sia_use_surrogate(sia_handle *h, sia_surrogate *sg) {
+  if (!h) return SIA_ERR_ARGUMENT;
+  if (!sg) {
+    h->sg = nullptr;
+    return SIA_OK;
+  }
+  const sia_params &p = h->params;
+  if (!sg->have_norm) return set_err(h, SIA_ERR_CONFIG, "the surrogate has no normalization (not trained)");
+  if (p.sia_flow_law != SIA_FLOW_GPBLD || p.enthalpy_converter != SIA_EC_DEFAULT) {
+    return set_err(h, SIA_ERR_CONFIG, "the surrogate models GPBLD with the default enthalpy converter");
+  }
+  if (p.sia_Glen_exponent != sg->glen_n) {
+    return set_err(h, SIA_ERR_CONFIG, "the surrogate was trained for another Glen exponent");
+  }
+  if (sg->device != h->device) return set_err(h, SIA_ERR_CONFIG, "the surrogate lives on another device");
+  int rc;
+  if ((rc = ensure_sg_buffers(h))) return rc;
+  h->sg = sg;
+  return SIA_OK;
+}
+
+int sia_surrogate_counts(sia_handle *h, int64_t counts[3]) {
+  if (!h || !counts) return SIA_ERR_ARGUMENT;
+  if (!h->sg_counts) return set_err(h, SIA_ERR_STATE, "no surrogate update has run");
+  int c[3];
+  HIP_TRY(h, hipMemcpyAsync(c, h->sg_counts, sizeof(c), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  for (int k = 0; k < 3; k++) counts[k] = c[k];
+  return SIA_OK;
+}
 
 }  // extern "C"

@@ -366,12 +366,15 @@ __global__ __launch_bounds__(256) void k_column(DevParams P, ColumnArgs A) {
   const int s0 = (blockIdx.x * WAVES_PER_BLOCK + wv) * COLS_PER_WAVE;
   const int s = s0 + lane;
 
-  // lane c < COLS_PER_WAVE: column s0 + c. c_ks: -3 no column, -2 zero
-  // thickness (delta = 0, SIAFD.cc:635-641), -1 thickness above Lz, else kBelowHeight
+  // lane c < COLS_PER_WAVE: column s0 + c. c_ks: -4 not selected (A.only),
+  // -3 no column, -2 zero thickness (delta = 0, SIAFD.cc:635-641), -1
+  // thickness above Lz, else kBelowHeight
   int c_ks = -3, c_qa = 0, c_qb = 0;
   double c_thk = 0.0, c_alpha = 0.0, c_c0 = 0.0, c_slope = 0.0;
   bool c_edge = false;
-  if (lane < COLS_PER_WAVE && s < nstag) {
+  if (lane < COLS_PER_WAVE && s < nstag && A.only && A.only[s] != ONLY_MARK) {
+    c_ks = -4;
+  } else if (lane < COLS_PER_WAVE && s < nstag) {
     const int q = s >> 1, o = s & 1;
     const int i = q / My, j = q - i * My;
     const int q2 = (o == 0) ? wrapi(i + 1, Mx) * My + j : i * My + wrapi(j + 1, My);
@@ -417,6 +420,10 @@ __global__ __launch_bounds__(256) void k_column(DevParams P, ColumnArgs A) {
   for (int c = 0; c < COLS_PER_WAVE; c++) {
     const int ks = lane_value(c_ks, c);
     if (ks == -3) break;
+    if (ks == -4) {
+      if (c + 1 < COLS_PER_WAVE) prefetch(c + 1);
+      continue;
+    }
     const double Ea0 = nEa, Eb0 = nEb, Aa0 = nAa, Ab0 = nAb;
     if (c + 1 < COLS_PER_WAVE) prefetch(c + 1);
     const long qa = lane_value(c_qa, c), qb = lane_value(c_qb, c);
@@ -477,7 +484,7 @@ __global__ __launch_bounds__(256) void k_column(DevParams P, ColumnArgs A) {
   }
 
   double dmax = 0.0;
-  if (lane < COLS_PER_WAVE && c_ks != -3) {
+  if (lane < COLS_PER_WAVE && c_ks > -3) {
     const double Dout = c_edge ? 0.0 : myD;
     A.Dstag[s] = myD;
     A.flux[s] = (c_ks < 0) ? 0.0 : -Dout * c_slope;
@@ -859,7 +866,7 @@ __global__ void k_mask_surface(DevParams P, const double *__restrict__ H_new, co
 
 // ---------------------------------------------------------------- launchers
 template <int LAW>
-static void launch_column_law(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st) {
+static void launch_column_law(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st, int extra) {
   const int nb = column_blocks(P);
   const size_t lds = sizeof(double) * (size_t)P.Mz;
   if (full) {
@@ -867,18 +874,18 @@ static void launch_column_law(const DevParams &P, const ColumnArgs &A, bool full
   } else {
     hipLaunchKernelGGL((k_column<LAW, false>), dim3(nb), dim3(256), lds, st, P, A);
   }
-  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, st, A.dmax_part, nb, A.dmax_bits);
+  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, st, A.dmax_part, nb + extra, A.dmax_bits);
 }
 
-void launch_column(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st) {
+void launch_column(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st, int extra) {
   switch (P.law) {
-  case LAW_ISOGLEN: launch_column_law<LAW_ISOGLEN>(P, A, full, st); break;
-  case LAW_PB: launch_column_law<LAW_PB>(P, A, full, st); break;
-  case LAW_GPBLD: launch_column_law<LAW_GPBLD>(P, A, full, st); break;
-  case LAW_HOOKE: launch_column_law<LAW_HOOKE>(P, A, full, st); break;
-  case LAW_ARR: launch_column_law<LAW_ARR>(P, A, full, st); break;
-  case LAW_ARRWARM: launch_column_law<LAW_ARRWARM>(P, A, full, st); break;
-  default: launch_column_law<LAW_GK>(P, A, full, st); break;
+  case LAW_ISOGLEN: launch_column_law<LAW_ISOGLEN>(P, A, full, st, extra); break;
+  case LAW_PB: launch_column_law<LAW_PB>(P, A, full, st, extra); break;
+  case LAW_GPBLD: launch_column_law<LAW_GPBLD>(P, A, full, st, extra); break;
+  case LAW_HOOKE: launch_column_law<LAW_HOOKE>(P, A, full, st, extra); break;
+  case LAW_ARR: launch_column_law<LAW_ARR>(P, A, full, st, extra); break;
+  case LAW_ARRWARM: launch_column_law<LAW_ARRWARM>(P, A, full, st, extra); break;
+  default: launch_column_law<LAW_GK>(P, A, full, st, extra); break;
   }
 }
 

@@ -19,7 +19,11 @@ struct ColumnArgs {
   unsigned long long *dmax_bits;
   double *dmax_part;         // [column_blocks(P)] per-block maxima of D
   int *err;
+  const unsigned char *only; // nullptr, or per staggered column: compute it only if only[s] == ONLY_MARK
 };
+
+// value of ColumnArgs::only that selects a staggered column (the surrogate's fallback status)
+constexpr unsigned char ONLY_MARK = 2;
 
 struct VelArgs {
   const double *z, *mask, *I0, *I1, *hx, *hy, *ub, *vb, *bmr;
@@ -46,7 +50,9 @@ void launch_theta_thk(const DevParams &P, const double *H, const double *b, cons
                       const double *C3, const double *C4, double *theta, double *thks, int *err,
                       hipStream_t st);
 int column_blocks(const DevParams &P);
-void launch_column(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st);
+// extra_partials: further per-block D_max partials that other kernels wrote
+// after A.dmax_part[column_blocks(P) - 1]; all are reduced into A.dmax_bits
+void launch_column(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st, int extra_partials = 0);
 void launch_velocity_uvw(const DevParams &P, const VelArgs &A, hipStream_t st);
 void launch_cfl2d(const DevParams &P, const double *mask, const double *ub, const double *vb,
                   unsigned long long *cfl_bits, hipStream_t st);

@@ -1,0 +1,778 @@
+// sia_surrogate.hip -- learned column surrogate of the SIA diffusivity for gfx950.
+//
+// Kernels (the model is described in sia_surrogate.hpp):
+//   k_sg_classify / k_sg_scan / k_sg_scatter  status of every staggered column
+//       (zero, surrogate, FP64 fallback) and the ordered list of surrogate columns
+//   k_sg_features<TARGETS>  one pass over the levels of each listed column: exact
+//       temperate sum, 32-cell moments of the cold levels, physics baseline and
+//       the MLP inputs; with TARGETS also the exact per-cell cold sums of the
+//       plain FP64 flow law (training targets)
+//   k_sg_mlp   fused BF16 MFMA inference (v_mfma_f32_16x16x32_bf16, weights in
+//       LDS, activations never leave registers) + D, flux, D_max of the column
+//   training: k_sg_gather, k_sg_gemm (generic BF16 MFMA GEMM), k_sg_bias_act,
+//       k_sg_loss, k_sg_act_bwd, k_sg_colsum, k_sg_adam
+#include "sia_surrogate.hpp"
+
+namespace pism_sia {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------- helpers
+// FP32 -> BF16 round to nearest even (finite inputs)
+__host__ __device__ inline unsigned short f2bf(float f) {
+  unsigned u;
+  __builtin_memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ inline float bf2f(unsigned short b) {
+  const unsigned u = (unsigned)b << 16;
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+
+// GELU, tanh form: 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))), with
+// tanh(u) = 1 - 2 / (exp(2u) + 1)
+__device__ inline float gelu_f(float x) {
+  const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
+  const float t = __expf(2.0f * u);
+  const float th = 1.0f - 2.0f * __builtin_amdgcn_rcpf(t + 1.0f);
+  return 0.5f * x * (1.0f + th);
+}
+__device__ inline float gelu_grad_f(float x) {
+  const float u = 0.7978845608028654f * (x + 0.044715f * x * x * x);
+  const float t = __expf(2.0f * u);
+  const float th = 1.0f - 2.0f * __builtin_amdgcn_rcpf(t + 1.0f);
+  const float du = 0.7978845608028654f * (1.0f + 3.0f * 0.044715f * x * x);
+  return 0.5f * (1.0f + th) + 0.5f * x * (1.0f - th * th) * du;
+}
+
+__device__ inline bf16x8 as_bf16x8(uint4 v) {
+  bf16x8 r;
+  __builtin_memcpy(&r, &v, 16);
+  return r;
+}
+
+template <int CTRL, int ROW_MASK, bool BOUND_ZERO>
+__device__ __forceinline__ double sg_dpp(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xf, BOUND_ZERO);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xf, BOUND_ZERO);
+  return __hiloint2double(hi, lo);
+}
+// inclusive prefix sum over the 64 lanes (all lanes active)
+__device__ __forceinline__ double sg_scan(double v) {
+  v += sg_dpp<0x111, 0xf, true>(v);
+  v += sg_dpp<0x112, 0xf, true>(v);
+  v += sg_dpp<0x114, 0xf, true>(v);
+  v += sg_dpp<0x118, 0xf, true>(v);
+  v += sg_dpp<0x142, 0xa, false>(v);
+  v += sg_dpp<0x143, 0xc, false>(v);
+  return v;
+}
+__device__ __forceinline__ double sg_lane(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+// sum of v over the 32 lanes of this lane's half wave
+__device__ __forceinline__ double half_sum(double v, int half) {
+  const double s = sg_scan(v);
+  const double a = sg_lane(s, 31), b = sg_lane(s, 63);
+  return half ? b - a : a;
+}
+
+// ---------------------------------------------------------------- classification
+constexpr int SG_CLS_BLOCK = 256;
+
+int sg_classify_blocks(const DevParams &P) { return (2 * P.Mx * P.My + SG_CLS_BLOCK - 1) / SG_CLS_BLOCK; }
+
+// status of staggered column s: SG_ZERO (thk == 0 or zero slope: D = 0,
+// SIAFD.cc:635-641), SG_FALLBACK (thk below the second level, or above Lz:
+// left to the FP64 kernel, which also reports the error), else SG_SURR.
+__global__ __launch_bounds__(SG_CLS_BLOCK) void k_sg_classify(DevParams P, SgColumnArgs C, SgListArgs L) {
+  const int s = blockIdx.x * SG_CLS_BLOCK + threadIdx.x;
+  const int nstag = 2 * P.Mx * P.My;
+  int st = -1;
+  if (s < nstag) {
+    const int q = s >> 1, o = s & 1;
+    const int i = q / P.My, j = q - i * P.My;
+    const int q2 = (o == 0) ? wrapi(i + 1, P.Mx) * P.My + j : i * P.My + wrapi(j + 1, P.My);
+    const double thk = 0.5 * (C.thks[q] + C.thks[q2]);
+    const double hxs = C.hx[s], hys = C.hy[s];
+    const double alpha = sqrt(hxs * hxs + hys * hys);
+    if (thk == 0.0 || alpha == 0.0) {
+      st = SG_ZERO;
+      const double slope = (o == 0) ? hxs : hys;
+      L.flux[s] = (thk == 0.0) ? 0.0 : -0.0 * slope;
+      L.Dstag[s] = 0.0;
+    } else if (thk < C.z[1] || thk > P.Lz + 1.0e-6) {
+      st = SG_FALLBACK;
+    } else {
+      st = SG_SURR;
+    }
+    L.status[s] = (unsigned char)st;
+  }
+  const int nz = __syncthreads_count(st == SG_ZERO);
+  const int ns = __syncthreads_count(st == SG_SURR);
+  const int nf = __syncthreads_count(st == SG_FALLBACK);
+  if (threadIdx.x == 0) {
+    L.block_count[3 * blockIdx.x + 0] = nz;
+    L.block_count[3 * blockIdx.x + 1] = ns;
+    L.block_count[3 * blockIdx.x + 2] = nf;
+  }
+}
+
+// exclusive scan of the per-block surrogate counts (one block of 1024); the
+// offsets overwrite block_count[3 b + 1]; totals go to counts[0..2], n_list
+__global__ __launch_bounds__(1024) void k_sg_scan(int nb, SgListArgs L) {
+  __shared__ long long part[1024];
+  __shared__ long long tot[3];
+  const int t = threadIdx.x;
+  const int per = (nb + 1023) / 1024;
+  const int b0 = t * per, b1 = min(nb, b0 + per);
+  long long s = 0, z = 0, f = 0;
+  for (int b = b0; b < b1; b++) {
+    z += L.block_count[3 * b + 0];
+    s += L.block_count[3 * b + 1];
+    f += L.block_count[3 * b + 2];
+  }
+  part[t] = s;
+  if (t < 3) tot[t] = 0;
+  __syncthreads();
+  atomicAdd((unsigned long long *)&tot[0], (unsigned long long)z);
+  atomicAdd((unsigned long long *)&tot[2], (unsigned long long)f);
+  // Hillis-Steele over the 1024 partial sums
+  for (int off = 1; off < 1024; off <<= 1) {
+    const long long v = (t >= off) ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  long long run = part[t] - s;  // exclusive start of this thread's chunk
+  for (int b = b0; b < b1; b++) {
+    const int c = L.block_count[3 * b + 1];
+    L.block_count[3 * b + 1] = (int)run;
+    run += c;
+  }
+  if (t == 1023) {
+    *L.n_list = (int)part[1023];
+    L.counts[1] = (int)part[1023];
+  }
+  __syncthreads();
+  if (t == 0) {
+    L.counts[0] = (int)tot[0];
+    L.counts[2] = (int)tot[2];
+  }
+}
+
+// ordered compaction of the SG_SURR columns
+__global__ __launch_bounds__(SG_CLS_BLOCK) void k_sg_scatter(DevParams P, SgListArgs L) {
+  __shared__ int wsum[SG_CLS_BLOCK / 64];
+  const int s = blockIdx.x * SG_CLS_BLOCK + threadIdx.x;
+  const int nstag = 2 * P.Mx * P.My;
+  const bool flag = (s < nstag) && L.status[s] == SG_SURR;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const unsigned long long m = __ballot(flag);
+  const int rank = __popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) wsum[wv] = __popcll(m);
+  __syncthreads();
+  int base = L.block_count[3 * blockIdx.x + 1];
+  for (int w = 0; w < wv; w++) base += wsum[w];
+  if (flag) L.list[base + rank] = s;
+}
+
+void launch_sg_list(const DevParams &P, const SgColumnArgs &C, const SgListArgs &L, hipStream_t st) {
+  const int nb = sg_classify_blocks(P);
+  hipLaunchKernelGGL(k_sg_classify, dim3(nb), dim3(SG_CLS_BLOCK), 0, st, P, C, L);
+  hipLaunchKernelGGL(k_sg_scan, dim3(1), dim3(1024), 0, st, nb, L);
+  hipLaunchKernelGGL(k_sg_scatter, dim3(nb), dim3(SG_CLS_BLOCK), 0, st, P, L);
+}
+
+// ---------------------------------------------------------------- features
+// p^n with the integer power path of the flow laws
+__device__ __forceinline__ double pow_n(const DevParams &P, double p) { return stress_pow(P, p) * p; }
+
+__device__ __forceinline__ int cell_of(double zk, double s32) {
+  const int c = (int)(zk * s32);
+  return c > SG_NCELL - 1 ? SG_NCELL - 1 : c;
+}
+
+// first level k in [0, ks] with cell_of(z_k) >= target (ks + 1 if none)
+__device__ __forceinline__ int first_level_of_cell(const double *zs, int ks, double s32, int target) {
+  int lo = 0, hi = ks + 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (cell_of(zs[mid], s32) >= target) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
+// One wavefront takes two listed columns per step, one per half wave; lane m
+// of a half owns zeta-cell m and walks its levels. The level weights are the
+// reference's: W_k = d_k (dz_k [k >= 1] + dz_{k+1} [k < ks]) / 2 + d_ks^2 / 2 [k == ks]
+// (SIAFD.cc:677-684 regrouped per delta_k), G_k = W_k p_k^n.
+template <bool TARGETS>
+__global__ __launch_bounds__(256) void k_sg_features(DevParams P, SgColumnArgs C, SgFeatureArgs F) {
+  extern __shared__ double zs[];
+  const int Mz = P.Mz;
+  for (int k = threadIdx.x; k < Mz; k += blockDim.x) zs[k] = C.z[k];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, half = lane >> 5, m = lane & 31;
+  const int n = *F.n_list;
+  const int stride = TARGETS ? F.stride : 1;
+  const int nsel = (n + stride - 1) / stride;  // selected columns: list slots 0, stride, 2 stride, ...
+  const long long base_sample = TARGETS ? *F.n_samples : 0;
+  const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int nw = gridDim.x * (blockDim.x >> 6);
+  const double T_ref = 263.15;
+  for (int t0 = 2 * gw; t0 < nsel; t0 += 2 * nw) {
+    const int t = t0 + half;
+    const bool live = t < nsel;
+    const int slot = live ? t * stride : 0;
+    const int s = live ? F.list[slot] : F.list[0];
+    const int q = s >> 1, o = s & 1;
+    const int i = q / P.My, j = q - i * P.My;
+    const int q2 = (o == 0) ? wrapi(i + 1, P.Mx) * P.My + j : i * P.My + wrapi(j + 1, P.My);
+    const double thk = 0.5 * (C.thks[q] + C.thks[q2]);
+    const double hxs = C.hx[s], hys = C.hy[s];
+    const double alpha = sqrt(hxs * hxs + hys * hys);
+    const double theta_local = 0.5 * (C.theta[q] + C.theta[q2]);
+    const double Cf = P.e * theta_local * 2.0 * stress_pow(P, alpha);
+    const int ks = kbelow(zs, Mz, P.Lz, thk);  // >= 1 for listed columns
+    const double s32 = (double)SG_NCELL / thk;
+    const int klo = first_level_of_cell(zs, ks, s32, m);
+    const int khi = first_level_of_cell(zs, ks, s32, m + 1);
+    int cnt = live ? khi - klo : 0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) cnt = max(cnt, __shfl_xor(cnt, off, 64));
+    const double *__restrict__ Ea = C.E + (long)q * Mz;
+    const double *__restrict__ Eb = C.E + (long)q2 * Mz;
+    double st = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0, qref = 0.0;
+    for (int it = 0; it < cnt; it++) {
+      const int k = klo + it;
+      if (live && k < khi) {
+        const double d = thk - zs[k];
+        const double p = ec_pressure(P, d);
+        const double dzk = (k >= 1) ? zs[k] - zs[k - 1] : 0.0;
+        const double dzk1 = (k < ks) ? zs[k + 1] - zs[k] : 0.0;
+        double W = 0.5 * d * (dzk + dzk1);
+        if (k == ks) W += 0.5 * d * d;
+        const double G = W * pow_n(P, p);
+        const double Ek = 0.5 * (Ea[k] + Eb[k]);
+        if (Ek < ec_enthalpy_cts(P, p)) {  // cold branch of GPBLD (flowlaws.cc:212-214)
+          const double Tpa = ec_pressure_adjusted_temperature(P, Ek, p);
+          const double x = Tpa - T_ref;
+          g0 += G;
+          g1 += G * x;
+          g2 += G * x * x;
+          if (TARGETS) qref += G * softness_pb(P, Tpa);
+        } else {  // temperate branch (flowlaws.cc:215-219)
+          double omega = ec_water_fraction(P, Ek, p);
+          if (P.wf_limit < omega) omega = P.wf_limit;
+          st += G * (P.A_T0 * (1.0 + P.wf_coeff * omega));
+        }
+      }
+    }
+    const double S_t = half_sum(st, half);
+    const bool has = g0 > 0.0;
+    double Qb = 0.0, feat = 2.0;
+    if (has) {
+      const double mx = g1 / g0;
+      const double Tbar = T_ref + mx;
+      const double var = fmax(g2 / g0 - mx * mx, 0.0);
+      const double Qa = (Tbar < P.T_crit) ? P.Q_cold : P.Q_warm;
+      const double kk = Qa / (P.R * Tbar * Tbar);
+      Qb = g0 * softness_pb(P, Tbar) * (1.0 + 0.5 * (kk * kk - 2.0 * kk / Tbar) * var);
+      feat = (Tbar - 250.0) / 15.0;
+    }
+    if (!TARGETS) {
+      if (live) {
+        F.F[(long)slot * SG_FSTRIDE + m] = (float)feat;
+        F.CQb[(long)slot * SG_NCELL + m] = (float)(Cf * Qb);
+        if (m == 0) {
+          F.F[(long)slot * SG_FSTRIDE + 32] = (float)(thk / 1000.0);
+          F.F[(long)slot * SG_FSTRIDE + 33] = (float)(log10(alpha) + 2.0);
+          F.CSt[slot] = Cf * S_t;
+        }
+      }
+    } else {
+      const double Dref = Cf * (S_t + half_sum(qref, half));
+      const unsigned long long hm = __ballot(has);
+      const unsigned hasbits = (unsigned)(half ? (hm >> 32) : (hm & 0xffffffffull));
+      const long long idx = base_sample + t;
+      if (live && idx < F.cap) {
+        SgSample &S = F.samples[idx];
+        S.f[m] = (float)feat;
+        S.r[m] = has ? (float)log(qref / Qb) : 0.0f;
+        S.share[m] = (Dref > 0.0) ? (float)(Cf * qref / Dref) : 0.0f;
+        if (m == 0) {
+          S.f[32] = (float)(thk / 1000.0);
+          S.f[33] = (float)(log10(alpha) + 2.0);
+          S.f[34] = 0.0f;
+          S.f[35] = 0.0f;
+          S.has = hasbits;
+          S.col = s;
+          S.dref = Dref;
+        }
+      }
+    }
+  }
+}
+
+void launch_sg_features(const DevParams &P, const SgColumnArgs &C, const SgFeatureArgs &F, bool targets,
+                        hipStream_t st) {
+  const int nb = 2048;  // persistent: 8192 wavefronts walk the list
+  const size_t lds = sizeof(double) * (size_t)P.Mz;
+  if (targets) {
+    hipLaunchKernelGGL(k_sg_features<true>, dim3(nb), dim3(256), lds, st, P, C, F);
+  } else {
+    hipLaunchKernelGGL(k_sg_features<false>, dim3(nb), dim3(256), lds, st, P, C, F);
+  }
+}
+
+// ---------------------------------------------------------------- inference
+// A-fragment index of (layer, output tile, k-chunk)
+__host__ __device__ constexpr int frag_index(int layer, int t, int c) {
+  return layer == 0 ? t * 2 + c : (layer < 4 ? 16 + (layer - 1) * 32 + t * 4 + c : 16 + 96 + t * 4 + c);
+}
+
+// input feature of k-slot (g, j) of a chunk: layer 0 reads its inputs from
+// memory in natural order; deeper layers take the previous layer's MFMA
+// accumulators, whose lane holds rows 4g..4g+3 of two consecutive 16-row tiles
+__host__ __device__ constexpr int slot_feature(int layer, int c, int g, int j) {
+  return layer == 0 ? 32 * c + 8 * g + j : 32 * c + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4));
+}
+
+// packs the FP32 master weights into the A-fragments (bf16) of the fused kernel
+__global__ void k_sg_pack(const float *__restrict__ w, unsigned short *__restrict__ frag) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;  // element of [SG_NFRAG][64][8]
+  if (e >= SG_NFRAG * 512) return;
+  const int f = e / 512, l = (e / 8) % 64, j = e % 8;
+  int layer, t, c;
+  if (f < 16) { layer = 0; t = f / 2; c = f % 2; }
+  else if (f < 112) { layer = 1 + (f - 16) / 32; t = ((f - 16) % 32) / 4; c = (f - 16) % 4; }
+  else { layer = 4; t = (f - 112) / 4; c = (f - 112) % 4; }
+  const int row = 16 * t + (l & 15);
+  const int col = slot_feature(layer, c, l >> 4, j);
+  float v = 0.0f;
+  if (layer == 0) v = (col < SG_NIN) ? w[SG_W1 + row * SG_NIN + col] : 0.0f;
+  else if (layer == 1) v = w[SG_W2 + row * SG_WIDTH + col];
+  else if (layer == 2) v = w[SG_W3 + row * SG_WIDTH + col];
+  else if (layer == 3) v = w[SG_W4 + row * SG_WIDTH + col];
+  else v = w[SG_W5 + row * SG_WIDTH + col];
+  frag[e] = f2bf(v);
+}
+
+void launch_sg_pack(const float *master, unsigned short *frag, hipStream_t st) {
+  hipLaunchKernelGGL(k_sg_pack, dim3((SG_NFRAG * 512 + 255) / 256), dim3(256), 0, st, master, frag);
+}
+
+constexpr int SG_MLP_BLOCKS = 256;  // one block per CU: the 120 KB of weights fill its LDS
+int sg_mlp_blocks() { return SG_MLP_BLOCKS; }
+
+__device__ __forceinline__ f32x4 mfma(const unsigned short *fr, int lane, bf16x8 b, f32x4 c) {
+  const uint4 a = *reinterpret_cast<const uint4 *>(fr + lane * 8);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(a), b, c, 0, 0, 0);
+}
+
+// bias + GELU of 8 accumulator tiles -> the 4 B-operand chunks of the next layer
+__device__ __forceinline__ void act_to_chunks(const f32x4 *acc, const float *bias, int g, bf16x8 *h) {
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    unsigned short u[8];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      u[i] = f2bf(gelu_f(acc[2 * c][i] + bias[16 * (2 * c) + 4 * g + i]));
+      u[4 + i] = f2bf(gelu_f(acc[2 * c + 1][i] + bias[16 * (2 * c + 1) + 4 * g + i]));
+    }
+    __builtin_memcpy(&h[c], u, 16);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_sg_mlp(DevParams P, SgMlpArgs A) {
+  extern __shared__ unsigned short wfrag[];  // [SG_NFRAG][64][8]
+  __shared__ float bias[4 * SG_WIDTH + SG_NOUT];
+  __shared__ double wmax[4];
+  __shared__ int wfall[4];
+  {
+    const uint4 *src = reinterpret_cast<const uint4 *>(A.frag);
+    uint4 *dst = reinterpret_cast<uint4 *>(wfrag);
+    for (int e = threadIdx.x; e < SG_NFRAG * 64; e += blockDim.x) dst[e] = src[e];
+    for (int e = threadIdx.x; e < SG_WIDTH; e += blockDim.x) {
+      bias[e] = A.bias[SG_B1 + e];
+      bias[SG_WIDTH + e] = A.bias[SG_B2 + e];
+      bias[2 * SG_WIDTH + e] = A.bias[SG_B3 + e];
+      bias[3 * SG_WIDTH + e] = A.bias[SG_B4 + e];
+    }
+    for (int e = threadIdx.x; e < SG_NOUT; e += blockDim.x) bias[4 * SG_WIDTH + e] = A.bias[SG_B5 + e];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int smp = lane & 15, g = lane >> 4;
+  const int n = *A.n_list;
+  const int ntile = (n + 15) / 16;
+  double dmax = 0.0;
+  int nfall = 0;
+  for (int tile = blockIdx.x * 4 + wv; tile < ntile; tile += gridDim.x * 4) {
+    const int slot = tile * 16 + smp;
+    const bool live = slot < n;
+    // layer-0 inputs: features 8g..8g+7 (chunk 0) and 32+8g..+7 (chunk 1)
+    bf16x8 x0, x1;
+    int ood = 0;
+    {
+      unsigned short u0[8], u1[8];
+      const float *fr = A.F + (long)(live ? slot : 0) * SG_FSTRIDE;
+#pragma unroll
+      for (int jj = 0; jj < 8; jj++) {
+        const int f = 8 * g + jj;
+        const float v = fr[f];
+        ood |= (v < A.norm.lo[f]) | (v > A.norm.hi[f]);
+        u0[jj] = f2bf((v - A.norm.mean[f]) * A.norm.inv_std[f]);
+        const int f1 = 32 + 8 * g + jj;
+        float v1 = 0.0f;
+        if (f1 < SG_NIN) {
+          const float r = fr[f1];
+          ood |= (r < A.norm.lo[f1]) | (r > A.norm.hi[f1]);
+          v1 = (r - A.norm.mean[f1]) * A.norm.inv_std[f1];
+        }
+        u1[jj] = f2bf(v1);
+      }
+      __builtin_memcpy(&x0, u0, 16);
+      __builtin_memcpy(&x1, u1, 16);
+    }
+    ood |= __shfl_xor(ood, 16, 64);
+    ood |= __shfl_xor(ood, 32, 64);
+    f32x4 acc[8];
+    bf16x8 h[4];
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+      f32x4 a = {0.0f, 0.0f, 0.0f, 0.0f};
+      a = mfma(wfrag + frag_index(0, t, 0) * 512, lane, x0, a);
+      a = mfma(wfrag + frag_index(0, t, 1) * 512, lane, x1, a);
+      acc[t] = a;
+    }
+    act_to_chunks(acc, bias, g, h);
+#pragma unroll
+    for (int layer = 1; layer < 4; layer++) {
+#pragma unroll
+      for (int t = 0; t < 8; t++) {
+        f32x4 a = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int c = 0; c < 4; c++) a = mfma(wfrag + frag_index(layer, t, c) * 512, lane, h[c], a);
+        acc[t] = a;
+      }
+      act_to_chunks(acc, bias + layer * SG_WIDTH, g, h);
+    }
+    f32x4 o0 = {0.0f, 0.0f, 0.0f, 0.0f}, o1 = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      o0 = mfma(wfrag + frag_index(4, 0, c) * 512, lane, h[c], o0);
+      o1 = mfma(wfrag + frag_index(4, 1, c) * 512, lane, h[c], o1);
+    }
+    // D = C S_t + sum_m C Qb_m exp(r_m); this lane holds cells 4g..4g+3, 16+4g..16+4g+3
+    float part = 0.0f;
+    if (live) {
+      const float *cq = A.CQb + (long)slot * SG_NCELL;
+      const float *bo = bias + 4 * SG_WIDTH;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        part += cq[4 * g + i] * __expf((o0[i] + bo[4 * g + i]) * A.norm.rscale);
+        part += cq[16 + 4 * g + i] * __expf((o1[i] + bo[16 + 4 * g + i]) * A.norm.rscale);
+      }
+    }
+    part += __shfl_xor(part, 16, 64);
+    part += __shfl_xor(part, 32, 64);
+    if (g == 0 && live) {
+      const int s = A.list[slot];
+      if (ood) {
+        A.status[s] = SG_FALLBACK;
+        nfall++;
+      } else {
+        const double D = A.CSt[slot] + (double)part;
+        const int q = s >> 1, o = s & 1;
+        const int i = q / P.My, j = q - i * P.My;
+        const double slope = (o == 0) ? A.hx[s] : A.hy[s];
+        const double Dout = (i >= P.Mx - 1 || j >= P.My - 1) ? 0.0 : D;  // SIAFD.cc:694-697
+        A.Dstag[s] = D;
+        A.flux[s] = -Dout * slope;
+        dmax = fmax(dmax, Dout);
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    dmax = fmax(dmax, __shfl_xor(dmax, off, 64));
+    nfall += __shfl_xor(nfall, off, 64);
+  }
+  if (lane == 0) {
+    wmax[wv] = dmax;
+    wfall[wv] = nfall;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mx = wmax[0];
+    int nf = wfall[0];
+    for (int w = 1; w < 4; w++) {
+      mx = fmax(mx, wmax[w]);
+      nf += wfall[w];
+    }
+    A.dmax_part[blockIdx.x] = mx;
+    if (nf) atomicAdd(&A.counts[2], nf);
+    if (nf) atomicSub(&A.counts[1], nf);
+  }
+}
+
+void launch_sg_mlp(const DevParams &P, const SgMlpArgs &A, hipStream_t st) {
+  const size_t lds = (size_t)SG_NFRAG * 512 * sizeof(unsigned short);
+  hipLaunchKernelGGL(k_sg_mlp, dim3(SG_MLP_BLOCKS), dim3(256), lds, st, P, A);
+}
+
+// ---------------------------------------------------------------- training
+__device__ inline unsigned long long sg_mix(unsigned long long x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+// random minibatch: normalized bf16 inputs, targets, shares, masks
+__global__ void k_sg_gather(const SgSample *__restrict__ samples, long long n, SgNorm norm, unsigned long long seed,
+                            long long step, SgTrainBuffers T) {
+  const int b = blockIdx.x;
+  const int e = threadIdx.x;  // 64 threads
+  const long long idx = (long long)(sg_mix(seed ^ sg_mix((unsigned long long)(step * 1000003ll + b))) % (unsigned long long)n);
+  const SgSample &S = samples[idx];
+  const float v = (e < SG_NIN) ? (S.f[e] - norm.mean[e]) * norm.inv_std[e] : 0.0f;
+  T.X0[(long)b * SG_KIN + e] = f2bf(v);
+  if (e < SG_NCELL) {
+    T.T[b * SG_NCELL + e] = S.r[e] / norm.rscale;
+    T.S[b * SG_NCELL + e] = S.share[e];
+    T.Hm[b * SG_NCELL + e] = ((S.has >> e) & 1u) ? 1.0f : 0.0f;
+  }
+}
+
+void launch_sg_gather(const SgSample *samples, long long n, const SgNorm &norm, unsigned long long seed, long long step,
+                      const SgTrainBuffers &T, hipStream_t st) {
+  hipLaunchKernelGGL(k_sg_gather, dim3(T.B), dim3(64), 0, st, samples, n, norm, seed, step, T);
+}
+
+// explicit inputs (tests): Xn [B][64] f32 already normalized; r in output units
+__global__ void k_sg_set_inputs(const float *Xn, const float *r, const float *share, const float *has, int B,
+                                SgTrainBuffers T) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < B * SG_KIN) T.X0[e] = f2bf(Xn[e]);
+  if (e < B * SG_NCELL) {
+    T.T[e] = r[e];
+    T.S[e] = share[e];
+    T.Hm[e] = has[e];
+  }
+}
+
+void launch_sg_set_inputs(const float *Xn, const float *r, const float *share, const float *has, int B,
+                          const SgTrainBuffers &T, hipStream_t st) {
+  hipLaunchKernelGGL(k_sg_set_inputs, dim3((B * SG_KIN + 255) / 256), dim3(256), 0, st, Xn, r, share, has, B, T);
+}
+
+// bf16 copies of the weights, W1 padded to 64 inputs: [W1 128x64][W2][W3][W4][W5 32x128]
+constexpr int SG_WBF_W1 = 0, SG_WBF_W2 = SG_WIDTH * SG_KIN, SG_WBF_W3 = SG_WBF_W2 + SG_WIDTH * SG_WIDTH,
+              SG_WBF_W4 = SG_WBF_W3 + SG_WIDTH * SG_WIDTH, SG_WBF_W5 = SG_WBF_W4 + SG_WIDTH * SG_WIDTH,
+              SG_WBF_N = SG_WBF_W5 + SG_NOUT * SG_WIDTH;
+
+__global__ void k_sg_weights_bf16(const float *__restrict__ w, unsigned short *__restrict__ wb) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= SG_WBF_N) return;
+  float v;
+  if (e < SG_WBF_W2) {
+    const int r = e / SG_KIN, c = e % SG_KIN;
+    v = (c < SG_NIN) ? w[SG_W1 + r * SG_NIN + c] : 0.0f;
+  } else if (e < SG_WBF_W3) v = w[SG_W2 + (e - SG_WBF_W2)];
+  else if (e < SG_WBF_W4) v = w[SG_W3 + (e - SG_WBF_W3)];
+  else if (e < SG_WBF_W5) v = w[SG_W4 + (e - SG_WBF_W4)];
+  else v = w[SG_W5 + (e - SG_WBF_W5)];
+  wb[e] = f2bf(v);
+}
+
+void launch_sg_weights_bf16(const float *master, unsigned short *Wbf, hipStream_t st) {
+  hipLaunchKernelGGL(k_sg_weights_bf16, dim3((SG_WBF_N + 255) / 256), dim3(256), 0, st, master, Wbf);
+}
+
+// C[M][N] (f32, row-major, ldc = N) = sum_k A(m,k) B(k,n), with bf16
+// A(m,k) = Ap[m sam + k sak], B(k,n) = Bp[k sbk + n sbn]. M, N multiples of 16,
+// K a multiple of 32. One wavefront per 16x16 tile.
+__global__ __launch_bounds__(256) void k_sg_gemm(int M, int N, int K, const unsigned short *__restrict__ Ap, long sam,
+                                                 long sak, const unsigned short *__restrict__ Bp, long sbk, long sbn,
+                                                 float *__restrict__ Cm) {
+  const int lane = threadIdx.x & 63;
+  const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int tn = N / 16;
+  if (tile >= (M / 16) * tn) return;
+  const int m0 = (tile / tn) * 16, n0 = (tile % tn) * 16;
+  const int r = lane & 15, g = lane >> 4;
+  f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    unsigned short ua[8], ub[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const long k = k0 + 8 * g + j;
+      ua[j] = Ap[(long)(m0 + r) * sam + k * sak];
+      ub[j] = Bp[k * sbk + (long)(n0 + r) * sbn];
+    }
+    bf16x8 a, b;
+    __builtin_memcpy(&a, ua, 16);
+    __builtin_memcpy(&b, ub, 16);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) Cm[(long)(m0 + 4 * g + i) * N + n0 + r] = acc[i];
+}
+
+static void gemm(int M, int N, int K, const unsigned short *A, long sam, long sak, const unsigned short *B, long sbk,
+                 long sbn, float *C, hipStream_t st) {
+  const int tiles = (M / 16) * (N / 16);
+  hipLaunchKernelGGL(k_sg_gemm, dim3((tiles + 3) / 4), dim3(256), 0, st, M, N, K, A, sam, sak, B, sbk, sbn, C);
+}
+
+// Z += bias; A = bf16(gelu(Z)) (hidden) or nothing (output layer)
+__global__ void k_sg_bias_act(float *Z, const float *bias, int B, int N, unsigned short *Abf) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= B * N) return;
+  const float z = Z[e] + bias[e % N];
+  Z[e] = z;
+  if (Abf) Abf[e] = f2bf(gelu_f(z));
+}
+
+// loss = mean_b [ (sum_m s (y - t))^2 + lambda sum_m h (y - t)^2 ] rscale^2, y = Z5 (output units),
+// t = target in output units; dZ5 and its bf16 copy
+__global__ void k_sg_loss(const float *Z5, SgTrainBuffers T, float rscale, float lambda, double *loss_acc) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= T.B) return;
+  float e = 0.0f, l2 = 0.0f;
+  for (int m = 0; m < SG_NCELL; m++) {
+    const float d = (Z5[b * SG_NOUT + m] - T.T[b * SG_NCELL + m]) * rscale;
+    e += T.S[b * SG_NCELL + m] * d;
+    l2 += T.Hm[b * SG_NCELL + m] * d * d;
+  }
+  const float inv = 1.0f / (float)T.B;
+  for (int m = 0; m < SG_NCELL; m++) {
+    const float d = (Z5[b * SG_NOUT + m] - T.T[b * SG_NCELL + m]) * rscale;
+    const float gr = 2.0f * inv * rscale * (e * T.S[b * SG_NCELL + m] + lambda * T.Hm[b * SG_NCELL + m] * d);
+    T.dZ[b * SG_NOUT + m] = gr;
+    T.dZbf[b * SG_NOUT + m] = f2bf(gr);
+  }
+  atomicAdd(loss_acc, (double)((e * e + lambda * l2) * inv));
+}
+
+// dZ = dA * gelu'(Z) (f32 and bf16)
+__global__ void k_sg_act_bwd(const float *dA, const float *Z, int n, float *dZ, unsigned short *dZbf) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const float v = dA[e] * gelu_grad_f(Z[e]);
+  dZ[e] = v;
+  dZbf[e] = f2bf(v);
+}
+
+// db[n] = sum_b dZ[b][n] (fixed order)
+__global__ void k_sg_colsum(const float *dZ, int B, int N, float *db) {
+  const int nn = blockIdx.x * blockDim.x + threadIdx.x;
+  if (nn >= N) return;
+  float s = 0.0f;
+  for (int b = 0; b < B; b++) s += dZ[(long)b * N + nn];
+  db[nn] = s;
+}
+
+// copies a [128][64] padded gradient of W1 into the [128][34] master layout
+__global__ void k_sg_unpad_w1(const float *dWpad, float *g) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= SG_WIDTH * SG_NIN) return;
+  g[SG_W1 + e] = dWpad[(e / SG_NIN) * SG_KIN + (e % SG_NIN)];
+}
+
+static inline unsigned nblk(long n) { return (unsigned)((n + 255) / 256); }
+
+void launch_sg_forward_backward(const float *master, const SgTrainBuffers &T, float rscale, float lambda,
+                                hipStream_t st) {
+  const int B = T.B;
+  const int Kin[SG_NLAYER] = {SG_KIN, SG_WIDTH, SG_WIDTH, SG_WIDTH, SG_WIDTH};
+  const int Nout[SG_NLAYER] = {SG_WIDTH, SG_WIDTH, SG_WIDTH, SG_WIDTH, SG_NOUT};
+  const int woff[SG_NLAYER] = {SG_WBF_W1, SG_WBF_W2, SG_WBF_W3, SG_WBF_W4, SG_WBF_W5};
+  const int boff[SG_NLAYER] = {SG_B1, SG_B2, SG_B3, SG_B4, SG_B5};
+  const int gw[SG_NLAYER] = {SG_W1, SG_W2, SG_W3, SG_W4, SG_W5};
+  launch_sg_weights_bf16(master, T.Wbf, st);
+  // forward: Z_l = A_l W_l^T + b_l
+  for (int l = 0; l < SG_NLAYER; l++) {
+    const unsigned short *Ain = T.Abf[l];
+    gemm(B, Nout[l], Kin[l], Ain, Kin[l], 1, T.Wbf + woff[l], 1, Kin[l], T.Z[l], st);
+    hipLaunchKernelGGL(k_sg_bias_act, dim3(nblk((long)B * Nout[l])), dim3(256), 0, st, T.Z[l], master + boff[l], B,
+                       Nout[l], l + 1 < SG_NLAYER ? T.Abf[l + 1] : nullptr);
+  }
+  hipLaunchKernelGGL(k_sg_loss, dim3(nblk(B)), dim3(256), 0, st, T.Z[4], T, rscale, lambda, T.loss);
+  // backward
+  for (int l = SG_NLAYER - 1; l >= 0; l--) {
+    const int K = Kin[l], N = Nout[l];
+    // dW_l[N][K] = dZ^T A_l  (k-dimension = batch)
+    float *dW = (l == 0) ? T.dWpad : T.grad + gw[l];
+    gemm(N, K, B, T.dZbf, 1, N, T.Abf[l], K, 1, dW, st);
+    if (l == 0) hipLaunchKernelGGL(k_sg_unpad_w1, dim3(nblk(SG_WIDTH * SG_NIN)), dim3(256), 0, st, T.dWpad, T.grad);
+    hipLaunchKernelGGL(k_sg_colsum, dim3(1), dim3(256), 0, st, T.dZ, B, N, T.grad + boff[l]);
+    if (l > 0) {
+      // dA_{l}[B][K] = dZ W_l ; dZ_{l-1} = dA * gelu'(Z_{l-1})
+      gemm(B, K, N, T.dZbf, N, 1, T.Wbf + woff[l], K, 1, T.dA, st);
+      hipLaunchKernelGGL(k_sg_act_bwd, dim3(nblk((long)B * K)), dim3(256), 0, st, T.dA, T.Z[l - 1], B * K, T.dZ,
+                         T.dZbf);
+    }
+  }
+}
+
+__global__ void k_sg_adam(float *w, float *m, float *v, const float *g, float lr, float b1, float b2, float eps,
+                          float c1, float c2) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= SG_NPARAM) return;
+  const float gg = g[e];
+  const float mm = b1 * m[e] + (1.0f - b1) * gg;
+  const float vv = b2 * v[e] + (1.0f - b2) * gg * gg;
+  m[e] = mm;
+  v[e] = vv;
+  w[e] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+}
+
+void launch_sg_adam(float *master, float *m, float *v, const float *grad, float lr, float b1, float b2, float eps,
+                    long long t, hipStream_t st) {
+  const float c1 = 1.0f - powf(b1, (float)t), c2 = 1.0f - powf(b2, (float)t);
+  hipLaunchKernelGGL(k_sg_adam, dim3(nblk(SG_NPARAM)), dim3(256), 0, st, master, m, v, grad, lr, b1, b2, eps, c1, c2);
+}
+
+// per-wavefront partial statistics of the features (lane f < SG_NIN takes
+// feature f of every sample of the wave's share) and of max |r| (lanes 32..63)
+__global__ void k_sg_norm_stats(const SgSample *__restrict__ samples, long long n, double *part) {
+  const int lane = threadIdx.x & 63;
+  const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int nw = gridDim.x * (blockDim.x >> 6);
+  double s = 0.0, s2 = 0.0, lo = 1e300, hi = -1e300, rmax = 0.0;
+  for (long long i = gw; i < n; i += nw) {
+    const SgSample &S = samples[i];
+    if (lane < SG_NIN) {
+      const double v = S.f[lane];
+      s += v;
+      s2 += v * v;
+      lo = fmin(lo, v);
+      hi = fmax(hi, v);
+    }
+    if (lane >= 32) rmax = fmax(rmax, fabs((double)S.r[lane - 32]));
+  }
+  if (lane < SG_NIN) {
+    double *o = part + ((long)gw * SG_NIN + lane) * 4;
+    o[0] = s;
+    o[1] = s2;
+    o[2] = lo;
+    o[3] = hi;
+  }
+  for (int off = 16; off > 0; off >>= 1) rmax = fmax(rmax, __shfl_xor(rmax, off, 64));
+  if (lane == 32) part[(long)nw * SG_NIN * 4 + gw] = rmax;
+}
+
+void launch_sg_norm_stats(const SgSample *samples, long long n, double *partial, hipStream_t st) {
+  hipLaunchKernelGGL(k_sg_norm_stats, dim3(SG_STAT_WAVES / 4), dim3(256), 0, st, samples, n, partial);
+}
+
+}  // namespace pism_sia

@@ -62,7 +62,29 @@ SIGNATURES = {
     "sia_event_name": (C.c_char_p, [C.c_int]),
     "sia_workload_generate": (C.c_int, [C.POINTER(SiaWorkloadSpec), C.POINTER(T.SiaGrid),
                                         C.POINTER(T.SiaParams), _vp, _vp, _vp, _vp, _vp, _vp]),
+    # learned surrogate (include/sia_surrogate.h)
+    "sia_surrogate_create": (C.c_int, [C.c_uint64, C.c_int64, _PP]),
+    "sia_surrogate_destroy": (C.c_int, [_vp]),
+    "sia_surrogate_set_stream": (C.c_int, [_vp, _vp]),
+    "sia_surrogate_collect": (C.c_int, [_vp, _vp, C.POINTER(T.SiaInputs), C.c_int, C.POINTER(C.c_int64)]),
+    "sia_surrogate_fit_normalization": (C.c_int, [_vp]),
+    "sia_surrogate_train": (C.c_int, [_vp, C.c_int64, C.c_int, C.c_double, C.c_double, C.c_uint64,
+                                      C.POINTER(C.c_double)]),
+    "sia_surrogate_get_params": (C.c_int, [_vp, C.POINTER(C.c_float), C.c_size_t]),
+    "sia_surrogate_set_params": (C.c_int, [_vp, C.POINTER(C.c_float), C.c_size_t]),
+    "sia_surrogate_grad": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_float)]),
+    "sia_surrogate_save": (C.c_int, [_vp, _vp, C.POINTER(C.c_size_t)]),
+    "sia_surrogate_load": (C.c_int, [_vp, _vp, C.c_size_t]),
+    "sia_surrogate_samples": (C.c_int, [_vp, _PP, C.POINTER(C.c_int64)]),
+    "sia_use_surrogate": (C.c_int, [_vp, _vp]),
+    "sia_surrogate_counts": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
 }
+
+SG_NPARAM = 58144
+SG_KIN = 64
+SG_SAMPLE_DTYPE = np.dtype([("f", "<f4", 36), ("r", "<f4", 32), ("share", "<f4", 32), ("has", "<u4"),
+                            ("col", "<i4"), ("dref", "<f8")])  # SgSample, 416 bytes
 
 _lib = None
 
@@ -250,6 +272,110 @@ class SIAFD:
     def step(self, geom, E, fast=False, t_end=0.0, t_dev=None, dt_dev=None):
         self._check(lib().sia_step(self.h, C.byref(geom), ptr(E), 1 if fast else 0, t_end,
                                    ptr(t_dev), ptr(dt_dev)))
+
+    def use_surrogate(self, sg):
+        """Fast updates use the learned surrogate `sg` (None: FP64 again)."""
+        self._check(lib().sia_use_surrogate(self.h, sg.h if sg is not None else None))
+        self._sg = sg
+
+    def surrogate_counts(self):
+        """(zero, surrogate, fallback) staggered columns of the last surrogate update."""
+        c = (C.c_int64 * 3)()
+        self._check(lib().sia_surrogate_counts(self.h, c))
+        return tuple(c)
+
+
+class Surrogate:
+    """The learned column surrogate (include/sia_surrogate.h): training samples
+    from the FP64 path, training on the GPU, save/load, gradient hook."""
+
+    def __init__(self, seed=1, capacity=0, stream=None):
+        h = C.c_void_p()
+        rc = lib().sia_surrogate_create(seed, capacity, C.byref(h))
+        if rc != 0:
+            raise SIAError(rc, "sia_surrogate_create")
+        self.h = h
+        if stream is not None:
+            self._check(lib().sia_surrogate_set_stream(self.h, C.c_void_p(stream)))
+
+    def _check(self, rc, what=""):
+        if rc != 0:
+            raise SIAError(rc, what)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sia_surrogate_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def collect(self, sia, H, b, h, mask, E, stride=1):
+        inp = T.SiaInputs(ptr(H), ptr(b), ptr(h), ptr(mask), ptr(E), None, None, None, None, 0)
+        n = C.c_int64()
+        rc = lib().sia_surrogate_collect(self.h, sia.h, C.byref(inp), stride, C.byref(n))
+        if rc != 0:
+            raise SIAError(rc, lib().sia_last_error(sia.h).decode())
+        return n.value
+
+    def fit_normalization(self):
+        self._check(lib().sia_surrogate_fit_normalization(self.h), "fit_normalization")
+
+    def train(self, steps, batch=1024, lr_start=2e-3, lr_end=1e-5, seed=7):
+        loss = np.zeros(int(steps))
+        self._check(lib().sia_surrogate_train(self.h, int(steps), int(batch), lr_start, lr_end, seed,
+                                              loss.ctypes.data_as(C.POINTER(C.c_double))), "train")
+        return loss
+
+    def get_params(self):
+        w = np.zeros(SG_NPARAM, dtype=np.float32)
+        self._check(lib().sia_surrogate_get_params(self.h, w.ctypes.data_as(C.POINTER(C.c_float)), SG_NPARAM))
+        return w
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == SG_NPARAM
+        self._check(lib().sia_surrogate_set_params(self.h, w.ctypes.data_as(C.POINTER(C.c_float)), SG_NPARAM))
+
+    def grad(self, x, r, share, has):
+        """Loss and FP32 gradient of one batch; arrays are host float32:
+        x [B][64] normalized inputs, r/share/has [B][32] (targets in output units)."""
+        torch = _torch()
+        B = x.shape[0]
+        dev = [torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to("cuda") for a in (x, r, share, has)]
+        loss = C.c_double()
+        g = np.zeros(SG_NPARAM, dtype=np.float32)
+        torch.cuda.synchronize()
+        self._check(lib().sia_surrogate_grad(self.h, B, *[ptr(t) for t in dev], C.byref(loss),
+                                             g.ctypes.data_as(C.POINTER(C.c_float))), "grad")
+        return loss.value, g
+
+    def save(self):
+        n = C.c_size_t()
+        self._check(lib().sia_surrogate_save(self.h, None, C.byref(n)), "save")
+        buf = (C.c_char * n.value)()
+        self._check(lib().sia_surrogate_save(self.h, buf, C.byref(n)), "save")
+        return bytes(buf)
+
+    def load(self, blob):
+        buf = (C.c_char * len(blob)).from_buffer_copy(blob)
+        self._check(lib().sia_surrogate_load(self.h, buf, len(blob)), "load")
+
+    def samples(self):
+        """The training samples as a numpy structured array (copied to the host)."""
+        p, n = C.c_void_p(), C.c_int64()
+        self._check(lib().sia_surrogate_samples(self.h, C.byref(p), C.byref(n)), "samples")
+        out = np.zeros(n.value, dtype=SG_SAMPLE_DTYPE)
+        if n.value:
+            _torch().cuda.synchronize()
+            rc = _hip().hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(p.value),
+                                  C.c_size_t(out.nbytes), 2)
+            if rc != 0:
+                raise RuntimeError("hipMemcpy failed: %d" % rc)
+        return out
 
 
 def geometry(H, h, mask, b, smb=None, bmr=None):

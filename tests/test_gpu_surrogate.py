@@ -1,0 +1,238 @@
+"""GPU tests of the learned column surrogate (include/sia_surrogate.h), all
+through the C ABI, checked against the CPU oracle (oracle/surrogate_oracle.py)
+and against the FP64 HIP path.
+
+Tolerances (stated per check):
+* training samples: features and shares 1e-5, target log-corrections 1e-6
+  absolute (FP32 storage of FP64 values); the exact per-level D of each sample
+  equals the FP64 kernel's D to 1e-12 relative;
+* gradient of one batch vs the oracle with the same BF16 rounding: 2e-2
+  relative in the L2 norm of every parameter tensor (FP32 accumulation order
+  moves a few BF16 activations by one unit);
+* fused inference vs the oracle's forward pass on the same inputs and weights:
+  D to 1e-4 relative;
+* accuracy of a trained model vs the FP64 path on a held-out window: max
+  relative error of D below 1e-3, and the network must beat the physics
+  baseline alone (the ablation);
+* out-of-distribution columns fall back to the FP64 kernel: their D is
+  bitwise the FP64 one.
+"""
+import struct
+
+import numpy as np
+import pytest
+
+import surrogate_oracle as SO
+import workloads as W
+from parity_util import say
+
+pytestmark = pytest.mark.gpu
+
+WINDOWS_TRAIN = [(600, 700, 1100, 1200), (300, 400, 700, 800), (900, 1000, 1500, 1600), (200, 300, 2000, 2100),
+                 (1100, 1200, 900, 1000), (700, 800, 2300, 2400), (450, 550, 1300, 1400), (1250, 1350, 1700, 1800)]
+WINDOW_TEST = (800, 880, 1800, 1880)
+
+
+@pytest.fixture(scope="module")
+def S():
+    import pism_sia
+    import torch
+    assert torch.cuda.is_available()
+    return pism_sia
+
+
+def window_fields(S, win, **kw):
+    w = W.greenland_like(*win, **kw)
+    return w, {k: S.to_device(v) for k, v in w.fields.items()}
+
+
+def plain_fast(S, w, dev):
+    sia = S.SIAFD(w.grid, w.params)
+    sia.update(dev["H"], dev["b"], dev["h"], dev["mask"], dev["E"], fast=True)
+    out = dict(D=sia.diffusivity_staggered(), flux=sia.diffusive_flux(), D_max=sia.max_diffusivity())
+    out["theta"], out["thks"] = sia.theta_thksmooth()
+    out["hx"], out["hy"] = sia.surface_gradient()
+    sia.close()
+    return out
+
+
+def eligible(w, st):
+    z = np.array([w.grid.z[k] for k in range(w.grid.Mz)])
+    return (st["thk"] > 0) & (st["alpha"] > 0) & (st["thk"] >= z[1]) & (st["thk"] <= z[-1] + 1e-6)
+
+
+def test_collected_samples_match_oracle(S):
+    say("[gpu] surrogate samples vs oracle")
+    w, dev = window_fields(S, (700, 740, 1180, 1230))
+    sg = S.Surrogate(seed=1, capacity=20000)
+    sia = S.SIAFD(w.grid, w.params)
+    n = sg.collect(sia, dev["H"], dev["b"], dev["h"], dev["mask"], dev["E"], stride=1)
+    smp = sg.samples()
+    assert n == len(smp) > 1000
+    ref = plain_fast(S, w, dev)
+    st = SO.staggered_inputs(w.grid, w.fields, ref["thks"], ref["theta"], ref["hx"], ref["hy"])
+    sel = np.nonzero(eligible(w, st))[0]
+    assert np.array_equal(smp["col"], sel)  # the ordered list of eligible staggered columns
+    z = np.array([w.grid.z[k] for k in range(w.grid.Mz)])
+    c = SO.columns(w.params, z, st["thk"][sel], st["alpha"][sel], st["theta"][sel], st["E"][sel])
+    np.testing.assert_allclose(smp["f"][:, :34], c["F"], rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(smp["r"], c["r"].astype(np.float32), atol=1e-6)
+    np.testing.assert_allclose(smp["share"], c["share"].astype(np.float32), atol=1e-5)
+    bits = (c["has"] * (1 << np.arange(32))).sum(axis=1).astype(np.uint32)
+    assert np.array_equal(smp["has"], bits)
+    D = ref["D"][sel]
+    assert np.max(np.abs(smp["dref"] - D) / D) < 1e-12
+    sia.close()
+    sg.close()
+
+
+def test_gradient_matches_oracle(S):
+    say("[gpu] surrogate gradient vs oracle")
+    rng = np.random.default_rng(11)
+    sg = S.Surrogate(seed=3)
+    w = rng.normal(0, 0.08, SO.NPARAM).astype(np.float32)
+    sg.set_params(w)
+    B = 64
+    x = np.zeros((B, SO.KIN), dtype=np.float32)
+    x[:, :SO.NIN] = rng.normal(0, 1, (B, SO.NIN))
+    r = rng.normal(0, 0.4, (B, 32)).astype(np.float32)
+    share = rng.uniform(0, 0.06, (B, 32)).astype(np.float32)
+    has = (rng.uniform(size=(B, 32)) < 0.75).astype(np.float32)
+    loss, g = sg.grad(x, r, share, has)
+    lo, go = SO.loss_grad(w.astype(np.float64), x, r, share, has, rscale=1.0, lam=1e-3, rounding=True)
+    assert abs(loss - lo) <= 1e-3 * abs(lo)
+    for name, off, shape in SO.LAYOUT:
+        k = int(np.prod(shape))
+        a, b = g[off:off + k], go[off:off + k]
+        err = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+        say("  %s rel L2 err %.2e" % (name, err))
+        assert err < 2e-2, name
+    sg.close()
+
+
+@pytest.fixture(scope="module")
+def trained(S):
+    say("[gpu] collect + train the surrogate")
+    sg = S.Surrogate(seed=5, capacity=400000)
+    for win in WINDOWS_TRAIN:
+        w, dev = window_fields(S, win)
+        sia = S.SIAFD(w.grid, w.params)
+        n = sg.collect(sia, dev["H"], dev["b"], dev["h"], dev["mask"], dev["E"], stride=1)
+        sia.close()
+    say("  samples", n)
+    sg.fit_normalization()
+    loss = sg.train(3000, batch=512, lr_start=2e-3, lr_end=1e-5, seed=9)
+    say("  loss first %.3e last %.3e" % (loss[:100].mean(), loss[-100:].mean()))
+    return sg, loss
+
+
+def test_training_reduces_loss(trained):
+    _, loss = trained
+    assert np.all(np.isfinite(loss))
+    assert loss[-200:].mean() < 0.2 * loss[:200].mean()
+
+
+def parse_norm(blob):
+    hdr = struct.calcsize("8sii d ii")
+    vals = np.frombuffer(blob, dtype=np.float32, count=4 * 34 + 1, offset=hdr)
+    mean, inv_std, lo, hi = (vals[34 * i:34 * (i + 1)] for i in range(4))
+    w = np.frombuffer(blob, dtype=np.float32, offset=hdr + 4 * (4 * 34 + 1))
+    return dict(mean=mean, inv_std=inv_std, lo=lo, hi=hi, rscale=float(vals[-1])), w
+
+
+def surrogate_fast(S, sg, w, dev):
+    sia = S.SIAFD(w.grid, w.params)
+    sia.use_surrogate(sg)
+    sia.update(dev["H"], dev["b"], dev["h"], dev["mask"], dev["E"], fast=True)
+    out = dict(D=sia.diffusivity_staggered(), flux=sia.diffusive_flux(), D_max=sia.max_diffusivity(),
+               counts=sia.surrogate_counts())
+    sia.close()
+    return out
+
+
+def test_inference_matches_oracle(S, trained):
+    say("[gpu] fused inference vs oracle forward")
+    sg, _ = trained
+    w, dev = window_fields(S, WINDOW_TEST)
+    ref = plain_fast(S, w, dev)
+    sur = surrogate_fast(S, sg, w, dev)
+    norm, wts = parse_norm(sg.save())
+    st = SO.staggered_inputs(w.grid, w.fields, ref["thks"], ref["theta"], ref["hx"], ref["hy"])
+    sel = np.nonzero(eligible(w, st))[0]
+    z = np.array([w.grid.z[k] for k in range(w.grid.Mz)])
+    c = SO.columns(w.params, z, st["thk"][sel], st["alpha"][sel], st["theta"][sel], st["E"][sel], targets=False)
+    F = c["F"].astype(np.float32)
+    inside = np.all((F >= norm["lo"]) & (F <= norm["hi"]), axis=1)
+    X = np.zeros((len(sel), SO.KIN), dtype=np.float32)
+    X[:, :34] = (F - norm["mean"]) * norm["inv_std"]
+    out, _ = SO.forward(wts.astype(np.float64), X, rounding=True)
+    Dor = c["CSt"] + (c["CQb"].astype(np.float64) * np.exp(out * norm["rscale"])).sum(axis=1)
+    Ds = sur["D"][sel]
+    rel = np.abs(Ds[inside] - Dor[inside]) / Dor[inside]
+    say("  columns %d (in range %d), max rel diff to oracle %.2e" % (len(sel), inside.sum(), rel.max()))
+    assert rel.max() < 1e-4
+    # columns outside the training range were computed by the FP64 kernel
+    assert np.array_equal(Ds[~inside], ref["D"][sel][~inside])
+
+
+def test_accuracy_and_ablation(S, trained):
+    say("[gpu] surrogate accuracy vs FP64 on a held-out window, and the ablation")
+    sg, _ = trained
+    w, dev = window_fields(S, WINDOW_TEST)
+    ref = plain_fast(S, w, dev)
+    sur = surrogate_fast(S, sg, w, dev)
+    zero, nsur, nfall = sur["counts"]
+    assert nsur > 0.8 * (nsur + nfall)
+    D0 = ref["D"]
+    pos = D0 > 0
+    rel = np.abs(sur["D"][pos] - D0[pos]) / D0[pos]
+    say("  counts", sur["counts"], "rel err median %.2e p99 %.2e max %.2e" % (
+        np.median(rel), np.percentile(rel, 99), rel.max()))
+    assert rel.max() < 1e-3
+    assert abs(sur["D_max"] - ref["D_max"]) <= 1e-3 * ref["D_max"]
+    # ablation: the same pipeline with the network's output forced to 0 (physics baseline only)
+    base = S.Surrogate(seed=5)
+    base.load(sg.save())
+    wts = base.get_params()
+    wts[SO.LAYOUT[8][1]:] = 0.0  # W5 and b5
+    base.set_params(wts)
+    abl = surrogate_fast(S, base, w, dev)
+    rel0 = np.abs(abl["D"][pos] - D0[pos]) / D0[pos]
+    say("  baseline only: median %.2e p99 %.2e max %.2e" % (np.median(rel0), np.percentile(rel0, 99), rel0.max()))
+    assert np.median(rel) < 0.7 * np.median(rel0)
+    assert np.percentile(rel, 99) < np.percentile(rel0, 99)
+    base.close()
+
+
+def test_out_of_distribution_falls_back(S, trained):
+    say("[gpu] out-of-distribution columns fall back to FP64")
+    sg, _ = trained
+    w = W.greenland_like(*WINDOW_TEST)
+    E = w.fields["E"].reshape(w.grid.Mx, w.grid.My, w.grid.Mz).copy()
+    E[20:40, 20:40, :] -= 2009.0 * 60.0  # 60 K colder than any training column
+    w.fields["E"] = E.ravel()
+    dev = {k: S.to_device(v) for k, v in w.fields.items()}
+    ref = plain_fast(S, w, dev)
+    sur = surrogate_fast(S, sg, w, dev)
+    assert sur["counts"][2] > 100
+    cold = np.zeros((w.grid.Mx, w.grid.My, 2), dtype=bool)
+    cold[21:39, 21:39, :] = True
+    c = cold.ravel() & (ref["D"] > 0)
+    assert np.array_equal(sur["D"][c], ref["D"][c])
+
+
+def test_surrogate_config_errors(S, trained):
+    sg, _ = trained
+    w = W.dome(Mx=11, My=11, Mz=11)  # isothermal Glen: not the surrogate's flow law
+    sia = S.SIAFD(w.grid, w.params)
+    with pytest.raises(S.SIAError) as e:
+        sia.use_surrogate(sg)
+    assert e.value.code == 6
+    fresh = S.Surrogate(seed=1)
+    w2 = W.greenland_like(700, 710, 1180, 1190)
+    sia2 = S.SIAFD(w2.grid, w2.params)
+    with pytest.raises(S.SIAError):
+        sia2.use_surrogate(fresh)  # no normalization yet
+    sia.close()
+    sia2.close()
+    fresh.close()
