@@ -198,6 +198,8 @@ void fill_dev_params(sia_handle *h) {
   d.hooke_k = p.Hooke_k;
   d.hooke_Tr = p.Hooke_Tr;
   d.sec_to_ka = 1.0 / (1000.0 * p.seconds_per_year);
+  d.inv_ci = 1.0 / p.ice_specific_heat_capacity;
+  d.inv_L = 1.0 / p.water_latent_heat_fusion;
   d.alpha_float = 1 - p.ice_density / p.sea_water_density;
   d.sea_level = p.sea_level;
   d.icefree_thk = p.mask_icefree_thickness_standard;
@@ -886,6 +888,7 @@ int sia_surrogate_collect(sia_surrogate *sg, sia_handle *h, const sia_inputs *in
   if (p.sia_flow_law != SIA_FLOW_GPBLD || p.enthalpy_converter != SIA_EC_DEFAULT) {
     return set_err(h, SIA_ERR_CONFIG, "the surrogate models GPBLD with the default enthalpy converter");
   }
+  if (h->dp.nm1_int < 1) return set_err(h, SIA_ERR_CONFIG, "the surrogate needs n in {2, 3, 4}");
   if (sg->glen_n != 0.0 && sg->glen_n != p.sia_Glen_exponent) {
     return set_err(h, SIA_ERR_CONFIG, "samples with another Glen exponent are already held");
   }
@@ -1080,7 +1083,7 @@ int sia_use_surrogate(sia_handle *h, sia_surrogate *sg) {
   if (p.sia_flow_law != SIA_FLOW_GPBLD || p.enthalpy_converter != SIA_EC_DEFAULT) {
     return set_err(h, SIA_ERR_CONFIG, "the surrogate models GPBLD with the default enthalpy converter");
   }
-  if (p.sia_Glen_exponent != sg->glen_n) {
+  if (p.sia_Glen_exponent != sg->glen_n || h->dp.nm1_int < 1) {
     return set_err(h, SIA_ERR_CONFIG, "the surrogate was trained for another Glen exponent");
   }
   if (sg->device != h->device) return set_err(h, SIA_ERR_CONFIG, "the surrogate lives on another device");

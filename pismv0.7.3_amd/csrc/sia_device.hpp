@@ -41,6 +41,7 @@ struct DevParams {
   double A_glen;            // ice_softness
   double hooke_A, hooke_Q, hooke_C, hooke_k, hooke_Tr;
   double sec_to_ka;
+  double inv_ci, inv_L;      // 1 / c_i, 1 / L (the surrogate's per-level pass)
   // geometry / time stepping
   double alpha_float, sea_level, icefree_thk, floating_thk, max_dt, adapt_ratio;
 };

@@ -360,9 +360,18 @@ __global__ __launch_bounds__(256) void k_column(DevParams P, ColumnArgs A) {
   __shared__ double wmax[WAVES_PER_BLOCK];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int Mx = P.Mx, My = P.My, Mz = P.Mz;
+  const int nstag = 2 * Mx * My;
+  if (A.only) {
+    // a block none of whose columns is selected leaves at once
+    const int sb = blockIdx.x * COLS_PER_BLOCK + (int)threadIdx.x;
+    const bool mine = (int)threadIdx.x < COLS_PER_BLOCK && sb < nstag && A.only[sb] == ONLY_MARK;
+    if (!__syncthreads_or(mine)) {
+      if (threadIdx.x == 0) A.dmax_part[blockIdx.x] = 0.0;
+      return;
+    }
+  }
   for (int k = threadIdx.x; k < Mz; k += blockDim.x) zs[k] = A.z[k];
   __syncthreads();
-  const int nstag = 2 * Mx * My;
   const int s0 = (blockIdx.x * WAVES_PER_BLOCK + wv) * COLS_PER_WAVE;
   const int s = s0 + lane;
 

@@ -13,6 +13,8 @@
 //       k_sg_loss, k_sg_act_bwd, k_sg_colsum, k_sg_adam
 #include "sia_surrogate.hpp"
 
+#include <type_traits>
+
 namespace pism_sia {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -76,6 +78,9 @@ __device__ __forceinline__ double sg_lane(double v, int l) {
   const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
   return __hiloint2double(hi, lo);
 }
+__device__ __forceinline__ double lane_value_d(double v, int l) { return sg_lane(v, l); }
+__device__ __forceinline__ int lane_value_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
 // sum of v over the 32 lanes of this lane's half wave
 __device__ __forceinline__ double half_sum(double v, int half) {
   const double s = sg_scan(v);
@@ -191,8 +196,12 @@ void launch_sg_list(const DevParams &P, const SgColumnArgs &C, const SgListArgs 
 }
 
 // ---------------------------------------------------------------- features
-// p^n with the integer power path of the flow laws
-__device__ __forceinline__ double pow_n(const DevParams &P, double p) { return stress_pow(P, p) * p; }
+// s^(n-1) and p^n for the integer exponents the surrogate supports (n - 1 in {1, 2, 3})
+__device__ __forceinline__ double ipow_nm1(const DevParams &P, double s) {
+  const double s2 = s * s;
+  return P.nm1_int == 1 ? s : (P.nm1_int == 2 ? s2 : s2 * s);
+}
+__device__ __forceinline__ double pow_n(const DevParams &P, double p) { return ipow_nm1(P, p) * p; }
 
 __device__ __forceinline__ int cell_of(double zk, double s32) {
   const int c = (int)(zk * s32);
@@ -209,126 +218,249 @@ __device__ __forceinline__ int first_level_of_cell(const double *zs, int ks, dou
   return lo;
 }
 
-// One wavefront takes two listed columns per step, one per half wave; lane m
-// of a half owns zeta-cell m and walks its levels. The level weights are the
-// reference's: W_k = d_k (dz_k [k >= 1] + dz_{k+1} [k < ks]) / 2 + d_ks^2 / 2 [k == ks]
+// A wavefront owns 16 consecutive list entries. Lane c prepares entry c (2-D
+// inputs, kBelowHeight), so the setup of all 16 is in flight at once. The
+// entries are then taken in pairs: all 64 lanes stage the averaged enthalpy
+// of both columns in LDS with coalesced loads (up to 4 x 64 levels of both
+// columns in flight before the first store), then half wave hh takes column
+// hh of the pair and lane m its zeta-cell m, walking the cell's levels from
+// LDS: the level weight G_k, the exact temperate sum, and the moments of
+// x = T_pa - 263.15 over the cold levels. The level weights are the reference's:
+// W_k = d_k (dz_k [k >= 1] + dz_{k+1} [k < ks]) / 2 + d_ks^2 / 2 [k == ks]
 // (SIAFD.cc:677-684 regrouped per delta_k), G_k = W_k p_k^n.
+constexpr int SG_FEAT_COLS = 16;
+
+// buckets of the level search: bucket b holds the first level with z >= b Lz / SG_NBUCKET
+constexpr int SG_NBUCKET = 1024;
+
+// first level k in [0, ks] with z_k * s32 >= m (ks + 1 if none), i.e.
+// cell_of(z_k) >= m for m <= 31: start at the bucket of height m / s32 and
+// step to the exact answer (0-2 steps for equal or quadratic spacing)
+__device__ __forceinline__ int cell_first_level(const double *zs, const int *bucket, double bscale, int ks,
+                                                double s32, int m) {
+  const double hgt = (double)m / s32;
+  int b = (int)(hgt * bscale);
+  b = b < 0 ? 0 : (b > SG_NBUCKET - 1 ? SG_NBUCKET - 1 : b);
+  int k = min(bucket[b], ks + 1);
+  while (k > 0 && zs[k - 1] * s32 >= (double)m) k--;
+  while (k <= ks && zs[k] * s32 < (double)m) k++;
+  return k;
+}
+
 template <bool TARGETS>
 __global__ __launch_bounds__(256) void k_sg_features(DevParams P, SgColumnArgs C, SgFeatureArgs F) {
+  // LDS: z levels [Mz], half spacings Sk[Mz], then per wavefront the two staged
+  // enthalpy profiles (FP32 in inference, FP64 for the exact training targets)
+  typedef typename std::conditional<TARGETS, double, float>::type EType;
   extern __shared__ double zs[];
   const int Mz = P.Mz;
-  for (int k = threadIdx.x; k < Mz; k += blockDim.x) zs[k] = C.z[k];
+  double *Sk = zs + Mz;  // (dz_k [k >= 1] + dz_{k+1}) / 2
+  __shared__ int bucket[SG_NBUCKET];
+  for (int k = threadIdx.x; k < Mz; k += blockDim.x) {
+    const double zk = C.z[k];
+    zs[k] = zk;
+    Sk[k] = 0.5 * (((k >= 1) ? zk - C.z[k - 1] : 0.0) + ((k + 1 < Mz) ? C.z[k + 1] - zk : 0.0));
+  }
+  const double bscale = (double)SG_NBUCKET / C.z[Mz - 1];
+  for (int b = threadIdx.x; b < SG_NBUCKET; b += blockDim.x) {
+    // first level at or above the bucket's lower edge
+    const double lo = (double)b / bscale;
+    int a = 0, c = Mz;
+    while (a < c) {
+      const int mid = (a + c) >> 1;
+      if (C.z[mid] >= lo) c = mid; else a = mid + 1;
+    }
+    bucket[b] = a;
+  }
   __syncthreads();
-  const int lane = threadIdx.x & 63, half = lane >> 5, m = lane & 31;
+  const int nwb = blockDim.x >> 6;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  EType *ebw = reinterpret_cast<EType *>(zs + 2 * Mz) + (long)wv * 2 * Mz;
   const int n = *F.n_list;
   const int stride = TARGETS ? F.stride : 1;
-  const int nsel = (n + stride - 1) / stride;  // selected columns: list slots 0, stride, 2 stride, ...
+  const int nsel = (n + stride - 1) / stride;
   const long long base_sample = TARGETS ? *F.n_samples : 0;
-  const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int nw = gridDim.x * (blockDim.x >> 6);
   const double T_ref = 263.15;
-  for (int t0 = 2 * gw; t0 < nsel; t0 += 2 * nw) {
-    const int t = t0 + half;
-    const bool live = t < nsel;
-    const int slot = live ? t * stride : 0;
-    const int s = live ? F.list[slot] : F.list[0];
-    const int q = s >> 1, o = s & 1;
-    const int i = q / P.My, j = q - i * P.My;
-    const int q2 = (o == 0) ? wrapi(i + 1, P.Mx) * P.My + j : i * P.My + wrapi(j + 1, P.My);
-    const double thk = 0.5 * (C.thks[q] + C.thks[q2]);
-    const double hxs = C.hx[s], hys = C.hy[s];
-    const double alpha = sqrt(hxs * hxs + hys * hys);
-    const double theta_local = 0.5 * (C.theta[q] + C.theta[q2]);
-    const double Cf = P.e * theta_local * 2.0 * stress_pow(P, alpha);
-    const int ks = kbelow(zs, Mz, P.Lz, thk);  // >= 1 for listed columns
-    const double s32 = (double)SG_NCELL / thk;
-    const int klo = first_level_of_cell(zs, ks, s32, m);
-    const int khi = first_level_of_cell(zs, ks, s32, m + 1);
-    int cnt = live ? khi - klo : 0;
+  const double rhog = P.rho_i * P.g, T_off = P.T0 + P.T_melt - T_ref;
+  const int hh = lane >> 5, m = lane & (SG_NCELL - 1);
+  for (int t0 = (blockIdx.x * nwb + wv) * SG_FEAT_COLS; t0 < nsel; t0 += gridDim.x * nwb * SG_FEAT_COLS) {
+    // ---- setup: lane c < 16 prepares entry t0 + c (c_ks = -1: no entry)
+    int c_ks = -1, c_q = 0, c_q2 = 0, c_s = 0;
+    double c_thk = 0.0, c_lgs = 0.0, c_Cf = 0.0;
+    if (lane < SG_FEAT_COLS && t0 + lane < nsel) {
+      const int s = F.list[(t0 + lane) * stride];
+      const int q = s >> 1, o = s & 1;
+      const int i = q / P.My, j = q - i * P.My;
+      const int q2 = (o == 0) ? wrapi(i + 1, P.Mx) * P.My + j : i * P.My + wrapi(j + 1, P.My);
+      const double thk = 0.5 * (C.thks[q] + C.thks[q2]);
+      const double hxs = C.hx[s], hys = C.hy[s];
+      const double alpha = sqrt(hxs * hxs + hys * hys);
+      c_lgs = log10(alpha) + 2.0;  // the slope input of the network
+      c_Cf = P.e * (0.5 * (C.theta[q] + C.theta[q2])) * 2.0 * ipow_nm1(P, alpha);
+      c_ks = kbelow(zs, Mz, P.Lz, thk);  // >= 1 for listed columns
+      c_thk = thk;
+      c_q = q;
+      c_q2 = q2;
+      c_s = s;
+    }
+    for (int cc = 0; cc < SG_FEAT_COLS; cc += 2) {
+      if (lane_value_i(c_ks, cc) < 0) break;  // entries are consecutive: nothing left
+      // ---- stage the averaged enthalpy of columns cc, cc + 1
+      int ksp[2];
+      const double *Ep[2][2];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) cnt = max(cnt, __shfl_xor(cnt, off, 64));
-    const double *__restrict__ Ea = C.E + (long)q * Mz;
-    const double *__restrict__ Eb = C.E + (long)q2 * Mz;
-    double st = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0, qref = 0.0;
-    for (int it = 0; it < cnt; it++) {
-      const int k = klo + it;
-      if (live && k < khi) {
-        const double d = thk - zs[k];
-        const double p = ec_pressure(P, d);
-        const double dzk = (k >= 1) ? zs[k] - zs[k - 1] : 0.0;
-        const double dzk1 = (k < ks) ? zs[k + 1] - zs[k] : 0.0;
-        double W = 0.5 * d * (dzk + dzk1);
-        if (k == ks) W += 0.5 * d * d;
-        const double G = W * pow_n(P, p);
-        const double Ek = 0.5 * (Ea[k] + Eb[k]);
-        if (Ek < ec_enthalpy_cts(P, p)) {  // cold branch of GPBLD (flowlaws.cc:212-214)
-          const double Tpa = ec_pressure_adjusted_temperature(P, Ek, p);
-          const double x = Tpa - T_ref;
+      for (int h = 0; h < 2; h++) {
+        ksp[h] = lane_value_i(c_ks, cc + h);
+        Ep[h][0] = C.E + (long)lane_value_i(c_q, cc + h) * Mz;
+        Ep[h][1] = C.E + (long)lane_value_i(c_q2, cc + h) * Mz;
+      }
+      const int ksmax = max(ksp[0], ksp[1]);
+      for (int kb = 0; kb <= ksmax; kb += 128) {
+        double ea[2][2], ebv[2][2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+#pragma unroll
+          for (int u = 0; u < 2; u++) {
+            const int k = kb + 64 * u + lane;
+            ea[h][u] = 0.0;
+            ebv[h][u] = 0.0;
+            if (k <= ksp[h]) {
+              ea[h][u] = Ep[h][0][k];
+              ebv[h][u] = Ep[h][1][k];
+            }
+          }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+#pragma unroll
+          for (int u = 0; u < 2; u++) {
+            const int k = kb + 64 * u + lane;
+            if (k <= ksp[h]) ebw[h * Mz + k] = (EType)(0.5 * (ea[h][u] + ebv[h][u]));
+          }
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // ---- half wave hh: column cc + hh, lane m: zeta-cell m
+      const int ks = hh ? ksp[1] : ksp[0];
+      const bool live = ks >= 0;
+      const double thk = hh ? lane_value_d(c_thk, cc + 1) : lane_value_d(c_thk, cc);
+      const double lgs = hh ? lane_value_d(c_lgs, cc + 1) : lane_value_d(c_lgs, cc);
+      const double Cf = hh ? lane_value_d(c_Cf, cc + 1) : lane_value_d(c_Cf, cc);
+      const int s = hh ? lane_value_i(c_s, cc + 1) : lane_value_i(c_s, cc);
+      // cell m: levels [klo, khi); khi is the next lane's klo
+      const double s32 = (double)SG_NCELL / (live ? thk : 1.0);
+      const int klo = live ? cell_first_level(zs, bucket, bscale, ks, s32, m) : 0;
+      int khi = __shfl_down(klo, 1, 64);
+      if (m == SG_NCELL - 1) khi = ks + 1;
+      if (!live) khi = 0;
+      // G is accumulated relative to thk^2 p_base^n
+      const double gs = live ? thk * thk * pow_n(P, P.p_air + rhog * thk) : 1.0;
+      const double inv_gs = 1.0 / gs;
+      const EType *eb = ebw + hh * Mz;
+      double st = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0, qref = 0.0, dsum = 0.0;
+#pragma unroll 4
+      for (int k = klo; k < khi; k++) {
+        const double zk = zs[k];
+        const double d = thk - zk;  // >= 0 up to ks
+        const double p = P.p_air + rhog * d;
+        const double W = (k < ks) ? d * Sk[k] : 0.5 * d * (((k >= 1) ? zk - zs[k - 1] : 0.0) + d);
+        const double G = W * pow_n(P, p) * inv_gs;
+        const double Ek = (double)eb[k];
+        // default enthalpy converter: T_m = T_melt - beta p, E_cts = c_i (T_m - T0)
+        // (enthalpyConverter.cc:128-143)
+        const double Tm = P.T_melt - P.beta * p;
+        const double Ects = P.ci * (Tm - P.T0);
+        if (Ek < Ects) {  // cold branch of GPBLD (flowlaws.cc:212-214): T_pa - 263.15
+          const double x = Ek * P.inv_ci + T_off - Tm;
           g0 += G;
           g1 += G * x;
           g2 += G * x * x;
-          if (TARGETS) qref += G * softness_pb(P, Tpa);
-        } else {  // temperate branch (flowlaws.cc:215-219)
-          double omega = ec_water_fraction(P, Ek, p);
-          if (P.wf_limit < omega) omega = P.wf_limit;
+          if (TARGETS) {
+            const double A = softness_pb(P, ec_pressure_adjusted_temperature(P, Ek, p));
+            qref += G * A;
+            dsum += G * A;
+          }
+        } else {  // temperate branch (flowlaws.cc:215-219), omega = (E - E_cts) / L
+          const double omega = fmin((Ek - Ects) * P.inv_L, P.wf_limit);
           st += G * (P.A_T0 * (1.0 + P.wf_coeff * omega));
         }
       }
-    }
-    const double S_t = half_sum(st, half);
-    const bool has = g0 > 0.0;
-    double Qb = 0.0, feat = 2.0;
-    if (has) {
-      const double mx = g1 / g0;
-      const double Tbar = T_ref + mx;
-      const double var = fmax(g2 / g0 - mx * mx, 0.0);
-      const double Qa = (Tbar < P.T_crit) ? P.Q_cold : P.Q_warm;
-      const double kk = Qa / (P.R * Tbar * Tbar);
-      Qb = g0 * softness_pb(P, Tbar) * (1.0 + 0.5 * (kk * kk - 2.0 * kk / Tbar) * var);
-      feat = (Tbar - 250.0) / 15.0;
-    }
-    if (!TARGETS) {
-      if (live) {
-        F.F[(long)slot * SG_FSTRIDE + m] = (float)feat;
-        F.CQb[(long)slot * SG_NCELL + m] = (float)(Cf * Qb);
-        if (m == 0) {
-          F.F[(long)slot * SG_FSTRIDE + 32] = (float)(thk / 1000.0);
-          F.F[(long)slot * SG_FSTRIDE + 33] = (float)(log10(alpha) + 2.0);
-          F.CSt[slot] = Cf * S_t;
+      const double S_t = half_sum(st, hh) * gs;
+      const bool has = live && g0 > 0.0;
+      double Qb = 0.0, feat = 2.0;
+      if (has) {
+        const double inv_g0 = 1.0 / g0;
+        const double mx = g1 * inv_g0;
+        const double Tbar = T_ref + mx;
+        const double var = fmax(g2 * inv_g0 - mx * mx, 0.0);
+        const double inv_T = 1.0 / Tbar;
+        const bool cold = Tbar < P.T_crit;
+        const double QR = (cold ? P.Q_cold : P.Q_warm) / P.R;
+        const double kk = QR * inv_T * inv_T;
+        // Paterson-Budd softness at the cell mean (flowlaws.cc:90-95)
+        const double Ab = (cold ? P.A_cold : P.A_warm) * exp(-QR * inv_T);
+        Qb = g0 * gs * Ab * (1.0 + 0.5 * (kk * kk - 2.0 * kk * inv_T) * var);
+        feat = (Tbar - 250.0) * (1.0 / 15.0);
+      }
+      const int t = t0 + cc + hh;
+      if (!TARGETS) {
+        if (live) {
+          const int slot = t;
+          F.F[(long)slot * SG_FSTRIDE + m] = (float)feat;
+          F.CQb[(long)slot * SG_NCELL + m] = (float)(Cf * Qb);
+          if (m == 0) {
+            F.F[(long)slot * SG_FSTRIDE + 32] = (float)(thk / 1000.0);
+            F.F[(long)slot * SG_FSTRIDE + 33] = (float)lgs;
+            F.CSt[slot] = Cf * S_t;
+          }
+        }
+      } else {
+        const double qcell = qref * gs;  // exact cold sum of the cell
+        const double Dref = Cf * (S_t + half_sum(dsum, hh) * gs);
+        const unsigned long long hm = __ballot(has);
+        const unsigned hasbits = (unsigned)(hh ? (hm >> 32) : (hm & 0xffffffffull));
+        const long long idx = base_sample + t;
+        if (live && idx < F.cap) {
+          SgSample &S = F.samples[idx];
+          S.f[m] = (float)feat;
+          S.r[m] = has ? (float)log(qcell / Qb) : 0.0f;
+          S.share[m] = (Dref > 0.0) ? (float)(Cf * qcell / Dref) : 0.0f;
+          if (m == 0) {
+            S.f[32] = (float)(thk / 1000.0);
+            S.f[33] = (float)lgs;
+            S.f[34] = 0.0f;
+            S.f[35] = 0.0f;
+            S.has = hasbits;
+            S.col = s;
+            S.dref = Dref;
+          }
         }
       }
-    } else {
-      const double Dref = Cf * (S_t + half_sum(qref, half));
-      const unsigned long long hm = __ballot(has);
-      const unsigned hasbits = (unsigned)(half ? (hm >> 32) : (hm & 0xffffffffull));
-      const long long idx = base_sample + t;
-      if (live && idx < F.cap) {
-        SgSample &S = F.samples[idx];
-        S.f[m] = (float)feat;
-        S.r[m] = has ? (float)log(qref / Qb) : 0.0f;
-        S.share[m] = (Dref > 0.0) ? (float)(Cf * qref / Dref) : 0.0f;
-        if (m == 0) {
-          S.f[32] = (float)(thk / 1000.0);
-          S.f[33] = (float)(log10(alpha) + 2.0);
-          S.f[34] = 0.0f;
-          S.f[35] = 0.0f;
-          S.has = hasbits;
-          S.col = s;
-          S.dref = Dref;
-        }
-      }
+      // the next pair overwrites the enthalpy buffers
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
   }
 }
 
+// dynamic LDS of k_sg_features: z levels and half spacings + per wavefront two enthalpy profiles
+static size_t sg_features_lds(int Mz, int waves, bool targets) {
+  return 2 * sizeof(double) * (size_t)Mz + (size_t)waves * 2 * Mz * (targets ? sizeof(double) : sizeof(float));
+}
+
 void launch_sg_features(const DevParams &P, const SgColumnArgs &C, const SgFeatureArgs &F, bool targets,
                         hipStream_t st) {
-  const int nb = 2048;  // persistent: 8192 wavefronts walk the list
-  const size_t lds = sizeof(double) * (size_t)P.Mz;
+  // four wavefronts a block while their enthalpy buffers fit the LDS, else one
+  const int waves = sg_features_lds(P.Mz, 4, targets) <= 64 * 1024 ? 4 : 1;
+  const int nb = 8192 / waves;  // persistent: 8192 wavefronts walk the list, 16 entries at a time
+  const size_t lds = sg_features_lds(P.Mz, waves, targets);
   if (targets) {
-    hipLaunchKernelGGL(k_sg_features<true>, dim3(nb), dim3(256), lds, st, P, C, F);
+    hipLaunchKernelGGL(k_sg_features<true>, dim3(nb), dim3(64 * waves), lds, st, P, C, F);
   } else {
-    hipLaunchKernelGGL(k_sg_features<false>, dim3(nb), dim3(256), lds, st, P, C, F);
+    hipLaunchKernelGGL(k_sg_features<false>, dim3(nb), dim3(64 * waves), lds, st, P, C, F);
   }
 }
 
@@ -372,30 +504,80 @@ void launch_sg_pack(const float *master, unsigned short *frag, hipStream_t st) {
 constexpr int SG_MLP_BLOCKS = 256;  // one block per CU: the 120 KB of weights fill its LDS
 int sg_mlp_blocks() { return SG_MLP_BLOCKS; }
 
-__device__ __forceinline__ f32x4 mfma(const unsigned short *fr, int lane, bf16x8 b, f32x4 c) {
-  const uint4 a = *reinterpret_cast<const uint4 *>(fr + lane * 8);
-  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(a), b, c, 0, 0, 0);
+__device__ __forceinline__ bf16x8 load_frag(const unsigned short *fr, int lane) {
+  return as_bf16x8(*reinterpret_cast<const uint4 *>(fr + lane * 8));
+}
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// GELU in inference: linear interpolation in a table of GELU at 1024 points
+// on [-8, 8) (|error| < 3e-5, below BF16 resolution); beyond the ends the
+// first and last segments extrapolate linearly (GELU ~ 0 and ~ x there).
+constexpr int SG_GTAB = 1024;
+constexpr float SG_GLO = -8.0f, SG_GSCALE = (float)SG_GTAB / 16.0f;
+typedef float f32x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ float gelu_tab(const float2 *tab, float x) {
+  const float u = (x - SG_GLO) * SG_GSCALE;
+  const int i = min(max((int)u, 0), SG_GTAB - 1);
+  const float2 e = tab[i];
+  return e.x + e.y * (u - (float)i);
 }
 
 // bias + GELU of 8 accumulator tiles -> the 4 B-operand chunks of the next layer
-__device__ __forceinline__ void act_to_chunks(const f32x4 *acc, const float *bias, int g, bf16x8 *h) {
+__device__ __forceinline__ void act_to_chunks(const f32x4 *acc, const float *bias, const float2 *tab, int g,
+                                              bf16x8 *h) {
 #pragma unroll
   for (int c = 0; c < 4; c++) {
-    unsigned short u[8];
+    f32x8 v;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      u[i] = f2bf(gelu_f(acc[2 * c][i] + bias[16 * (2 * c) + 4 * g + i]));
-      u[4 + i] = f2bf(gelu_f(acc[2 * c + 1][i] + bias[16 * (2 * c + 1) + 4 * g + i]));
+      v[i] = gelu_tab(tab, acc[2 * c][i] + bias[16 * (2 * c) + 4 * g + i]);
+      v[4 + i] = gelu_tab(tab, acc[2 * c + 1][i] + bias[16 * (2 * c + 1) + 4 * g + i]);
     }
-    __builtin_memcpy(&h[c], u, 16);
+    h[c] = __builtin_convertvector(v, bf16x8);  // round to nearest even
   }
 }
 
-__global__ __launch_bounds__(256) void k_sg_mlp(DevParams P, SgMlpArgs A) {
-  extern __shared__ unsigned short wfrag[];  // [SG_NFRAG][64][8]
+// layer-0 inputs of one sample row: features 8g..8g+7 (chunk 0), 32+8g..+7 (chunk 1);
+// returns nonzero when a feature lies outside the training range
+__device__ __forceinline__ int load_inputs(const SgMlpArgs &A, int slot, bool live, int g, bf16x8 &x0, bf16x8 &x1) {
+  f32x8 v0, v1;
+  int ood = 0;
+  const float *fr = A.F + (long)(live ? slot : 0) * SG_FSTRIDE;
+#pragma unroll
+  for (int jj = 0; jj < 8; jj++) {
+    const int f = 8 * g + jj;
+    const float v = fr[f];
+    ood |= (v < A.norm.lo[f]) | (v > A.norm.hi[f]);
+    v0[jj] = (v - A.norm.mean[f]) * A.norm.inv_std[f];
+    const int f1 = 32 + 8 * g + jj;
+    float w1 = 0.0f;
+    if (f1 < SG_NIN) {
+      const float r = fr[f1];
+      ood |= (r < A.norm.lo[f1]) | (r > A.norm.hi[f1]);
+      w1 = (r - A.norm.mean[f1]) * A.norm.inv_std[f1];
+    }
+    v1[jj] = w1;
+  }
+  x0 = __builtin_convertvector(v0, bf16x8);
+  x1 = __builtin_convertvector(v1, bf16x8);
+  ood |= __shfl_xor(ood, 16, 64);
+  ood |= __shfl_xor(ood, 32, 64);
+  return live ? ood : 0;
+}
+
+// Blocks of 16 wavefronts (4 per SIMD) share one copy of the weights in LDS;
+// each wavefront evaluates 16-sample tiles.
+constexpr int SG_MLP_WAVES = 16;
+
+__global__ __launch_bounds__(64 * SG_MLP_WAVES) void k_sg_mlp(DevParams P, SgMlpArgs A) {
+  extern __shared__ unsigned short wfrag[];  // [SG_NFRAG][64][8], then the GELU table
   __shared__ float bias[4 * SG_WIDTH + SG_NOUT];
-  __shared__ double wmax[4];
-  __shared__ int wfall[4];
+  __shared__ double wmax[SG_MLP_WAVES];
+  __shared__ int wfall[SG_MLP_WAVES];
+  float2 *gtab = reinterpret_cast<float2 *>(wfrag + SG_NFRAG * 512);
   {
     const uint4 *src = reinterpret_cast<const uint4 *>(A.frag);
     uint4 *dst = reinterpret_cast<uint4 *>(wfrag);
@@ -407,6 +589,11 @@ __global__ __launch_bounds__(256) void k_sg_mlp(DevParams P, SgMlpArgs A) {
       bias[3 * SG_WIDTH + e] = A.bias[SG_B4 + e];
     }
     for (int e = threadIdx.x; e < SG_NOUT; e += blockDim.x) bias[4 * SG_WIDTH + e] = A.bias[SG_B5 + e];
+    for (int e = threadIdx.x; e < SG_GTAB; e += blockDim.x) {
+      const float x0 = SG_GLO + (float)e / SG_GSCALE, x1 = SG_GLO + (float)(e + 1) / SG_GSCALE;
+      const float g0 = gelu_f(x0), g1 = gelu_f(x1);
+      gtab[e] = make_float2(g0, g1 - g0);
+    }
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -415,72 +602,49 @@ __global__ __launch_bounds__(256) void k_sg_mlp(DevParams P, SgMlpArgs A) {
   const int ntile = (n + 15) / 16;
   double dmax = 0.0;
   int nfall = 0;
-  for (int tile = blockIdx.x * 4 + wv; tile < ntile; tile += gridDim.x * 4) {
+  for (int tile = blockIdx.x * SG_MLP_WAVES + wv; tile < ntile; tile += gridDim.x * SG_MLP_WAVES) {
     const int slot = tile * 16 + smp;
     const bool live = slot < n;
-    // layer-0 inputs: features 8g..8g+7 (chunk 0) and 32+8g..+7 (chunk 1)
     bf16x8 x0, x1;
-    int ood = 0;
-    {
-      unsigned short u0[8], u1[8];
-      const float *fr = A.F + (long)(live ? slot : 0) * SG_FSTRIDE;
-#pragma unroll
-      for (int jj = 0; jj < 8; jj++) {
-        const int f = 8 * g + jj;
-        const float v = fr[f];
-        ood |= (v < A.norm.lo[f]) | (v > A.norm.hi[f]);
-        u0[jj] = f2bf((v - A.norm.mean[f]) * A.norm.inv_std[f]);
-        const int f1 = 32 + 8 * g + jj;
-        float v1 = 0.0f;
-        if (f1 < SG_NIN) {
-          const float r = fr[f1];
-          ood |= (r < A.norm.lo[f1]) | (r > A.norm.hi[f1]);
-          v1 = (r - A.norm.mean[f1]) * A.norm.inv_std[f1];
-        }
-        u1[jj] = f2bf(v1);
-      }
-      __builtin_memcpy(&x0, u0, 16);
-      __builtin_memcpy(&x1, u1, 16);
-    }
-    ood |= __shfl_xor(ood, 16, 64);
-    ood |= __shfl_xor(ood, 32, 64);
+    const int ood = load_inputs(A, slot, live, g, x0, x1);
     f32x4 acc[8];
     bf16x8 h[4];
 #pragma unroll
     for (int t = 0; t < 8; t++) {
-      f32x4 a = {0.0f, 0.0f, 0.0f, 0.0f};
-      a = mfma(wfrag + frag_index(0, t, 0) * 512, lane, x0, a);
-      a = mfma(wfrag + frag_index(0, t, 1) * 512, lane, x1, a);
-      acc[t] = a;
+      f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+      c = mfma(load_frag(wfrag + frag_index(0, t, 0) * 512, lane), x0, c);
+      acc[t] = mfma(load_frag(wfrag + frag_index(0, t, 1) * 512, lane), x1, c);
     }
-    act_to_chunks(acc, bias, g, h);
-#pragma unroll
+    act_to_chunks(acc, bias, gtab, g, h);
     for (int layer = 1; layer < 4; layer++) {
 #pragma unroll
       for (int t = 0; t < 8; t++) {
-        f32x4 a = {0.0f, 0.0f, 0.0f, 0.0f};
+        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int c = 0; c < 4; c++) a = mfma(wfrag + frag_index(layer, t, c) * 512, lane, h[c], a);
-        acc[t] = a;
+        for (int cc = 0; cc < 4; cc++) c = mfma(load_frag(wfrag + frag_index(layer, t, cc) * 512, lane), h[cc], c);
+        acc[t] = c;
+        // keep each tile's weight reads next to its MFMAs (hoisting them all spills)
+        __builtin_amdgcn_sched_barrier(0);
       }
-      act_to_chunks(acc, bias + layer * SG_WIDTH, g, h);
+      act_to_chunks(acc, bias + layer * SG_WIDTH, gtab, g, h);
     }
     f32x4 o0 = {0.0f, 0.0f, 0.0f, 0.0f}, o1 = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-      o0 = mfma(wfrag + frag_index(4, 0, c) * 512, lane, h[c], o0);
-      o1 = mfma(wfrag + frag_index(4, 1, c) * 512, lane, h[c], o1);
+    for (int cc = 0; cc < 4; cc++) {
+      o0 = mfma(load_frag(wfrag + frag_index(4, 0, cc) * 512, lane), h[cc], o0);
+      o1 = mfma(load_frag(wfrag + frag_index(4, 1, cc) * 512, lane), h[cc], o1);
     }
     // D = C S_t + sum_m C Qb_m exp(r_m); this lane holds cells 4g..4g+3, 16+4g..16+4g+3
+    const float *bo = bias + 4 * SG_WIDTH;
     float part = 0.0f;
     if (live) {
-      const float *cq = A.CQb + (long)slot * SG_NCELL;
-      const float *bo = bias + 4 * SG_WIDTH;
-#pragma unroll
-      for (int i = 0; i < 4; i++) {
-        part += cq[4 * g + i] * __expf((o0[i] + bo[4 * g + i]) * A.norm.rscale);
-        part += cq[16 + 4 * g + i] * __expf((o1[i] + bo[16 + 4 * g + i]) * A.norm.rscale);
-      }
+      const float4 q0 = *reinterpret_cast<const float4 *>(A.CQb + (long)slot * SG_NCELL + 4 * g);
+      const float4 q1 = *reinterpret_cast<const float4 *>(A.CQb + (long)slot * SG_NCELL + 16 + 4 * g);
+      const float rs = A.norm.rscale;
+      part = q0.x * __expf((o0[0] + bo[4 * g + 0]) * rs) + q0.y * __expf((o0[1] + bo[4 * g + 1]) * rs) +
+             q0.z * __expf((o0[2] + bo[4 * g + 2]) * rs) + q0.w * __expf((o0[3] + bo[4 * g + 3]) * rs) +
+             q1.x * __expf((o1[0] + bo[16 + 4 * g + 0]) * rs) + q1.y * __expf((o1[1] + bo[16 + 4 * g + 1]) * rs) +
+             q1.z * __expf((o1[2] + bo[16 + 4 * g + 2]) * rs) + q1.w * __expf((o1[3] + bo[16 + 4 * g + 3]) * rs);
     }
     part += __shfl_xor(part, 16, 64);
     part += __shfl_xor(part, 32, 64);
@@ -491,9 +655,9 @@ __global__ __launch_bounds__(256) void k_sg_mlp(DevParams P, SgMlpArgs A) {
         nfall++;
       } else {
         const double D = A.CSt[slot] + (double)part;
-        const int q = s >> 1, o = s & 1;
+        const int q = s >> 1, oo = s & 1;
         const int i = q / P.My, j = q - i * P.My;
-        const double slope = (o == 0) ? A.hx[s] : A.hy[s];
+        const double slope = (oo == 0) ? A.hx[s] : A.hy[s];
         const double Dout = (i >= P.Mx - 1 || j >= P.My - 1) ? 0.0 : D;  // SIAFD.cc:694-697
         A.Dstag[s] = D;
         A.flux[s] = -Dout * slope;
@@ -513,7 +677,7 @@ __global__ __launch_bounds__(256) void k_sg_mlp(DevParams P, SgMlpArgs A) {
   if (threadIdx.x == 0) {
     double mx = wmax[0];
     int nf = wfall[0];
-    for (int w = 1; w < 4; w++) {
+    for (int w = 1; w < SG_MLP_WAVES; w++) {
       mx = fmax(mx, wmax[w]);
       nf += wfall[w];
     }
@@ -524,8 +688,8 @@ __global__ __launch_bounds__(256) void k_sg_mlp(DevParams P, SgMlpArgs A) {
 }
 
 void launch_sg_mlp(const DevParams &P, const SgMlpArgs &A, hipStream_t st) {
-  const size_t lds = (size_t)SG_NFRAG * 512 * sizeof(unsigned short);
-  hipLaunchKernelGGL(k_sg_mlp, dim3(SG_MLP_BLOCKS), dim3(256), lds, st, P, A);
+  const size_t lds = (size_t)SG_NFRAG * 512 * sizeof(unsigned short) + sizeof(float2) * SG_GTAB;
+  hipLaunchKernelGGL(k_sg_mlp, dim3(SG_MLP_BLOCKS), dim3(64 * SG_MLP_WAVES), lds, st, P, A);
 }
 
 // ---------------------------------------------------------------- training
@@ -672,13 +836,19 @@ __global__ void k_sg_act_bwd(const float *dA, const float *Z, int n, float *dZ, 
   dZbf[e] = f2bf(v);
 }
 
-// db[n] = sum_b dZ[b][n] (fixed order)
-__global__ void k_sg_colsum(const float *dZ, int B, int N, float *db) {
-  const int nn = blockIdx.x * blockDim.x + threadIdx.x;
-  if (nn >= N) return;
+// db[n] = sum_b dZ[b][n]: one block per n, fixed-order tree
+__global__ __launch_bounds__(256) void k_sg_colsum(const float *dZ, int B, int N, float *db) {
+  __shared__ float red[256];
+  const int nn = blockIdx.x;
   float s = 0.0f;
-  for (int b = 0; b < B; b++) s += dZ[(long)b * N + nn];
-  db[nn] = s;
+  for (int b = threadIdx.x; b < B; b += blockDim.x) s += dZ[(long)b * N + nn];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) db[nn] = red[0];
 }
 
 // copies a [128][64] padded gradient of W1 into the [128][34] master layout
@@ -714,7 +884,7 @@ void launch_sg_forward_backward(const float *master, const SgTrainBuffers &T, fl
     float *dW = (l == 0) ? T.dWpad : T.grad + gw[l];
     gemm(N, K, B, T.dZbf, 1, N, T.Abf[l], K, 1, dW, st);
     if (l == 0) hipLaunchKernelGGL(k_sg_unpad_w1, dim3(nblk(SG_WIDTH * SG_NIN)), dim3(256), 0, st, T.dWpad, T.grad);
-    hipLaunchKernelGGL(k_sg_colsum, dim3(1), dim3(256), 0, st, T.dZ, B, N, T.grad + boff[l]);
+    hipLaunchKernelGGL(k_sg_colsum, dim3(N), dim3(256), 0, st, T.dZ, B, N, T.grad + boff[l]);
     if (l > 0) {
       // dA_{l}[B][K] = dZ W_l ; dZ_{l-1} = dA * gelu'(Z_{l-1})
       gemm(B, K, N, T.dZbf, N, 1, T.Wbf + woff[l], K, 1, T.dA, st);
