@@ -253,7 +253,9 @@ enum sia_event {
   SIA_EV_3D_HOR_VEL = 3,     /* "SIA 3D hor. vel." */
   SIA_EV_VERT_VEL = 4,       /* "SB vert. vel." */
   SIA_EV_MASS_TRANSPORT = 5, /* "mass transport" (+ time step, surface and mask) */
-  SIA_EV_COUNT = 6
+  SIA_EV_SG_FEATURES = 6,    /* "surrogate features": the learned surrogate's per-level pass (inside "SIA flux") */
+  SIA_EV_SG_MLP = 7,         /* "surrogate MLP": its fused network (inside "SIA flux") */
+  SIA_EV_COUNT = 8
 };
 int sia_profiling(sia_handle *h, int on);
 int sia_profiling_read(sia_handle *h, double *ms /* [SIA_EV_COUNT] */, int64_t *count /* [SIA_EV_COUNT] */);

@@ -400,7 +400,10 @@ int surrogate_flux(sia_handle *h, const sia_inputs *in, ColumnArgs A) {
   F.CQb = h->sg_CQb;
   F.CSt = h->sg_CSt;
   F.status = h->sg_status;
-  launch_sg_features(P, C, F, false, st);
+  {
+    Stage stage(h, SIA_EV_SG_FEATURES);
+    launch_sg_features(P, C, F, false, st);
+  }
   SgMlpArgs M{};
   M.list = h->sg_list;
   M.n_list = h->sg_nlist;
@@ -417,7 +420,10 @@ int surrogate_flux(sia_handle *h, const sia_inputs *in, ColumnArgs A) {
   M.Dstag = h->Dstag;
   M.dmax_part = h->dmax_part + column_blocks(P);
   M.counts = h->sg_counts;
-  launch_sg_mlp(P, M, st);
+  {
+    Stage stage(h, SIA_EV_SG_MLP);
+    launch_sg_mlp(P, M, st);
+  }
   A.only = h->sg_status;
   launch_column(P, A, false, st, sg_mlp_blocks());
   HIP_TRY(h, hipGetLastError());
@@ -818,8 +824,9 @@ int sia_profiling_read(sia_handle *h, double *ms, int64_t *count) {
 }
 
 const char *sia_event_name(int event) {
-  static const char *names[SIA_EV_COUNT] = {"SIA bed smoother", "SIA gradient", "SIA flux",
-                                            "SIA 3D hor. vel.", "SB vert. vel.", "mass transport"};
+  static const char *names[SIA_EV_COUNT] = {"SIA bed smoother", "SIA gradient",       "SIA flux",
+                                            "SIA 3D hor. vel.", "SB vert. vel.",      "mass transport",
+                                            "surrogate features", "surrogate MLP"};
   return (event >= 0 && event < SIA_EV_COUNT) ? names[event] : "unknown";
 }
 
@@ -928,7 +935,7 @@ int sia_surrogate_collect(sia_surrogate *sg, sia_handle *h, const sia_inputs *in
 int sia_surrogate_fit_normalization(sia_surrogate *sg) {
   if (!sg) return SIA_ERR_ARGUMENT;
   if (sg->n < 2) return SIA_ERR_STATE;
-  const size_t np = (size_t)SG_STAT_WAVES * SG_NIN * 4 + SG_STAT_WAVES;
+  const size_t np = (size_t)SG_STAT_WAVES * SG_NIN * 4 + (size_t)SG_STAT_WAVES * 3;
   double *d = nullptr;
   if (hipMalloc((void **)&d, sizeof(double) * np) != hipSuccess) return SIA_ERR_OUT_OF_MEMORY;
   launch_sg_norm_stats(sg->samples, sg->n, d, sg->stream);
@@ -956,9 +963,16 @@ int sia_surrogate_fit_normalization(sia_surrogate *sg) {
     sg->norm.lo[f] = (float)(lo - margin);
     sg->norm.hi[f] = (float)(hi + margin);
   }
-  double rmax = 0.0;
-  for (int w = 0; w < nw; w++) rmax = std::max(rmax, hbuf[(size_t)SG_STAT_WAVES * SG_NIN * 4 + w]);
-  sg->norm.rscale = (float)(rmax > 0.0 ? rmax : 1.0);
+  // output unit: 4 x the RMS target log-correction (robust to rare outliers)
+  double rmax = 0.0, r2 = 0.0, rn = 0.0;
+  for (int w = 0; w < nw; w++) {
+    const double *o = &hbuf[(size_t)SG_STAT_WAVES * SG_NIN * 4 + (size_t)w * 3];
+    rmax = std::max(rmax, o[0]);
+    r2 += o[1];
+    rn += o[2];
+  }
+  const double rms = rn > 0.0 ? std::sqrt(r2 / rn) : 0.0;
+  sg->norm.rscale = (float)(rms > 0.0 ? 4.0 * rms : (rmax > 0.0 ? rmax : 1.0));
   sg->have_norm = true;
   return SIA_OK;
 }

@@ -918,7 +918,7 @@ __global__ void k_sg_norm_stats(const SgSample *__restrict__ samples, long long 
   const int lane = threadIdx.x & 63;
   const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int nw = gridDim.x * (blockDim.x >> 6);
-  double s = 0.0, s2 = 0.0, lo = 1e300, hi = -1e300, rmax = 0.0;
+  double s = 0.0, s2 = 0.0, lo = 1e300, hi = -1e300, rmax = 0.0, r2 = 0.0, rn = 0.0;
   for (long long i = gw; i < n; i += nw) {
     const SgSample &S = samples[i];
     if (lane < SG_NIN) {
@@ -928,7 +928,12 @@ __global__ void k_sg_norm_stats(const SgSample *__restrict__ samples, long long 
       lo = fmin(lo, v);
       hi = fmax(hi, v);
     }
-    if (lane >= 32) rmax = fmax(rmax, fabs((double)S.r[lane - 32]));
+    if (lane >= 32 && ((S.has >> (lane - 32)) & 1u)) {
+      const double r = S.r[lane - 32];
+      rmax = fmax(rmax, fabs(r));
+      r2 += r * r;
+      rn += 1.0;
+    }
   }
   if (lane < SG_NIN) {
     double *o = part + ((long)gw * SG_NIN + lane) * 4;
@@ -937,8 +942,17 @@ __global__ void k_sg_norm_stats(const SgSample *__restrict__ samples, long long 
     o[2] = lo;
     o[3] = hi;
   }
-  for (int off = 16; off > 0; off >>= 1) rmax = fmax(rmax, __shfl_xor(rmax, off, 64));
-  if (lane == 32) part[(long)nw * SG_NIN * 4 + gw] = rmax;
+  for (int off = 16; off > 0; off >>= 1) {
+    rmax = fmax(rmax, __shfl_xor(rmax, off, 64));
+    r2 += __shfl_xor(r2, off, 64);
+    rn += __shfl_xor(rn, off, 64);
+  }
+  if (lane == 32) {
+    double *o = part + (long)nw * SG_NIN * 4 + (long)gw * 3;
+    o[0] = rmax;
+    o[1] = r2;
+    o[2] = rn;
+  }
 }
 
 void launch_sg_norm_stats(const SgSample *samples, long long n, double *partial, hipStream_t st) {

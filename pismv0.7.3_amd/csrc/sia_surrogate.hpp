@@ -144,7 +144,8 @@ void launch_sg_forward_backward(const float *master, const SgTrainBuffers &T, fl
 void launch_sg_adam(float *master, float *m, float *v, const float *grad, float lr, float b1, float b2, float eps,
                     long long t, hipStream_t st);
 // per-wavefront partial statistics: [nwaves][SG_NIN][4] (sum, sum of squares,
-// min, max of each feature) followed by [nwaves] max |r|
+// min, max of each feature) followed by [nwaves][3] (max |r|, sum r^2, count)
+// over the cells with cold levels
 constexpr int SG_STAT_WAVES = 2048;
 void launch_sg_norm_stats(const SgSample *samples, long long n, double *partial, hipStream_t st);
 

@@ -81,6 +81,7 @@ SIGNATURES = {
     "sia_surrogate_counts": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
 }
 
+N_EVENTS = 8  # SIA_EV_COUNT
 SG_NPARAM = 58144
 SG_KIN = 64
 SG_SAMPLE_DTYPE = np.dtype([("f", "<f4", 36), ("r", "<f4", 32), ("share", "<f4", 32), ("has", "<u4"),
@@ -264,10 +265,10 @@ class SIAFD:
 
     def profiling_read(self):
         """{event name: (total ms, launches)} since the last read."""
-        ms = (C.c_double * 6)()
-        cnt = (C.c_int64 * 6)()
+        ms = (C.c_double * N_EVENTS)()
+        cnt = (C.c_int64 * N_EVENTS)()
         self._check(lib().sia_profiling_read(self.h, ms, cnt))
-        return {lib().sia_event_name(e).decode(): (ms[e], cnt[e]) for e in range(6)}
+        return {lib().sia_event_name(e).decode(): (ms[e], cnt[e]) for e in range(N_EVENTS)}
 
     def step(self, geom, E, fast=False, t_end=0.0, t_dev=None, dt_dev=None):
         self._check(lib().sia_step(self.h, C.byref(geom), ptr(E), 1 if fast else 0, t_end,
