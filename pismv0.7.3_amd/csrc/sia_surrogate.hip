@@ -19,6 +19,9 @@ namespace pism_sia {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+// two consecutive levels of a column (columns start at any 8-byte boundary)
+typedef double d2u __attribute__((ext_vector_type(2), aligned(8)));
+typedef float f2a __attribute__((ext_vector_type(2), aligned(8)));
 
 // ---------------------------------------------------------------- helpers
 // FP32 -> BF16 round to nearest even (finite inputs)
@@ -86,6 +89,35 @@ __device__ __forceinline__ double half_sum(double v, int half) {
   const double s = sg_scan(v);
   const double a = sg_lane(s, 31), b = sg_lane(s, 63);
   return half ? b - a : a;
+}
+
+// the same with five DPP steps: row scans, then row 0 (2) added into row 1 (3)
+__device__ __forceinline__ double half_total(double v, int half) {
+  v += sg_dpp<0x111, 0xf, true>(v);
+  v += sg_dpp<0x112, 0xf, true>(v);
+  v += sg_dpp<0x114, 0xf, true>(v);
+  v += sg_dpp<0x118, 0xf, true>(v);
+  v += sg_dpp<0x142, 0xa, false>(v);
+  return half ? sg_lane(v, 63) : sg_lane(v, 31);
+}
+
+// 1 / x: hardware reciprocal estimate refined by two Newton steps (about 1 ulp)
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
+// exp(a) = 2^n 2^f with n = rint(a log2 e) and |f| <= 1/2 evaluated by the
+// FP32 hardware exp2: relative error about 1e-7 for |a| < 700 (the surrogate's
+// physics baseline only; the training targets use the FP64 exp)
+__device__ __forceinline__ double exp_split(double a) {
+  const double t = a * 1.4426950408889634;
+  const double n = __builtin_rint(t);
+  const float f = (float)(t - n);
+  return __builtin_ldexp((double)__builtin_amdgcn_exp2f(f), (int)n);
 }
 
 // ---------------------------------------------------------------- classification
@@ -203,21 +235,6 @@ __device__ __forceinline__ double ipow_nm1(const DevParams &P, double s) {
 }
 __device__ __forceinline__ double pow_n(const DevParams &P, double p) { return ipow_nm1(P, p) * p; }
 
-__device__ __forceinline__ int cell_of(double zk, double s32) {
-  const int c = (int)(zk * s32);
-  return c > SG_NCELL - 1 ? SG_NCELL - 1 : c;
-}
-
-// first level k in [0, ks] with cell_of(z_k) >= target (ks + 1 if none)
-__device__ __forceinline__ int first_level_of_cell(const double *zs, int ks, double s32, int target) {
-  int lo = 0, hi = ks + 1;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (cell_of(zs[mid], s32) >= target) hi = mid; else lo = mid + 1;
-  }
-  return lo;
-}
-
 // A wavefront owns 16 consecutive list entries. Lane c prepares entry c (2-D
 // inputs, kBelowHeight), so the setup of all 16 is in flight at once. The
 // entries are then taken in pairs: all 64 lanes stage the averaged enthalpy
@@ -229,37 +246,88 @@ __device__ __forceinline__ int first_level_of_cell(const double *zs, int ks, dou
 // W_k = d_k (dz_k [k >= 1] + dz_{k+1} [k < ks]) / 2 + d_ks^2 / 2 [k == ks]
 // (SIAFD.cc:677-684 regrouped per delta_k), G_k = W_k p_k^n.
 constexpr int SG_FEAT_COLS = 16;
+// 128-level batches of a pair's enthalpy loads issued together
+constexpr int SG_STAGE_BATCHES = 3;
+// floats of a wavefront's output rows: F [16][36], CQb [16][32], CSt [16] (FP64)
+constexpr int SG_FEAT_OUT = SG_FEAT_COLS * (SG_FSTRIDE + SG_NCELL) + 2 * SG_FEAT_COLS;
 
 // buckets of the level search: bucket b holds the first level with z >= b Lz / SG_NBUCKET
-constexpr int SG_NBUCKET = 1024;
+constexpr int SG_NBUCKET = 256;
 
 // first level k in [0, ks] with z_k * s32 >= m (ks + 1 if none), i.e.
 // cell_of(z_k) >= m for m <= 31: start at the bucket of height m / s32 and
 // step to the exact answer (0-2 steps for equal or quadratic spacing)
-__device__ __forceinline__ int cell_first_level(const double *zs, const int *bucket, double bscale, int ks,
+// kbelow over the interleaved (z, half spacing) table
+__device__ __forceinline__ int kbelow_zsk(const double2 *zsk, int Mz, double Lz, double height) {
+  if (height < 0.0 - 1.0e-6 || height > Lz + 1.0e-6) return -1;
+  int ilo = 0, ihi = Mz - 1;
+  while (ihi > ilo + 1) {
+    const int i = (ihi + ilo) >> 1;
+    if (zsk[i].x > height) ihi = i; else ilo = i;
+  }
+  return ilo;
+}
+
+__device__ __forceinline__ int cell_first_level(const double2 *zsk, const int *bucket, double bscale, int ks,
                                                 double s32, int m) {
   const double hgt = (double)m / s32;
   int b = (int)(hgt * bscale);
   b = b < 0 ? 0 : (b > SG_NBUCKET - 1 ? SG_NBUCKET - 1 : b);
   int k = min(bucket[b], ks + 1);
-  while (k > 0 && zs[k - 1] * s32 >= (double)m) k--;
-  while (k <= ks && zs[k] * s32 < (double)m) k++;
+  while (k > 0 && zsk[k - 1].x * s32 >= (double)m) k--;
+  while (k <= ks && zsk[k].x * s32 < (double)m) k++;
   return k;
 }
 
-template <bool TARGETS>
+// p^N for a compile-time N in {2, 3, 4}
+template <int N>
+__device__ __forceinline__ double pown(double p) {
+  const double p2 = p * p;
+  return N == 2 ? p2 : (N == 3 ? p2 * p : p2 * p2);
+}
+
+// Inference pass over the levels of cell m (levels [klo, khi)) of one column.
+// x_k = T_pa - T_ref = E_k / c_i + T0 - T_ref + beta p_k, staged without the
+// beta p_k term as y_k; cold iff x_k < T_melt - T_ref (E_k < E_cts, since
+// E - E_cts = c_i (x - (T_melt - T_ref))). p_k = p_air + rho g (thk - z_k).
+// Sums are raw (not scaled by thk^2 p_base^n): g0 = sum G, g1 = sum G x,
+// g2 = sum G x^2 over the cold levels, st = exact temperate sum of G A.
+template <int NP>
+__device__ __forceinline__ void sg_cell_sums(const DevParams &P, const double2 *zsk, const float *yb, int klo,
+                                             int khi, int ks, double thk, double sk_top, double pc, double rhog,
+                                             double x_cts, double ta, double tcap, double &g0, double &g1,
+                                             double &g2, double &st) {
+#pragma unroll 2
+  for (int k = klo; k < khi; k++) {
+    const double2 zk = zsk[k];  // z_k, (dz_k [k >= 1] + dz_{k+1}) / 2
+    const double d = thk - zk.x;
+    const double p = __builtin_fma(-rhog, zk.x, pc);
+    const double W = d * ((k == ks) ? sk_top : zk.y);
+    const double G = W * pown<NP>(p);
+    const double x = __builtin_fma(P.beta, p, (double)yb[k]);
+    const double Gc = (x < x_cts) ? G : 0.0;
+    g0 += Gc;
+    const double gx = Gc * x;
+    g1 += gx;
+    g2 = __builtin_fma(gx, x, g2);
+    // temperate: A_T0 (1 + wf_coeff min(omega, wf_limit)), omega = c_i (x - x_cts) / L
+    const double om = fmin(x - x_cts, tcap);
+    st = __builtin_fma(G - Gc, __builtin_fma(ta, om, P.A_T0), st);
+  }
+}
+
+template <bool TARGETS, int NP>
 __global__ __launch_bounds__(256) void k_sg_features(DevParams P, SgColumnArgs C, SgFeatureArgs F) {
-  // LDS: z levels [Mz], half spacings Sk[Mz], then per wavefront the two staged
-  // enthalpy profiles (FP32 in inference, FP64 for the exact training targets)
+  // LDS: (z_k, (dz_k [k >= 1] + dz_{k+1}) / 2) per level, then per wavefront
+  // the two staged profiles: FP64 enthalpy for the exact training targets, in
+  // inference FP32 y_k = E_k / c_i + T0 - T_ref (x_k without the beta p_k term)
   typedef typename std::conditional<TARGETS, double, float>::type EType;
-  extern __shared__ double zs[];
+  extern __shared__ double2 zsk[];
   const int Mz = P.Mz;
-  double *Sk = zs + Mz;  // (dz_k [k >= 1] + dz_{k+1}) / 2
   __shared__ int bucket[SG_NBUCKET];
   for (int k = threadIdx.x; k < Mz; k += blockDim.x) {
     const double zk = C.z[k];
-    zs[k] = zk;
-    Sk[k] = 0.5 * (((k >= 1) ? zk - C.z[k - 1] : 0.0) + ((k + 1 < Mz) ? C.z[k + 1] - zk : 0.0));
+    zsk[k] = make_double2(zk, 0.5 * (((k >= 1) ? zk - C.z[k - 1] : 0.0) + ((k + 1 < Mz) ? C.z[k + 1] - zk : 0.0)));
   }
   const double bscale = (double)SG_NBUCKET / C.z[Mz - 1];
   for (int b = threadIdx.x; b < SG_NBUCKET; b += blockDim.x) {
@@ -275,13 +343,24 @@ __global__ __launch_bounds__(256) void k_sg_features(DevParams P, SgColumnArgs C
   __syncthreads();
   const int nwb = blockDim.x >> 6;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  EType *ebw = reinterpret_cast<EType *>(zs + 2 * Mz) + (long)wv * 2 * Mz;
+  const int Mzp = (Mz + 1) & ~1;  // even stride: 8-byte aligned level pairs
+  EType *ebw = reinterpret_cast<EType *>(zsk + Mz) + (long)wv * 2 * Mzp;
+  // inference: the chunk's outputs gather in LDS and leave in one coalesced
+  // flush per 16 entries (no global store between the pairs of a chunk)
+  float *oF = reinterpret_cast<float *>(reinterpret_cast<EType *>(zsk + Mz) + (long)nwb * 2 * Mzp) +
+              (long)wv * SG_FEAT_OUT;
+  float *oQ = oF + SG_FEAT_COLS * SG_FSTRIDE;
+  double *oS = reinterpret_cast<double *>(oQ + SG_FEAT_COLS * SG_NCELL);
   const int n = *F.n_list;
   const int stride = TARGETS ? F.stride : 1;
   const int nsel = (n + stride - 1) / stride;
   const long long base_sample = TARGETS ? *F.n_samples : 0;
   const double T_ref = 263.15;
   const double rhog = P.rho_i * P.g, T_off = P.T0 + P.T_melt - T_ref;
+  const double y_off = P.T0 - T_ref, x_cts = P.T_melt - T_ref;
+  // temperate softness A_T0 (1 + wf_coeff min(omega, wf_limit)) with omega = (c_i / L) (x - x_cts)
+  const double ta = P.A_T0 * P.wf_coeff * P.ci * P.inv_L, tcap = P.wf_limit / (P.ci * P.inv_L);
+  const double QR_cold = P.Q_cold / P.R, QR_warm = P.Q_warm / P.R;
   const int hh = lane >> 5, m = lane & (SG_NCELL - 1);
   for (int t0 = (blockIdx.x * nwb + wv) * SG_FEAT_COLS; t0 < nsel; t0 += gridDim.x * nwb * SG_FEAT_COLS) {
     // ---- setup: lane c < 16 prepares entry t0 + c (c_ks = -1: no entry)
@@ -297,7 +376,7 @@ __global__ __launch_bounds__(256) void k_sg_features(DevParams P, SgColumnArgs C
       const double alpha = sqrt(hxs * hxs + hys * hys);
       c_lgs = log10(alpha) + 2.0;  // the slope input of the network
       c_Cf = P.e * (0.5 * (C.theta[q] + C.theta[q2])) * 2.0 * ipow_nm1(P, alpha);
-      c_ks = kbelow(zs, Mz, P.Lz, thk);  // >= 1 for listed columns
+      c_ks = kbelow_zsk(zsk, Mz, P.Lz, thk);  // >= 1 for listed columns
       c_thk = thk;
       c_q = q;
       c_q2 = q2;
@@ -314,28 +393,43 @@ __global__ __launch_bounds__(256) void k_sg_features(DevParams P, SgColumnArgs C
         Ep[h][0] = C.E + (long)lane_value_i(c_q, cc + h) * Mz;
         Ep[h][1] = C.E + (long)lane_value_i(c_q2, cc + h) * Mz;
       }
+      // lane l stages levels k, k + 1 with k = kb + 128 b + 2 l (16-byte loads;
+      // k <= ks <= Mz - 2 keeps k + 1 inside the column, level ks + 1 is never
+      // read); the loads of SG_STAGE_BATCHES x 128 levels are all in flight
+      // before the first is used
       const int ksmax = max(ksp[0], ksp[1]);
-      for (int kb = 0; kb <= ksmax; kb += 128) {
-        double ea[2][2], ebv[2][2];
+      const double hci = 0.5 * P.inv_ci;
+      for (int kb = 0; kb <= ksmax; kb += 128 * SG_STAGE_BATCHES) {
+        d2u ea[SG_STAGE_BATCHES][2], ebv[SG_STAGE_BATCHES][2];
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
+        for (int bt = 0; bt < SG_STAGE_BATCHES; bt++) {
+          const int k = kb + 128 * bt + 2 * lane;
 #pragma unroll
-          for (int u = 0; u < 2; u++) {
-            const int k = kb + 64 * u + lane;
-            ea[h][u] = 0.0;
-            ebv[h][u] = 0.0;
+          for (int h = 0; h < 2; h++) {
             if (k <= ksp[h]) {
-              ea[h][u] = Ep[h][0][k];
-              ebv[h][u] = Ep[h][1][k];
+              ea[bt][h] = *reinterpret_cast<const d2u *>(Ep[h][0] + k);
+              ebv[bt][h] = *reinterpret_cast<const d2u *>(Ep[h][1] + k);
             }
           }
         }
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
+        for (int bt = 0; bt < SG_STAGE_BATCHES; bt++) {
+          const int k = kb + 128 * bt + 2 * lane;
 #pragma unroll
-          for (int u = 0; u < 2; u++) {
-            const int k = kb + 64 * u + lane;
-            if (k <= ksp[h]) ebw[h * Mz + k] = (EType)(0.5 * (ea[h][u] + ebv[h][u]));
+          for (int h = 0; h < 2; h++) {
+            if (k <= ksp[h]) {
+              if (TARGETS) {
+                d2u e;
+                e.x = 0.5 * (ea[bt][h].x + ebv[bt][h].x);
+                e.y = 0.5 * (ea[bt][h].y + ebv[bt][h].y);
+                *reinterpret_cast<d2u *>(ebw + h * Mzp + k) = e;
+              } else {
+                f2a y;
+                y.x = (float)__builtin_fma(ea[bt][h].x + ebv[bt][h].x, hci, y_off);
+                y.y = (float)__builtin_fma(ea[bt][h].y + ebv[bt][h].y, hci, y_off);
+                *reinterpret_cast<f2a *>(ebw + h * Mzp + k) = y;
+              }
+            }
           }
         }
       }
@@ -351,70 +445,75 @@ __global__ __launch_bounds__(256) void k_sg_features(DevParams P, SgColumnArgs C
       const int s = hh ? lane_value_i(c_s, cc + 1) : lane_value_i(c_s, cc);
       // cell m: levels [klo, khi); khi is the next lane's klo
       const double s32 = (double)SG_NCELL / (live ? thk : 1.0);
-      const int klo = live ? cell_first_level(zs, bucket, bscale, ks, s32, m) : 0;
+      const int klo = live ? cell_first_level(zsk, bucket, bscale, ks, s32, m) : 0;
       int khi = __shfl_down(klo, 1, 64);
       if (m == SG_NCELL - 1) khi = ks + 1;
       if (!live) khi = 0;
-      // G is accumulated relative to thk^2 p_base^n
-      const double gs = live ? thk * thk * pow_n(P, P.p_air + rhog * thk) : 1.0;
-      const double inv_gs = 1.0 / gs;
-      const EType *eb = ebw + hh * Mz;
-      double st = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0, qref = 0.0, dsum = 0.0;
+      const EType *eb = ebw + hh * Mzp;
+      double st = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0, qref = 0.0, dsum = 0.0, gs = 1.0;
+      if (!TARGETS) {
+        // W_ks = d_ks (dz_ks + d_ks) / 2 (SIAFD.cc:677-684 regrouped per delta_k)
+        const double zt = zsk[live ? ks : 0].x;
+        const double sk_top = 0.5 * ((ks >= 1 ? zt - zsk[ks - 1].x : 0.0) + (thk - zt));
+        sg_cell_sums<NP>(P, zsk, reinterpret_cast<const float *>(eb), klo, khi, ks, thk, sk_top,
+                         P.p_air + rhog * thk, rhog, x_cts, ta, tcap, g0, g1, g2, st);
+      } else {
+        // G is accumulated relative to thk^2 p_base^n
+        gs = live ? thk * thk * pow_n(P, P.p_air + rhog * thk) : 1.0;
+        const double inv_gs = 1.0 / gs;
 #pragma unroll 4
-      for (int k = klo; k < khi; k++) {
-        const double zk = zs[k];
-        const double d = thk - zk;  // >= 0 up to ks
-        const double p = P.p_air + rhog * d;
-        const double W = (k < ks) ? d * Sk[k] : 0.5 * d * (((k >= 1) ? zk - zs[k - 1] : 0.0) + d);
-        const double G = W * pow_n(P, p) * inv_gs;
-        const double Ek = (double)eb[k];
-        // default enthalpy converter: T_m = T_melt - beta p, E_cts = c_i (T_m - T0)
-        // (enthalpyConverter.cc:128-143)
-        const double Tm = P.T_melt - P.beta * p;
-        const double Ects = P.ci * (Tm - P.T0);
-        if (Ek < Ects) {  // cold branch of GPBLD (flowlaws.cc:212-214): T_pa - 263.15
-          const double x = Ek * P.inv_ci + T_off - Tm;
-          g0 += G;
-          g1 += G * x;
-          g2 += G * x * x;
-          if (TARGETS) {
+        for (int k = klo; k < khi; k++) {
+          const double zk = zsk[k].x;
+          const double d = thk - zk;  // >= 0 up to ks
+          const double p = P.p_air + rhog * d;
+          const double W = (k < ks) ? d * zsk[k].y : 0.5 * d * (((k >= 1) ? zk - zsk[k - 1].x : 0.0) + d);
+          const double G = W * pow_n(P, p) * inv_gs;
+          const double Ek = (double)eb[k];
+          // default enthalpy converter: T_m = T_melt - beta p, E_cts = c_i (T_m - T0)
+          // (enthalpyConverter.cc:128-143)
+          const double Tm = P.T_melt - P.beta * p;
+          const double Ects = P.ci * (Tm - P.T0);
+          if (Ek < Ects) {  // cold branch of GPBLD (flowlaws.cc:212-214): T_pa - 263.15
+            const double x = Ek * P.inv_ci + T_off - Tm;
+            g0 += G;
+            g1 += G * x;
+            g2 += G * x * x;
             const double A = softness_pb(P, ec_pressure_adjusted_temperature(P, Ek, p));
             qref += G * A;
             dsum += G * A;
+          } else {  // temperate branch (flowlaws.cc:215-219), omega = (E - E_cts) / L
+            const double omega = fmin((Ek - Ects) * P.inv_L, P.wf_limit);
+            st += G * (P.A_T0 * (1.0 + P.wf_coeff * omega));
           }
-        } else {  // temperate branch (flowlaws.cc:215-219), omega = (E - E_cts) / L
-          const double omega = fmin((Ek - Ects) * P.inv_L, P.wf_limit);
-          st += G * (P.A_T0 * (1.0 + P.wf_coeff * omega));
         }
       }
-      const double S_t = half_sum(st, hh) * gs;
+      const double S_t = TARGETS ? half_sum(st, hh) * gs : half_total(st, hh);
       const bool has = live && g0 > 0.0;
       double Qb = 0.0, feat = 2.0;
       if (has) {
-        const double inv_g0 = 1.0 / g0;
+        const double inv_g0 = TARGETS ? 1.0 / g0 : rcp_nr(g0);
         const double mx = g1 * inv_g0;
         const double Tbar = T_ref + mx;
         const double var = fmax(g2 * inv_g0 - mx * mx, 0.0);
-        const double inv_T = 1.0 / Tbar;
+        const double inv_T = TARGETS ? 1.0 / Tbar : rcp_nr(Tbar);
         const bool cold = Tbar < P.T_crit;
-        const double QR = (cold ? P.Q_cold : P.Q_warm) / P.R;
+        const double QR = cold ? QR_cold : QR_warm;
         const double kk = QR * inv_T * inv_T;
         // Paterson-Budd softness at the cell mean (flowlaws.cc:90-95)
-        const double Ab = (cold ? P.A_cold : P.A_warm) * exp(-QR * inv_T);
+        const double Ab = (cold ? P.A_cold : P.A_warm) * (TARGETS ? exp(-QR * inv_T) : exp_split(-QR * inv_T));
         Qb = g0 * gs * Ab * (1.0 + 0.5 * (kk * kk - 2.0 * kk * inv_T) * var);
         feat = (Tbar - 250.0) * (1.0 / 15.0);
       }
       const int t = t0 + cc + hh;
       if (!TARGETS) {
         if (live) {
-          const int slot = t;
-          F.F[(long)slot * SG_FSTRIDE + m] = (float)feat;
-          F.CQb[(long)slot * SG_NCELL + m] = (float)(Cf * Qb);
-          if (m == 0) {
-            F.F[(long)slot * SG_FSTRIDE + 32] = (float)(thk / 1000.0);
-            F.F[(long)slot * SG_FSTRIDE + 33] = (float)lgs;
-            F.CSt[slot] = Cf * S_t;
+          const int r = cc + hh;
+          oF[r * SG_FSTRIDE + m] = (float)feat;
+          oQ[r * SG_NCELL + m] = (float)(Cf * Qb);
+          if (m < SG_FSTRIDE - SG_NCELL) {
+            oF[r * SG_FSTRIDE + SG_NCELL + m] = m == 0 ? (float)(thk / 1000.0) : (m == 1 ? (float)lgs : 0.0f);
           }
+          if (m == 0) oS[r] = Cf * S_t;
         }
       } else {
         const double qcell = qref * gs;  // exact cold sum of the cell
@@ -443,12 +542,29 @@ __global__ __launch_bounds__(256) void k_sg_features(DevParams P, SgColumnArgs C
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+    if (!TARGETS) {
+      // rows t0 .. t0 + nr - 1 of F (36 floats), CQb (32 floats), CSt
+      const int nr = min(SG_FEAT_COLS, nsel - t0);
+      const float4 *sF = reinterpret_cast<const float4 *>(oF);
+      const float4 *sQ = reinterpret_cast<const float4 *>(oQ);
+      float4 *dF = reinterpret_cast<float4 *>(F.F + (long)t0 * SG_FSTRIDE);
+      float4 *dQ = reinterpret_cast<float4 *>(F.CQb + (long)t0 * SG_NCELL);
+      for (int v = lane; v < nr * (SG_FSTRIDE / 4); v += 64) dF[v] = sF[v];
+      for (int v = lane; v < nr * (SG_NCELL / 4); v += 64) dQ[v] = sQ[v];
+      if (lane < nr) F.CSt[t0 + lane] = oS[lane];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
   }
 }
 
-// dynamic LDS of k_sg_features: z levels and half spacings + per wavefront two enthalpy profiles
+// dynamic LDS of k_sg_features: (z, half spacing) per level + per wavefront two
+// staged profiles (+ in inference the chunk's output rows)
 static size_t sg_features_lds(int Mz, int waves, bool targets) {
-  return 2 * sizeof(double) * (size_t)Mz + (size_t)waves * 2 * Mz * (targets ? sizeof(double) : sizeof(float));
+  return sizeof(double2) * (size_t)Mz +
+         (size_t)waves * 2 * ((Mz + 1) & ~1) * (targets ? sizeof(double) : sizeof(float)) +
+         (targets ? 0 : (size_t)waves * SG_FEAT_OUT * sizeof(float));
 }
 
 void launch_sg_features(const DevParams &P, const SgColumnArgs &C, const SgFeatureArgs &F, bool targets,
@@ -458,9 +574,13 @@ void launch_sg_features(const DevParams &P, const SgColumnArgs &C, const SgFeatu
   const int nb = 8192 / waves;  // persistent: 8192 wavefronts walk the list, 16 entries at a time
   const size_t lds = sg_features_lds(P.Mz, waves, targets);
   if (targets) {
-    hipLaunchKernelGGL(k_sg_features<true>, dim3(nb), dim3(64 * waves), lds, st, P, C, F);
+    hipLaunchKernelGGL((k_sg_features<true, 3>), dim3(nb), dim3(64 * waves), lds, st, P, C, F);
+  } else if (P.nm1_int == 1) {
+    hipLaunchKernelGGL((k_sg_features<false, 2>), dim3(nb), dim3(64 * waves), lds, st, P, C, F);
+  } else if (P.nm1_int == 2) {
+    hipLaunchKernelGGL((k_sg_features<false, 3>), dim3(nb), dim3(64 * waves), lds, st, P, C, F);
   } else {
-    hipLaunchKernelGGL(k_sg_features<false>, dim3(nb), dim3(64 * waves), lds, st, P, C, F);
+    hipLaunchKernelGGL((k_sg_features<false, 4>), dim3(nb), dim3(64 * waves), lds, st, P, C, F);
   }
 }
 
@@ -518,24 +638,31 @@ constexpr int SG_GTAB = 1024;
 constexpr float SG_GLO = -8.0f, SG_GSCALE = (float)SG_GTAB / 16.0f;
 typedef float f32x8 __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ float gelu_tab(const float2 *tab, float x) {
-  const float u = (x - SG_GLO) * SG_GSCALE;
-  const int i = min(max((int)u, 0), SG_GTAB - 1);
+// GELU(a + b) from the table position u = a S + (b - lo) S; ub holds (b - lo) S
+__device__ __forceinline__ float gelu_tab(const float2 *tab, float a, float ub) {
+  const float u = __builtin_fmaf(a, SG_GSCALE, ub);
+  const int i = min(max((int)u, 0), SG_GTAB - 1);  // one v_med3_i32
   const float2 e = tab[i];
-  return e.x + e.y * (u - (float)i);
+  return __builtin_fmaf(e.y, u - (float)i, e.x);
 }
 
 // bias + GELU of 8 accumulator tiles -> the 4 B-operand chunks of the next layer
-__device__ __forceinline__ void act_to_chunks(const f32x4 *acc, const float *bias, const float2 *tab, int g,
+// (ub: the layer's biases as table offsets)
+__device__ __forceinline__ void act_to_chunks(const f32x4 *acc, const float *ub, const float2 *tab, int g,
                                               bf16x8 *h) {
 #pragma unroll
   for (int c = 0; c < 4; c++) {
     f32x8 v;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      v[i] = gelu_tab(tab, acc[2 * c][i] + bias[16 * (2 * c) + 4 * g + i]);
-      v[4 + i] = gelu_tab(tab, acc[2 * c + 1][i] + bias[16 * (2 * c + 1) + 4 * g + i]);
-    }
+    const float4 b0 = *reinterpret_cast<const float4 *>(ub + 16 * (2 * c) + 4 * g);
+    const float4 b1 = *reinterpret_cast<const float4 *>(ub + 16 * (2 * c + 1) + 4 * g);
+    v[0] = gelu_tab(tab, acc[2 * c][0], b0.x);
+    v[1] = gelu_tab(tab, acc[2 * c][1], b0.y);
+    v[2] = gelu_tab(tab, acc[2 * c][2], b0.z);
+    v[3] = gelu_tab(tab, acc[2 * c][3], b0.w);
+    v[4] = gelu_tab(tab, acc[2 * c + 1][0], b1.x);
+    v[5] = gelu_tab(tab, acc[2 * c + 1][1], b1.y);
+    v[6] = gelu_tab(tab, acc[2 * c + 1][2], b1.z);
+    v[7] = gelu_tab(tab, acc[2 * c + 1][3], b1.w);
     h[c] = __builtin_convertvector(v, bf16x8);  // round to nearest even
   }
 }
@@ -574,7 +701,8 @@ constexpr int SG_MLP_WAVES = 16;
 
 __global__ __launch_bounds__(64 * SG_MLP_WAVES) void k_sg_mlp(DevParams P, SgMlpArgs A) {
   extern __shared__ unsigned short wfrag[];  // [SG_NFRAG][64][8], then the GELU table
-  __shared__ float bias[4 * SG_WIDTH + SG_NOUT];
+  // hidden biases as GELU-table offsets (b - lo) S, then the raw output biases
+  __shared__ __align__(16) float bias[4 * SG_WIDTH + SG_NOUT];
   __shared__ double wmax[SG_MLP_WAVES];
   __shared__ int wfall[SG_MLP_WAVES];
   float2 *gtab = reinterpret_cast<float2 *>(wfrag + SG_NFRAG * 512);
@@ -583,10 +711,10 @@ __global__ __launch_bounds__(64 * SG_MLP_WAVES) void k_sg_mlp(DevParams P, SgMlp
     uint4 *dst = reinterpret_cast<uint4 *>(wfrag);
     for (int e = threadIdx.x; e < SG_NFRAG * 64; e += blockDim.x) dst[e] = src[e];
     for (int e = threadIdx.x; e < SG_WIDTH; e += blockDim.x) {
-      bias[e] = A.bias[SG_B1 + e];
-      bias[SG_WIDTH + e] = A.bias[SG_B2 + e];
-      bias[2 * SG_WIDTH + e] = A.bias[SG_B3 + e];
-      bias[3 * SG_WIDTH + e] = A.bias[SG_B4 + e];
+      bias[e] = (A.bias[SG_B1 + e] - SG_GLO) * SG_GSCALE;
+      bias[SG_WIDTH + e] = (A.bias[SG_B2 + e] - SG_GLO) * SG_GSCALE;
+      bias[2 * SG_WIDTH + e] = (A.bias[SG_B3 + e] - SG_GLO) * SG_GSCALE;
+      bias[3 * SG_WIDTH + e] = (A.bias[SG_B4 + e] - SG_GLO) * SG_GSCALE;
     }
     for (int e = threadIdx.x; e < SG_NOUT; e += blockDim.x) bias[4 * SG_WIDTH + e] = A.bias[SG_B5 + e];
     for (int e = threadIdx.x; e < SG_GTAB; e += blockDim.x) {
