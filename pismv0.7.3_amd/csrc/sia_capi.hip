@@ -3,6 +3,7 @@
 // continuity on one HIP stream.
 #include "../../include/pism_sia.h"
 #include "../../include/sia_surrogate.h"
+#include "../../include/sia_ensemble.h"
 #include "sia_kernels.hpp"
 #include "sia_surrogate.hpp"
 
@@ -1116,5 +1117,403 @@ int sia_surrogate_counts(sia_handle *h, int64_t counts[3]) {
   for (int k = 0; k < 3; k++) counts[k] = c[k];
   return SIA_OK;
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- ensembles
+struct sia_ensemble {
+  int device = 0, n = 0, law = 0, max_mz = 0, cpw = 16;
+  std::vector<sia_handle *> members;
+  std::vector<sia_geometry> geom;
+  std::vector<EnsMember> table;        // host copy; A.dmax_part / n_part follow cpw
+  EnsMember *d_table = nullptr;
+  std::vector<int> pt_start, col_start[3];
+  int *d_pt_start = nullptr, *d_col_start[3] = {nullptr, nullptr, nullptr};
+  double *d_t = nullptr, *d_dt = nullptr, *d_part = nullptr;
+  int *d_err_acc = nullptr;
+  std::vector<size_t> part_off;        // member offsets into d_part (sized for 8 columns per wave)
+  hipStream_t own = nullptr, user = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_steps = 0, graph_cpw = 0;
+  std::string last_error;
+};
+
+namespace {
+
+const int ENS_CPW[3] = {8, 16, 32};
+
+int cpw_slot(int cpw) { return cpw == 8 ? 0 : (cpw == 32 ? 2 : 1); }
+
+int ens_err(sia_ensemble *e, int code, const std::string &msg) {
+  if (e) e->last_error = msg;
+  return code;
+}
+
+#define ENS_TRY(e, call)                                                                       \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess) return ens_err((e), SIA_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// member table for the current tile width, uploaded on the ensemble's stream
+int ens_upload(sia_ensemble *e) {
+  const int c = cpw_slot(e->cpw);
+  for (int i = 0; i < e->n; i++) {
+    e->table[i].A.dmax_part = e->d_part + e->part_off[i];
+    e->table[i].n_part = e->col_start[c][i + 1] - e->col_start[c][i];
+  }
+  ENS_TRY(e, hipMemcpyAsync(e->d_table, e->table.data(), sizeof(EnsMember) * e->n, hipMemcpyHostToDevice, e->own));
+  ENS_TRY(e, hipStreamSynchronize(e->own));
+  return SIA_OK;
+}
+
+EnsLaunch ens_launch(const sia_ensemble *e) {
+  const int c = cpw_slot(e->cpw);
+  EnsLaunch L{};
+  L.table = e->d_table;
+  L.pt_start = e->d_pt_start;
+  L.col_start = e->d_col_start[c];
+  L.nm = e->n;
+  L.pt_blocks = e->pt_start[e->n];
+  L.col_blocks = e->col_start[c][e->n];
+  L.cpw = e->cpw;
+  L.law = e->law;
+  L.max_mz = e->max_mz;
+  return L;
+}
+
+void ens_drop_graph(sia_ensemble *e) {
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->exec = nullptr;
+  e->graph = nullptr;
+  e->graph_steps = 0;
+}
+
+void ens_mark_members(sia_ensemble *e) {
+  for (int i = 0; i < e->n; i++) {
+    sia_handle *h = e->members[i];
+    h->have_flux = true;
+    h->have_3d = false;
+    h->last_ub = nullptr;
+    h->last_vb = nullptr;
+    h->last_mask = e->geom[i].mask;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int sia_ensemble_create(sia_handle *const *members, const sia_geometry *geometry, const double *const *enthalpy,
+                        const double *t_end, int n, sia_ensemble **out) {
+  if (!members || !geometry || !enthalpy || !out || n < 1) return SIA_ERR_ARGUMENT;
+  *out = nullptr;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return SIA_ERR_HIP;
+  sia_ensemble *e = new (std::nothrow) sia_ensemble();
+  if (!e) return SIA_ERR_OUT_OF_MEMORY;
+  e->device = dev;
+  e->n = n;
+  e->members.assign(members, members + n);
+  e->geom.assign(geometry, geometry + n);
+  e->table.resize(n);
+  e->pt_start.assign(n + 1, 0);
+  for (int c = 0; c < 3; c++) e->col_start[c].assign(n + 1, 0);
+  e->part_off.assign(n, 0);
+  size_t part_total = 0;
+  long long pt_total = 0, col_total[3] = {0, 0, 0};
+  for (int i = 0; i < n; i++) {
+    sia_handle *h = members[i];
+    const sia_geometry &g = geometry[i];
+    if (!h || !enthalpy[i] || !g.land_ice_thickness || !g.surface_altitude || !g.mask || !g.bedrock_altitude) {
+      delete e;
+      return SIA_ERR_ARGUMENT;
+    }
+    if (h->device != dev) {
+      e->last_error = "member " + std::to_string(i) + " lives on another device";
+      sia_ensemble_destroy(e);
+      return SIA_ERR_ARGUMENT;
+    }
+    if (i == 0) e->law = h->params.sia_flow_law;
+    if (h->params.sia_flow_law != e->law ||
+        (law_uses_grain_size(h->params.sia_flow_law) && h->params.compute_grain_size_using_age)) {
+      delete e;
+      return SIA_ERR_CONFIG;
+    }
+    if (!h->bed_ready) {
+      delete e;
+      return SIA_ERR_STATE;
+    }
+    e->max_mz = std::max(e->max_mz, h->grid.Mz);
+    e->pt_start[i] = (int)pt_total;
+    pt_total += ens_point_blocks(h->dp);
+    for (int c = 0; c < 3; c++) {
+      e->col_start[c][i] = (int)col_total[c];
+      col_total[c] += ens_column_blocks(h->dp, ENS_CPW[c]);
+    }
+    e->part_off[i] = part_total;
+    part_total += (size_t)ens_column_blocks(h->dp, 8);
+  }
+  if (pt_total >= 2147483647LL || col_total[0] >= 2147483647LL) {
+    delete e;
+    return SIA_ERR_ARGUMENT;
+  }
+  e->pt_start[n] = (int)pt_total;
+  for (int c = 0; c < 3; c++) e->col_start[c][n] = (int)col_total[c];
+  auto fail = [&](hipError_t err, const char *what) {
+    e->last_error = std::string(what) + ": " + hipGetErrorString(err);
+    sia_ensemble_destroy(e);
+    return SIA_ERR_HIP;
+  };
+  hipError_t err;
+  if ((err = hipStreamCreateWithFlags(&e->own, hipStreamNonBlocking)) != hipSuccess) return fail(err, "stream");
+  if ((err = hipEventCreateWithFlags(&e->ev_in, hipEventDisableTiming)) != hipSuccess ||
+      (err = hipEventCreateWithFlags(&e->ev_out, hipEventDisableTiming)) != hipSuccess) {
+    return fail(err, "events");
+  }
+  if ((err = hipMalloc((void **)&e->d_table, sizeof(EnsMember) * n)) != hipSuccess ||
+      (err = hipMalloc((void **)&e->d_pt_start, sizeof(int) * (n + 1))) != hipSuccess ||
+      (err = hipMalloc((void **)&e->d_t, sizeof(double) * n)) != hipSuccess ||
+      (err = hipMalloc((void **)&e->d_dt, sizeof(double) * n)) != hipSuccess ||
+      (err = hipMalloc((void **)&e->d_err_acc, sizeof(int) * n)) != hipSuccess ||
+      (err = hipMalloc((void **)&e->d_part, sizeof(double) * part_total)) != hipSuccess) {
+    return fail(err, "hipMalloc");
+  }
+  for (int c = 0; c < 3; c++) {
+    if ((err = hipMalloc((void **)&e->d_col_start[c], sizeof(int) * (n + 1))) != hipSuccess) return fail(err, "hipMalloc");
+    if ((err = hipMemcpy(e->d_col_start[c], e->col_start[c].data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice)) !=
+        hipSuccess) {
+      return fail(err, "hipMemcpy");
+    }
+  }
+  if ((err = hipMemcpy(e->d_pt_start, e->pt_start.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice)) !=
+          hipSuccess ||
+      (err = hipMemset(e->d_t, 0, sizeof(double) * n)) != hipSuccess ||
+      (err = hipMemset(e->d_dt, 0, sizeof(double) * n)) != hipSuccess ||
+      (err = hipMemset(e->d_err_acc, 0, sizeof(int) * n)) != hipSuccess) {
+    return fail(err, "hipMemcpy");
+  }
+  for (int i = 0; i < n; i++) {
+    sia_handle *h = members[i];
+    const sia_geometry &g = geometry[i];
+    EnsMember &M = e->table[i];
+    M = EnsMember{};
+    M.P = h->dp;
+    M.P.use_age = 0;
+    M.F = Fields2D{g.land_ice_thickness, g.bedrock_altitude, g.surface_altitude, g.mask};
+    M.topgs = h->topgs;
+    M.maxtl = h->maxtl;
+    M.C2 = h->C2;
+    M.C3 = h->C3;
+    M.C4 = h->C4;
+    M.hx = h->hx;
+    M.hy = h->hy;
+    M.theta = h->theta;
+    M.thks = h->thks;
+    M.A.z = h->d_z;
+    M.A.E = enthalpy[i];
+    M.A.thks = h->thks;
+    M.A.theta = h->theta;
+    M.A.hx = h->hx;
+    M.A.hy = h->hy;
+    M.A.flux = h->flux;
+    M.A.Dstag = h->Dstag;
+    M.A.dmax_bits = h->dmax_bits;
+    M.A.err = h->err;
+    M.M.dt = e->d_dt + i;
+    M.M.H = g.land_ice_thickness;
+    M.M.flux = h->flux;
+    M.M.mask = g.mask;
+    M.M.smb = g.climatic_mass_balance;
+    M.M.bmr = g.basal_melt_rate;
+    M.M.H_new = h->H_tmp;
+    M.H = g.land_ice_thickness;
+    M.mask = g.mask;
+    M.h = g.surface_altitude;
+    M.cfl_bits = h->cfl_bits;
+    M.t = e->d_t + i;
+    M.dt = e->d_dt + i;
+    M.err_acc = e->d_err_acc + i;
+    M.t_end = t_end ? t_end[i] : 0.0;
+  }
+  int rc = ens_upload(e);
+  if (rc) {
+    sia_ensemble_destroy(e);
+    return rc;
+  }
+  *out = e;
+  return SIA_OK;
+}
+
+int sia_ensemble_destroy(sia_ensemble *e) {
+  if (!e) return SIA_OK;
+  if (e->own) (void)hipStreamSynchronize(e->own);
+  ens_drop_graph(e);
+  void *bufs[] = {e->d_table, e->d_pt_start, e->d_col_start[0], e->d_col_start[1], e->d_col_start[2],
+                  e->d_t, e->d_dt, e->d_err_acc, e->d_part};
+  for (void *b : bufs) dfree(b);
+  if (e->ev_in) (void)hipEventDestroy(e->ev_in);
+  if (e->ev_out) (void)hipEventDestroy(e->ev_out);
+  if (e->own) (void)hipStreamDestroy(e->own);
+  delete e;
+  return SIA_OK;
+}
+
+int sia_ensemble_set_stream(sia_ensemble *e, void *hip_stream) {
+  if (!e) return SIA_ERR_ARGUMENT;
+  e->user = (hipStream_t)hip_stream;
+  return SIA_OK;
+}
+
+int sia_ensemble_set_tile(sia_ensemble *e, int cols_per_wave) {
+  if (!e) return SIA_ERR_ARGUMENT;
+  if (cols_per_wave != 8 && cols_per_wave != 16 && cols_per_wave != 32) {
+    return ens_err(e, SIA_ERR_ARGUMENT, "columns per wavefront must be 8, 16 or 32");
+  }
+  if (cols_per_wave == e->cpw) return SIA_OK;
+  // the previous steps may still read the table
+  ENS_TRY(e, hipStreamSynchronize(e->own));
+  e->cpw = cols_per_wave;
+  return ens_upload(e);
+}
+
+int sia_ensemble_step(sia_ensemble *e, int n_steps, int use_graph) {
+  if (!e || n_steps < 0) return SIA_ERR_ARGUMENT;
+  if (n_steps == 0) return SIA_OK;
+  ENS_TRY(e, hipEventRecord(e->ev_in, e->user));
+  ENS_TRY(e, hipStreamWaitEvent(e->own, e->ev_in, 0));
+  const EnsLaunch L = ens_launch(e);
+  if (use_graph) {
+    if (!e->exec || e->graph_steps != n_steps || e->graph_cpw != e->cpw) {
+      ens_drop_graph(e);
+      ENS_TRY(e, hipStreamBeginCapture(e->own, hipStreamCaptureModeThreadLocal));
+      for (int s = 0; s < n_steps; s++) launch_ens_step(L, e->own);
+      hipGraph_t g = nullptr;
+      const hipError_t ce = hipStreamEndCapture(e->own, &g);
+      if (ce != hipSuccess) return ens_err(e, SIA_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+      e->graph = g;
+      ENS_TRY(e, hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+      e->graph_steps = n_steps;
+      e->graph_cpw = e->cpw;
+    }
+    ENS_TRY(e, hipGraphLaunch(e->exec, e->own));
+  } else {
+    for (int s = 0; s < n_steps; s++) launch_ens_step(L, e->own);
+    ENS_TRY(e, hipGetLastError());
+  }
+  ENS_TRY(e, hipEventRecord(e->ev_out, e->own));
+  ENS_TRY(e, hipStreamWaitEvent(e->user, e->ev_out, 0));
+  ens_mark_members(e);
+  return SIA_OK;
+}
+
+int sia_ensemble_autotune(sia_ensemble *e, int steps, int *chosen, double *ms) {
+  if (!e || steps < 1) return SIA_ERR_ARGUMENT;
+  // save every member's geometry, time and dt
+  std::vector<double *> saved(e->n, nullptr);
+  double *t_save = nullptr;
+  auto release = [&]() {
+    for (double *p : saved) dfree(p);
+    dfree(t_save);
+  };
+  ENS_TRY(e, hipEventRecord(e->ev_in, e->user));
+  ENS_TRY(e, hipStreamWaitEvent(e->own, e->ev_in, 0));
+  hipError_t err = hipMalloc((void **)&t_save, sizeof(double) * 2 * e->n);
+  for (int i = 0; err == hipSuccess && i < e->n; i++) {
+    const size_t n2 = (size_t)e->table[i].P.Mx * e->table[i].P.My;
+    err = hipMalloc((void **)&saved[i], sizeof(double) * 3 * n2);
+  }
+  if (err != hipSuccess) {
+    release();
+    return ens_err(e, SIA_ERR_OUT_OF_MEMORY, "autotune: no room to save the ensemble state");
+  }
+  auto copy_state = [&](bool save) -> hipError_t {
+    hipError_t r = hipSuccess;
+    for (int i = 0; r == hipSuccess && i < e->n; i++) {
+      const size_t n2 = (size_t)e->table[i].P.Mx * e->table[i].P.My, b = n2 * sizeof(double);
+      double *f[3] = {e->geom[i].land_ice_thickness, e->geom[i].surface_altitude, e->geom[i].mask};
+      for (int k = 0; r == hipSuccess && k < 3; k++) {
+        r = save ? hipMemcpyAsync(saved[i] + k * n2, f[k], b, hipMemcpyDeviceToDevice, e->own)
+                 : hipMemcpyAsync(f[k], saved[i] + k * n2, b, hipMemcpyDeviceToDevice, e->own);
+      }
+    }
+    if (r == hipSuccess) {
+      r = save ? hipMemcpyAsync(t_save, e->d_t, sizeof(double) * e->n, hipMemcpyDeviceToDevice, e->own)
+               : hipMemcpyAsync(e->d_t, t_save, sizeof(double) * e->n, hipMemcpyDeviceToDevice, e->own);
+    }
+    if (r == hipSuccess) {
+      r = save ? hipMemcpyAsync(t_save + e->n, e->d_dt, sizeof(double) * e->n, hipMemcpyDeviceToDevice, e->own)
+               : hipMemcpyAsync(e->d_dt, t_save + e->n, sizeof(double) * e->n, hipMemcpyDeviceToDevice, e->own);
+    }
+    return r;
+  };
+  hipEvent_t a = nullptr, b = nullptr;
+  double best = 1e300, res[3] = {0, 0, 0};
+  int best_cpw = e->cpw;
+  err = copy_state(true);
+  if (err == hipSuccess) err = hipEventCreate(&a);
+  if (err == hipSuccess) err = hipEventCreate(&b);
+  for (int c = 0; err == hipSuccess && c < 3; c++) {
+    if (sia_ensemble_set_tile(e, ENS_CPW[c]) != SIA_OK) {
+      err = hipErrorUnknown;
+      break;
+    }
+    const EnsLaunch L = ens_launch(e);
+    launch_ens_step(L, e->own);  // warm-up
+    err = hipEventRecord(a, e->own);
+    for (int s = 0; err == hipSuccess && s < steps; s++) launch_ens_step(L, e->own);
+    if (err == hipSuccess) err = hipEventRecord(b, e->own);
+    if (err == hipSuccess) err = hipEventSynchronize(b);
+    float t = 0.0f;
+    if (err == hipSuccess) err = hipEventElapsedTime(&t, a, b);
+    res[c] = (double)t / steps;
+    if (res[c] < best) {
+      best = res[c];
+      best_cpw = ENS_CPW[c];
+    }
+    if (err == hipSuccess) err = copy_state(false);
+  }
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+  if (err == hipSuccess) err = hipStreamSynchronize(e->own);
+  release();
+  if (err != hipSuccess) return ens_err(e, SIA_ERR_HIP, std::string("autotune: ") + hipGetErrorString(err));
+  int rc = sia_ensemble_set_tile(e, best_cpw);
+  if (rc) return rc;
+  if (chosen) *chosen = best_cpw;
+  if (ms) std::copy(res, res + 3, ms);
+  ENS_TRY(e, hipEventRecord(e->ev_out, e->own));
+  ENS_TRY(e, hipStreamWaitEvent(e->user, e->ev_out, 0));
+  return SIA_OK;
+}
+
+int sia_ensemble_time(sia_ensemble *e, int member, const double **t_device, const double **dt_device) {
+  if (!e || member < 0 || member >= e->n) return SIA_ERR_ARGUMENT;
+  if (t_device) *t_device = e->d_t + member;
+  if (dt_device) *dt_device = e->d_dt + member;
+  return SIA_OK;
+}
+
+int sia_ensemble_check_errors(sia_ensemble *e) {
+  if (!e) return SIA_ERR_ARGUMENT;
+  ENS_TRY(e, hipStreamSynchronize(e->own));
+  std::vector<int> acc(e->n, 0);
+  ENS_TRY(e, hipMemcpy(acc.data(), e->d_err_acc, sizeof(int) * e->n, hipMemcpyDeviceToHost));
+  for (int i = 0; i < e->n; i++) {
+    int now = 0;
+    ENS_TRY(e, hipMemcpy(&now, e->members[i]->err, sizeof(int), hipMemcpyDeviceToHost));
+    const int f = acc[i] | now;
+    const std::string who = "member " + std::to_string(i) + ": ";
+    if (f & ERR_NEG_THK) return ens_err(e, SIA_ERR_NEGATIVE_THICKNESS, who + "negative ice thickness");
+    if (f & ERR_THK_LZ) return ens_err(e, SIA_ERR_THICKNESS_ABOVE_LZ, who + "ice thickness is above the top of the computational grid");
+    if (f & ERR_OMEGA) return ens_err(e, SIA_ERR_OMEGA_NEGATIVE, who + "omega is negative in BedSmoother::get_theta()");
+  }
+  return SIA_OK;
+}
+
+const char *sia_ensemble_last_error(const sia_ensemble *e) { return e ? e->last_error.c_str() : "NULL ensemble"; }
 
 }  // extern "C"

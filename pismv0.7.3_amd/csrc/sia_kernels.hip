@@ -168,9 +168,6 @@ __global__ void k_smooth_coeffs(DevParams P, double s2, double s3, double s4,
 }
 
 // ---------------------------------------------------------------- gradient
-struct Fields2D {
-  const double *H, *b, *h, *mask;
-};
 
 __device__ __forceinline__ long idx2(const DevParams &P, int i, int j) {
   return (long)wrapi(i, P.Mx) * P.My + wrapi(j, P.My);
@@ -204,9 +201,9 @@ __device__ __forceinline__ void haseloff_y(const DevParams &P, const Fields2D &F
   }
 }
 
-__global__ void k_gradient(DevParams P, Fields2D F, double *__restrict__ hx, double *__restrict__ hy) {
-  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= (long)P.Mx * P.My) return;
+// surface gradient at map point q on both staggered grids
+__device__ __forceinline__ void gradient_point(const DevParams &P, const Fields2D &F, double *__restrict__ hx,
+                                               double *__restrict__ hy, long q) {
   const int i = (int)(q / P.My), j = (int)(q % P.My);
   const double dx = P.dx, dy = P.dy;
   double hx0, hy0, hx1, hy1;
@@ -296,14 +293,19 @@ __global__ void k_gradient(DevParams P, Fields2D F, double *__restrict__ hx, dou
   hy[2 * q + 1] = hy1;
 }
 
-// ---------------------------------------------------------------- theta, thksmooth
-// get_theta (PISMBedSmoother.cc:360-420) and get_smoothed_thk (286-336)
-__global__ void k_theta_thk(DevParams P, Fields2D F, const double *__restrict__ topgs,
-                            const double *__restrict__ maxtl, const double *__restrict__ C2,
-                            const double *__restrict__ C3, const double *__restrict__ C4,
-                            double *__restrict__ theta, double *__restrict__ thks, int *err) {
+__global__ void k_gradient(DevParams P, Fields2D F, double *__restrict__ hx, double *__restrict__ hy) {
   const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (long)P.Mx * P.My) return;
+  gradient_point(P, F, hx, hy, q);
+}
+
+// ---------------------------------------------------------------- theta, thksmooth
+// get_theta (PISMBedSmoother.cc:360-420) and get_smoothed_thk (286-336)
+__device__ __forceinline__ void theta_thk_point(const DevParams &P, const Fields2D &F, const double *__restrict__ topgs,
+                                                const double *__restrict__ maxtl, const double *__restrict__ C2,
+                                                const double *__restrict__ C3, const double *__restrict__ C4,
+                                                double *__restrict__ theta, double *__restrict__ thks, int *err,
+                                                long q) {
   const double usurf = F.h[q], thk = F.H[q], mt = maxtl[q], bs = topgs[q];
   // theta
   double th;
@@ -342,6 +344,15 @@ __global__ void k_theta_thk(DevParams P, Fields2D F, const double *__restrict__ 
   thks[q] = r;
 }
 
+__global__ void k_theta_thk(DevParams P, Fields2D F, const double *__restrict__ topgs,
+                            const double *__restrict__ maxtl, const double *__restrict__ C2,
+                            const double *__restrict__ C3, const double *__restrict__ C4,
+                            double *__restrict__ theta, double *__restrict__ thks, int *err) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)P.Mx * P.My) return;
+  theta_thk_point(P, F, topgs, maxtl, C2, C3, C4, theta, thks, err, q);
+}
+
 // ---------------------------------------------------------------- column kernel
 // compute_diffusive_flux (SIAFD.cc:620-721) and, when FULL, compute_I
 // (SIAFD.cc:845-878). Each wavefront owns COLS_PER_WAVE consecutive
@@ -351,28 +362,28 @@ __global__ void k_theta_thk(DevParams P, Fields2D F, const double *__restrict__ 
 // first 64 levels of E of the next column loaded one column ahead. D, flux
 // and ks leave as one coalesced store per wave. D_max is reduced per block
 // into dmax_part[block]; a one-block kernel takes the max of the partials.
-constexpr int COLS_PER_WAVE = 16;
-constexpr int COLS_PER_BLOCK = WAVES_PER_BLOCK * COLS_PER_WAVE;
-
-template <int LAW, bool FULL>
-__global__ __launch_bounds__(256) void k_column(DevParams P, ColumnArgs A) {
-  extern __shared__ double zs[];  // Mz levels, staged once per block
-  __shared__ double wmax[WAVES_PER_BLOCK];
+// block bid of the column grid with COLS_PER_WAVE = CPW; zs: Mz doubles of
+// LDS, wmax: WAVES_PER_BLOCK doubles of LDS
+template <int LAW, bool FULL, int CPW>
+__device__ __forceinline__ void column_block(const DevParams &P, const ColumnArgs &A, int bid, double *zs,
+                                             double *wmax) {
+  constexpr int COLS_PER_WAVE = CPW;
+  constexpr int COLS_PER_BLOCK = WAVES_PER_BLOCK * COLS_PER_WAVE;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int Mx = P.Mx, My = P.My, Mz = P.Mz;
   const int nstag = 2 * Mx * My;
   if (A.only) {
     // a block none of whose columns is selected leaves at once
-    const int sb = blockIdx.x * COLS_PER_BLOCK + (int)threadIdx.x;
+    const int sb = bid * COLS_PER_BLOCK + (int)threadIdx.x;
     const bool mine = (int)threadIdx.x < COLS_PER_BLOCK && sb < nstag && A.only[sb] == ONLY_MARK;
     if (!__syncthreads_or(mine)) {
-      if (threadIdx.x == 0) A.dmax_part[blockIdx.x] = 0.0;
+      if (threadIdx.x == 0) A.dmax_part[bid] = 0.0;
       return;
     }
   }
   for (int k = threadIdx.x; k < Mz; k += blockDim.x) zs[k] = A.z[k];
   __syncthreads();
-  const int s0 = (blockIdx.x * WAVES_PER_BLOCK + wv) * COLS_PER_WAVE;
+  const int s0 = (bid * WAVES_PER_BLOCK + wv) * COLS_PER_WAVE;
   const int s = s0 + lane;
 
   // lane c < COLS_PER_WAVE: column s0 + c. c_ks: -4 not selected (A.only),
@@ -506,8 +517,18 @@ __global__ __launch_bounds__(256) void k_column(DevParams P, ColumnArgs A) {
   if (threadIdx.x == 0) {
     double m = wmax[0];
     for (int w = 1; w < WAVES_PER_BLOCK; w++) m = (m < wmax[w]) ? wmax[w] : m;
-    A.dmax_part[blockIdx.x] = m;
+    A.dmax_part[bid] = m;
   }
+}
+
+constexpr int COLS_PER_WAVE = 16;
+constexpr int COLS_PER_BLOCK = WAVES_PER_BLOCK * COLS_PER_WAVE;
+
+template <int LAW, bool FULL>
+__global__ __launch_bounds__(256) void k_column(DevParams P, ColumnArgs A) {
+  extern __shared__ double zs[];  // Mz levels, staged once per block
+  __shared__ double wmax[WAVES_PER_BLOCK];
+  column_block<LAW, FULL, COLS_PER_WAVE>(P, A, blockIdx.x, zs, wmax);
 }
 
 // max of the per-block partials (one block); D >= 0 so +0.0 is the identity
@@ -707,9 +728,9 @@ __global__ void k_cfl2d(DevParams P, const double *__restrict__ mask, const doub
 
 // dt = min(max, end of run, 2-D CFL, diffusivity) (iMadaptive.cc:157-171, 214-327);
 // accumulates t += dt when t is given.
-__global__ void k_dt(DevParams P, const unsigned long long *dmax_bits,
-                     const unsigned long long *cfl_bits, double t_end, double *t, double *dt_out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ __forceinline__ void dt_single(const DevParams &P, const unsigned long long *dmax_bits,
+                                          const unsigned long long *cfl_bits, double t_end, double *t,
+                                          double *dt_out) {
   const double D_max = __longlong_as_double((long long)*dmax_bits);
   double dt = P.max_dt;
   if (t != nullptr && t_end > 0.0) {
@@ -725,6 +746,12 @@ __global__ void k_dt(DevParams P, const unsigned long long *dmax_bits,
   }
   *dt_out = dt;
   if (t != nullptr) *t = *t + dt;
+}
+
+__global__ void k_dt(DevParams P, const unsigned long long *dmax_bits,
+                     const unsigned long long *cfl_bits, double t_end, double *t, double *dt_out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  dt_single(P, dmax_bits, cfl_bits, t_end, t, dt_out);
 }
 
 // ---------------------------------------------------------------- mass continuity
@@ -749,10 +776,10 @@ __device__ __forceinline__ void adjust_flow_dir(int c, int nb, double &ssa, doub
   if (m_ice_free_ocean(c) && m_ice_free_ocean(nb)) { sia = 0.0; ssa = 0.0; return; }
 }
 
-__global__ __launch_bounds__(256) void k_mass_continuity(DevParams P, MassArgs A) {
-  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+// block bid (256 map points) of the mass-continuity grid; red: LDS [8][256]
+__device__ __forceinline__ void mass_block(const DevParams &P, const MassArgs &A, long bid, double (*red)[256]) {
+  const long q = bid * 256 + threadIdx.x;
   const int Mx = P.Mx, My = P.My;
-  __shared__ double red[8][256];
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (q < (long)Mx * My) {
     const int i = (int)(q / My), j = (int)(q % My);
@@ -819,8 +846,13 @@ __global__ __launch_bounds__(256) void k_mass_continuity(DevParams P, MassArgs A
   }
   if (threadIdx.x == 0 && A.partials) {
 #pragma unroll
-    for (int c = 0; c < 8; c++) A.partials[(long)blockIdx.x * 8 + c] = red[c][0];
+    for (int c = 0; c < 8; c++) A.partials[bid * 8 + c] = red[c][0];
   }
+}
+
+__global__ __launch_bounds__(256) void k_mass_continuity(DevParams P, MassArgs A) {
+  __shared__ double red[8][256];
+  mass_block(P, A, blockIdx.x, red);
 }
 
 // sums the per-block partials in a fixed order (one block)
@@ -849,11 +881,9 @@ __global__ void k_reduce_partials(const double *__restrict__ partials, long nblo
 
 // GeometryCalculator::compute over the grid (Mask.hh:82-118; iMgeometry.cc:80-149);
 // copies H_new into H.
-__global__ void k_mask_surface(DevParams P, const double *__restrict__ H_new, const double *__restrict__ bed,
-                               double *__restrict__ H, double *__restrict__ mask, double *__restrict__ h,
-                               int *err) {
-  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= (long)P.Mx * P.My) return;
+__device__ __forceinline__ void mask_point(const DevParams &P, const double *__restrict__ H_new,
+                                           const double *__restrict__ bed, double *__restrict__ H,
+                                           double *__restrict__ mask, double *__restrict__ h, int *err, long q) {
   const double thickness = H_new[q];
   if (thickness < 0) atomicOr(err, ERR_NEG_THK);
   const double hgrounded = bed[q] + thickness;
@@ -871,6 +901,14 @@ __global__ void k_mask_surface(DevParams P, const double *__restrict__ H_new, co
   H[q] = thickness;
   mask[q] = m;
   h[q] = s;
+}
+
+__global__ void k_mask_surface(DevParams P, const double *__restrict__ H_new, const double *__restrict__ bed,
+                               double *__restrict__ H, double *__restrict__ mask, double *__restrict__ h,
+                               int *err) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)P.Mx * P.My) return;
+  mask_point(P, H_new, bed, H, mask, h, err, q);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -968,6 +1006,129 @@ void launch_mask_surface(const DevParams &P, const double *H_new, const double *
   const long n = (long)P.Mx * P.My;
   hipLaunchKernelGGL(k_mask_surface, dim3(blocks_for(n, 256)), dim3(256), 0, st, P, H_new, bed, H, mask, h,
                      err);
+}
+
+// ---------------------------------------------------------------- ensembles
+// One launch per stage for all members of an ensemble: block b of a stage grid
+// belongs to the member m with start[m] <= b < start[m + 1] and runs that
+// member's block b - start[m] of the single-member kernel, with the member's
+// parameters and fields read from the device table (a wave-uniform address:
+// scalar loads). The arithmetic is the single-member kernels', so every member's
+// results are bitwise those of stepping it alone.
+__device__ __forceinline__ int ens_member(const int *__restrict__ start, int nm, int b) {
+  int lo = 0, hi = nm;  // start[lo] <= b < start[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (start[mid] <= b) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// per member: D_max accumulators, error flags, 2-D CFL bound
+__global__ void k_ens_reset(const EnsMember *__restrict__ T, int nm) {
+  const int m = blockIdx.x;
+  if (m >= nm || threadIdx.x != 0) return;
+  const EnsMember &E = T[m];
+  *E.A.dmax_bits = 0ull;
+  *E.err_acc |= *E.A.err;
+  *E.A.err = 0;
+  *E.cfl_bits = (unsigned long long)__double_as_longlong(E.P.max_dt);
+}
+
+// surface gradient, theta and the smoothed thickness (256 map points a block)
+__global__ __launch_bounds__(256) void k_ens_gradient(const EnsMember *__restrict__ T, const int *__restrict__ start,
+                                                      int nm) {
+  const int m = ens_member(start, nm, blockIdx.x);
+  const EnsMember &E = T[m];
+  const long q = (long)(blockIdx.x - start[m]) * 256 + threadIdx.x;
+  if (q >= (long)E.P.Mx * E.P.My) return;
+  gradient_point(E.P, E.F, E.hx, E.hy, q);
+  theta_thk_point(E.P, E.F, E.topgs, E.maxtl, E.C2, E.C3, E.C4, E.theta, E.thks,
+                  E.A.err, q);
+}
+
+template <int LAW, bool FULL, int CPW>
+__global__ __launch_bounds__(256) void k_ens_column(const EnsMember *__restrict__ T, const int *__restrict__ start,
+                                                    int nm) {
+  extern __shared__ double zs[];
+  __shared__ double wmax[WAVES_PER_BLOCK];
+  const int m = ens_member(start, nm, blockIdx.x);
+  column_block<LAW, FULL, CPW>(T[m].P, T[m].A, blockIdx.x - start[m], zs, wmax);
+}
+
+// per member (one block each): D_max from the column partials, then dt and t += dt
+__global__ __launch_bounds__(1024) void k_ens_dt(const EnsMember *__restrict__ T, int nm) {
+  __shared__ double red[1024];
+  const int mb = blockIdx.x;
+  if (mb >= nm) return;
+  const EnsMember &E = T[mb];
+  double m = 0.0;
+  for (int b = threadIdx.x; b < E.n_part; b += blockDim.x) m = (m < E.A.dmax_part[b]) ? E.A.dmax_part[b] : m;
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = blockDim.x >> 1; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] = (red[threadIdx.x] < red[threadIdx.x + off]) ? red[threadIdx.x + off] : red[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    *E.A.dmax_bits = (unsigned long long)__double_as_longlong(red[0]);
+    dt_single(E.P, E.A.dmax_bits, E.cfl_bits, E.t_end, E.t, E.dt);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_ens_mass(const EnsMember *__restrict__ T, const int *__restrict__ start,
+                                                  int nm) {
+  __shared__ double red[8][256];
+  const int m = ens_member(start, nm, blockIdx.x);
+  mass_block(T[m].P, T[m].M, blockIdx.x - start[m], red);
+}
+
+__global__ __launch_bounds__(256) void k_ens_mask(const EnsMember *__restrict__ T, const int *__restrict__ start,
+                                                  int nm) {
+  const int m = ens_member(start, nm, blockIdx.x);
+  const EnsMember &E = T[m];
+  const long q = (long)(blockIdx.x - start[m]) * 256 + threadIdx.x;
+  if (q >= (long)E.P.Mx * E.P.My) return;
+  mask_point(E.P, E.M.H_new, E.F.b, E.H, E.mask, E.h, E.A.err, q);
+}
+
+int ens_point_blocks(const DevParams &P) { return (int)(((long)P.Mx * P.My + 255) / 256); }
+
+int ens_column_blocks(const DevParams &P, int cpw) {
+  return (int)((2L * P.Mx * P.My + WAVES_PER_BLOCK * cpw - 1) / (WAVES_PER_BLOCK * cpw));
+}
+
+template <int LAW, int CPW>
+static void ens_column_law(const EnsMember *T, const int *start, int nm, int nblocks, int max_mz, hipStream_t st) {
+  hipLaunchKernelGGL((k_ens_column<LAW, false, CPW>), dim3(nblocks), dim3(256), sizeof(double) * (size_t)max_mz,
+                     st, T, start, nm);
+}
+
+template <int CPW>
+static void ens_column_cpw(int law, const EnsMember *T, const int *start, int nm, int nblocks, int max_mz,
+                           hipStream_t st) {
+  switch (law) {
+  case LAW_ISOGLEN: ens_column_law<LAW_ISOGLEN, CPW>(T, start, nm, nblocks, max_mz, st); break;
+  case LAW_PB: ens_column_law<LAW_PB, CPW>(T, start, nm, nblocks, max_mz, st); break;
+  case LAW_GPBLD: ens_column_law<LAW_GPBLD, CPW>(T, start, nm, nblocks, max_mz, st); break;
+  case LAW_HOOKE: ens_column_law<LAW_HOOKE, CPW>(T, start, nm, nblocks, max_mz, st); break;
+  case LAW_ARR: ens_column_law<LAW_ARR, CPW>(T, start, nm, nblocks, max_mz, st); break;
+  case LAW_ARRWARM: ens_column_law<LAW_ARRWARM, CPW>(T, start, nm, nblocks, max_mz, st); break;
+  default: ens_column_law<LAW_GK, CPW>(T, start, nm, nblocks, max_mz, st); break;
+  }
+}
+
+void launch_ens_step(const EnsLaunch &L, hipStream_t st) {
+  hipLaunchKernelGGL(k_ens_reset, dim3(L.nm), dim3(64), 0, st, L.table, L.nm);
+  hipLaunchKernelGGL(k_ens_gradient, dim3(L.pt_blocks), dim3(256), 0, st, L.table, L.pt_start, L.nm);
+  switch (L.cpw) {
+  case 8: ens_column_cpw<8>(L.law, L.table, L.col_start, L.nm, L.col_blocks, L.max_mz, st); break;
+  case 32: ens_column_cpw<32>(L.law, L.table, L.col_start, L.nm, L.col_blocks, L.max_mz, st); break;
+  default: ens_column_cpw<16>(L.law, L.table, L.col_start, L.nm, L.col_blocks, L.max_mz, st); break;
+  }
+  hipLaunchKernelGGL(k_ens_dt, dim3(L.nm), dim3(1024), 0, st, L.table, L.nm);
+  hipLaunchKernelGGL(k_ens_mass, dim3(L.pt_blocks), dim3(256), 0, st, L.table, L.pt_start, L.nm);
+  hipLaunchKernelGGL(k_ens_mask, dim3(L.pt_blocks), dim3(256), 0, st, L.table, L.pt_start, L.nm);
 }
 
 }  // namespace pism_sia

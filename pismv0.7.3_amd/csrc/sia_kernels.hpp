@@ -5,6 +5,10 @@
 
 namespace pism_sia {
 
+struct Fields2D {
+  const double *H, *b, *h, *mask;
+};
+
 struct ColumnArgs {
   const double *z;           // [Mz]
   const double *E;           // [Mx][My][Mz]
@@ -62,5 +66,32 @@ long mass_partial_blocks(const DevParams &P);
 void launch_mass_continuity(const DevParams &P, const MassArgs &A, double *acc_out, hipStream_t st);
 void launch_mask_surface(const DevParams &P, const double *H_new, const double *bed, double *H,
                          double *mask, double *h, int *err, hipStream_t st);
+
+// ---- ensembles: one fast SIA step + dt + mass continuity of every member, one
+// launch per stage (device table of members; the member's fields are its own)
+struct EnsMember {
+  DevParams P;
+  Fields2D F;                       // H, b, h, mask read by the update
+  const double *topgs, *maxtl, *C2, *C3, *C4;
+  double *hx, *hy, *theta, *thks;
+  ColumnArgs A;                     // flux, D, D_max partials (n_part of them), errors
+  MassArgs M;                       // dt, H, flux, mask -> H_new (no accounting partials)
+  double *H, *mask, *h;             // geometry written back after mass continuity
+  unsigned long long *cfl_bits;
+  double *t, *dt;                   // device time and time step of the member
+  int *err_acc;                     // error flags of all earlier steps (A.err: this step)
+  double t_end;
+  int n_part;
+};
+
+struct EnsLaunch {
+  const EnsMember *table;           // device [nm]
+  const int *pt_start, *col_start;  // device [nm + 1]: first block of each member
+  int nm, pt_blocks, col_blocks, cpw, law, max_mz;
+};
+
+int ens_point_blocks(const DevParams &P);
+int ens_column_blocks(const DevParams &P, int cpw);
+void launch_ens_step(const EnsLaunch &L, hipStream_t st);
 
 }  // namespace pism_sia

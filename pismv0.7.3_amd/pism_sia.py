@@ -79,6 +79,17 @@ SIGNATURES = {
     "sia_surrogate_samples": (C.c_int, [_vp, _PP, C.POINTER(C.c_int64)]),
     "sia_use_surrogate": (C.c_int, [_vp, _vp]),
     "sia_surrogate_counts": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
+    # batched ensembles (include/sia_ensemble.h)
+    "sia_ensemble_create": (C.c_int, [C.POINTER(_vp), C.POINTER(T.SiaGeometry), C.POINTER(_vp),
+                                      C.POINTER(C.c_double), C.c_int, _PP]),
+    "sia_ensemble_destroy": (C.c_int, [_vp]),
+    "sia_ensemble_set_stream": (C.c_int, [_vp, _vp]),
+    "sia_ensemble_set_tile": (C.c_int, [_vp, C.c_int]),
+    "sia_ensemble_autotune": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "sia_ensemble_step": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "sia_ensemble_time": (C.c_int, [_vp, C.c_int, _PP, _PP]),
+    "sia_ensemble_check_errors": (C.c_int, [_vp]),
+    "sia_ensemble_last_error": (C.c_char_p, [_vp]),
 }
 
 N_EVENTS = 8  # SIA_EV_COUNT
@@ -377,6 +388,65 @@ class Surrogate:
             if rc != 0:
                 raise RuntimeError("hipMemcpy failed: %d" % rc)
         return out
+
+
+class Ensemble:
+    """Many SIAFD members advanced together (include/sia_ensemble.h): one kernel
+    launch per stage for all members, optional HIP-graph capture of multi-step
+    runs. members: SIAFD objects whose bed smoother is ready; geoms: their
+    geometry() structs (kept alive here); Es: their enthalpy tensors."""
+
+    def __init__(self, members, geoms, Es, t_end=None, stream=None):
+        n = len(members)
+        self.members, self.geoms, self.Es = list(members), list(geoms), list(Es)
+        hs = (_vp * n)(*[m.h.value for m in members])
+        gs = (T.SiaGeometry * n)(*geoms)
+        es = (_vp * n)(*[E.data_ptr() for E in Es])
+        te = (C.c_double * n)(*t_end) if t_end is not None else None
+        h = C.c_void_p()
+        rc = lib().sia_ensemble_create(hs, gs, es, te, n, C.byref(h))
+        if rc != 0:
+            raise SIAError(rc, "sia_ensemble_create")
+        self.h = h
+        if stream is not None:
+            self._check(lib().sia_ensemble_set_stream(self.h, C.c_void_p(stream)))
+
+    def _check(self, rc):
+        if rc != 0:
+            raise SIAError(rc, lib().sia_ensemble_last_error(self.h).decode())
+
+    def set_tile(self, cols_per_wave):
+        self._check(lib().sia_ensemble_set_tile(self.h, cols_per_wave))
+
+    def autotune(self, steps=3):
+        """Times each tile width, keeps the fastest; returns (chosen, {width: ms per step})."""
+        chosen = C.c_int()
+        ms = (C.c_double * 3)()
+        self._check(lib().sia_ensemble_autotune(self.h, steps, C.byref(chosen), ms))
+        return chosen.value, {8: ms[0], 16: ms[1], 32: ms[2]}
+
+    def step(self, n_steps=1, graph=False):
+        self._check(lib().sia_ensemble_step(self.h, n_steps, 1 if graph else 0))
+
+    def time(self, member):
+        """(t, dt) of a member in seconds (synchronises)."""
+        t, dt = C.c_void_p(), C.c_void_p()
+        self._check(lib().sia_ensemble_time(self.h, member, C.byref(t), C.byref(dt)))
+        return device_to_numpy(t.value, 1)[0], device_to_numpy(dt.value, 1)[0]
+
+    def check_errors(self):
+        self._check(lib().sia_ensemble_check_errors(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sia_ensemble_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def geometry(H, h, mask, b, smb=None, bmr=None):
