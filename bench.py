@@ -28,8 +28,13 @@ The CPU baseline times the oracle (the CPU restatement of the reference, one
 core) on seeded random windows cut from the same sheet, in the same update
 mode as value; it is a reported baseline, not the target.
 
+The default run also measures the batched ensemble (64 members of uneven
+sizes, one kernel launch per stage for all members, 50 steps captured in a
+HIP graph) and reports it under "ensemble", beside the same members stepped
+with per-member launches.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--path surrogate|plain]
-                       [--mode full|fast] [--no-cpu-baseline] [--copy-calibration GIB]
+                       [--mode full|fast] [--no-cpu-baseline] [--no-ensemble] [--copy-calibration GIB]
 """
 import argparse
 import json
@@ -181,6 +186,78 @@ def time_steps(S, torch, sia, f, steps, warmup, fast):
     return start.elapsed_time(end) / steps, stages, g
 
 
+ENS_SIZES = (201, 401, 601, 801)
+
+
+def ensemble_members(n=64, seed=2026):
+    """The ensemble workload: n members with seeded sizes Mx, My in ENS_SIZES,
+    dx ~ U[2, 5] km, Mz ~ U{101..201} (Lz 4000 m), GPBLD with e ~ U[1, 5] and
+    n in {3, 4}, each a polythermal sheet of its own seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for m in range(n):
+        Mx, My = (int(v) for v in rng.choice(ENS_SIZES, 2))
+        out.append(dict(Mx=Mx, My=My, dx=float(rng.uniform(2e3, 5e3)), Mz=int(rng.integers(101, 202)),
+                        e=float(rng.uniform(1.0, 5.0)), n=float(rng.choice([3.0, 4.0])), seed=1000 + m))
+    return out
+
+
+def run_ensemble(S, torch, steps=50, reps=2):
+    """One persistent-launch-per-stage ensemble step of 64 members, 50 steps
+    captured in a HIP graph; beside it the same members stepped one by one
+    (sia_step per member, its own launches)."""
+    import sia_types as T
+    import workloads as W
+    spec = ensemble_members()
+    members = []
+    for m in spec:
+        z = T.vertical_levels(LZ, m["Mz"])
+        grid = T.make_grid(m["Mx"], m["My"], m["dx"], m["dx"], z)
+        params = W.gl_params(sia_enhancement_factor=m["e"], sia_Glen_exponent=m["n"])
+        f = S.generate_greenland_like(grid, params, Mx_full=m["Mx"], My_full=m["My"], seed=m["seed"], dx=m["dx"])
+        sia = S.SIAFD(grid, params)
+        sia.set_sync_check(False)
+        sia.update(f["H"], f["b"], f["h"], f["mask"], f["E"], fast=True)  # bed smoother, buffers
+        members.append((sia, S.geometry(f["H"], f["h"], f["mask"], f["b"]), f, m["Mx"] * m["My"]))
+    torch.cuda.synchronize()
+    cols = sum(c for *_, c in members)
+    ens = S.Ensemble([m[0] for m in members], [m[1] for m in members], [m[2]["E"] for m in members])
+    chosen, tile_ms = ens.autotune(steps=3)
+    ens.step(steps, graph=True)  # capture + first replay
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        ens.step(steps, graph=True)
+    b.record()
+    torch.cuda.synchronize()
+    ms_graph = a.elapsed_time(b) / (reps * steps)
+    ens.check_errors()
+    # the same members, one sia_step per member per step
+    ts = [(torch.zeros(1, dtype=torch.float64, device="cuda"), torch.zeros(1, dtype=torch.float64, device="cuda"))
+          for _ in members]
+    k = 5
+    a.record()
+    for _ in range(k):
+        for (sia, geom, f, _), (t, dt) in zip(members, ts):
+            sia.step(geom, f["E"], fast=True, t_end=0.0, t_dev=t, dt_dev=dt)
+    b.record()
+    torch.cuda.synchronize()
+    ms_loop = a.elapsed_time(b) / k
+    ens.close()
+    for sia, *_ in members:
+        sia.close()
+    return {"metric": "SIA column-updates/s, batched ensemble", "value": round(cols / (ms_graph / 1000.0), 1),
+            "unit": "column-updates/s", "ms_per_step": round(ms_graph, 4), "steps_per_graph": steps,
+            "members": len(members), "columns": cols, "tile_cols_per_wave": chosen,
+            "autotune_ms_per_step": {str(k_): round(v, 4) for k_, v in tile_ms.items()},
+            "per_member_launches_ms_per_step": round(ms_loop, 4),
+            "speedup_vs_per_member_launches": round(ms_loop / ms_graph, 3),
+            "workload": "64 seeded members: Mx, My in {201,401,601,801}, dx U[2,5] km, Mz U{101..201}, "
+                        "GPBLD e U[1,5], n in {3,4}, polythermal sheets; one step = fast SIA update + dt + "
+                        "mass continuity of every member", "dtype": "f64"}
+
+
 def error_stats(a, b):
     """Relative error of a against b over the entries where b != 0."""
     nz = b != 0
@@ -209,6 +286,7 @@ def main():
     ap.add_argument("--train-steps", type=int, default=12000)
     ap.add_argument("--train-batch", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-member ensemble measurement")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="after the timed steps, run one plain device copy of this many GiB "
                          "(the reference kernel for scaling the rocprofv3 byte counters)")
@@ -319,6 +397,14 @@ def main():
         out_extra["mlp_roofline"] = {"bound": "mfma", "achieved": round(tf, 1), "peak": BF16_PEAK_TFLOPS,
                                      "unit": "TFLOP/s", "frac": round(tf / BF16_PEAK_TFLOPS, 4),
                                      "kernel": "surrogate MLP", "avg_launch_ms": round(mlp_ms, 4)}
+
+    if not args.no_ensemble:
+        log("64-member ensemble (one launch per stage, HIP graph of 50 steps)")
+        if args.path == "surrogate":
+            sur.close()
+            sg.close()  # frees the training samples
+        out_extra["ensemble"] = run_ensemble(S, torch)
+        log("  %s" % out_extra["ensemble"])
 
     cpu = None
     if not args.no_cpu_baseline:
