@@ -8,7 +8,22 @@ benchmark, and into the HBM traffic per launch of each kernel.
       the factor that makes the counter of the plain copy kernel (N bytes read,
       N bytes written) read exactly N
 
+  python pismv0.7.3_amd/rocprof_summary.py valu P1.csv [P2.csv ...] --cmd "..."
+      markdown table of VALU issue utilisation, FP64 instruction mix and rate
+      and lane utilisation per kernel (longest dispatch of each), from the SQ
+      counters SQ_INSTS_VALU, SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64,
+      SQ_INSTS_VALU_FLOPS_FP64, SQ_ACTIVE_INST_VALU, SQ_THREAD_CYCLES_VALU and
+      GRBM_GUI_ACTIVE (any split of them over several passes)
+
 FETCH_SIZE and WRITE_SIZE are reported by rocprofv3 in KiB per dispatch.
+
+VALU figures on MI355X (256 CUs, 4 SIMD16 units each, 8 XCDs): a wave64 VALU
+instruction occupies its SIMD for 4 cycles, so VALU issue utilisation =
+4 SQ_INSTS_VALU / (1024 x kernel cycles), kernel cycles = GRBM_GUI_ACTIVE / 8
+(the counter sums the 8 XCDs). SQ_INSTS_VALU_FLOPS_FP64 counts FLOPs per
+wave instruction (FMA = 2); with full wavefronts the FP64 rate is 64 x that
+over the dispatch time, against the 78.6 TFLOP/s FP64 vector peak. Lane
+utilisation = SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU).
 """
 import argparse
 import collections
@@ -84,6 +99,52 @@ def traffic(fetch_csv, write_csv, copy_bytes):
     return out
 
 
+FP64_VECTOR_PEAK_TFLOPS = 78.6
+N_SIMD, N_XCD = 1024, 8
+
+
+def valu_counters(paths):
+    """{kernel: {counter: value}} of the longest dispatch of each kernel, merged over passes."""
+    out = {}
+    for path in paths:
+        per = collections.defaultdict(lambda: collections.defaultdict(dict))
+        dur = collections.defaultdict(dict)
+        for r in csv.DictReader(open(path)):
+            d = int(r["Dispatch_Id"])
+            per[r["Kernel_Name"]][d][r["Counter_Name"]] = float(r["Counter_Value"])
+            dur[r["Kernel_Name"]][d] = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-9
+        for name, disp in per.items():
+            longest = max(dur[name], key=lambda k: dur[name][k])
+            o = out.setdefault(name, {})
+            o.update(disp[longest])
+            o.setdefault("seconds", dur[name][longest])
+    return out
+
+
+def valu_markdown(paths, cmd, min_seconds=1e-4):
+    c = valu_counters(paths)
+    lines = ["# VALU and FP64 counters per kernel (longest dispatch)", "", "Commands:", "", "```", cmd, "```", "",
+             "| kernel | ms | VALU instr | VALU issue util | FP64 instr share | FP64 TFLOP/s | % of FP64 peak | lane util |",
+             "|---|---:|---:|---:|---:|---:|---:|---:|"]
+    rows = []
+    for name, v in c.items():
+        if v.get("seconds", 0) < min_seconds or "SQ_INSTS_VALU" not in v or "GRBM_GUI_ACTIVE" not in v:
+            continue
+        cyc = v["GRBM_GUI_ACTIVE"] / N_XCD
+        issue = 4.0 * v["SQ_INSTS_VALU"] / (N_SIMD * cyc) if cyc > 0 else 0.0
+        f64 = sum(v.get("SQ_INSTS_VALU_%s_F64" % k, 0.0) for k in ("ADD", "MUL", "FMA", "TRANS"))
+        share = f64 / v["SQ_INSTS_VALU"] if v["SQ_INSTS_VALU"] else 0.0
+        tf = 64.0 * v.get("SQ_INSTS_VALU_FLOPS_FP64", 0.0) / v["seconds"] / 1e12
+        lane = (v["SQ_THREAD_CYCLES_VALU"] / (64.0 * v["SQ_ACTIVE_INST_VALU"])
+                if v.get("SQ_ACTIVE_INST_VALU") and "SQ_THREAD_CYCLES_VALU" in v else float("nan"))
+        rows.append((v["seconds"], "| `%s` | %.3f | %.3g | %.1f%% | %.1f%% | %.1f | %.1f%% | %.1f%% |" % (
+            short_name(name), 1e3 * v["seconds"], v["SQ_INSTS_VALU"], 100 * issue, 100 * share, tf,
+            100 * tf / FP64_VECTOR_PEAK_TFLOPS, 100 * lane)))
+    rows.sort(key=lambda r: -r[0])
+    lines += [r for _, r in rows]
+    return "\n".join(lines) + "\n"
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="what", required=True)
@@ -95,9 +156,14 @@ def main(argv=None):
     b.add_argument("--write", required=True)
     b.add_argument("--copy-bytes", type=float, required=True)
     b.add_argument("--cmd", required=True)
+    v = sub.add_parser("valu")
+    v.add_argument("csv", nargs="+")
+    v.add_argument("--cmd", required=True)
     args = ap.parse_args(argv)
     if args.what == "stats":
         sys.stdout.write(stats_markdown(args.csv, args.cmd))
+    elif args.what == "valu":
+        sys.stdout.write(valu_markdown(args.csv, args.cmd))
     else:
         t = traffic(args.fetch, args.write, args.copy_bytes)
         t["command"] = args.cmd
