@@ -194,6 +194,10 @@ int sia_max_diffusivity_device(sia_handle *h, const double **D_max);/* device sc
 int sia_velocity_u(sia_handle *h, const double **u);               /* [Mx][My][Mz] m s-1 */
 int sia_velocity_v(sia_handle *h, const double **v);
 int sia_velocity_w(sia_handle *h, const double **w);
+/* StressBalance::volumetric_strain_heating() (PISMStressBalance.cc:146-148,
+   computed by compute_volumetric_strain_heating, 370-500, on every full
+   update): [Mx][My][Mz] W m-3, zero above kBelowHeight(H). */
+int sia_volumetric_strain_heating(sia_handle *h, const double **Sigma);
 
 /* Diagnostics (SIAFD_diagnostics.cc:26-35). */
 int sia_surface_gradient(sia_handle *h, const double **h_x, const double **h_y); /* [Mx][My][2] each */
@@ -255,7 +259,8 @@ enum sia_event {
   SIA_EV_MASS_TRANSPORT = 5, /* "mass transport" (+ time step, surface and mask) */
   SIA_EV_SG_FEATURES = 6,    /* "surrogate features": the learned surrogate's per-level pass (inside "SIA flux") */
   SIA_EV_SG_MLP = 7,         /* "surrogate MLP": its fused network (inside "SIA flux") */
-  SIA_EV_COUNT = 8
+  SIA_EV_STRAIN_HEAT = 8,    /* "SB strain heat" (PISMStressBalance.cc:102-104) */
+  SIA_EV_COUNT = 9
 };
 int sia_profiling(sia_handle *h, int on);
 int sia_profiling_read(sia_handle *h, double *ms /* [SIA_EV_COUNT] */, int64_t *count /* [SIA_EV_COUNT] */);

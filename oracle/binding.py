@@ -59,6 +59,8 @@ def _load():
         "or_compute_I": (C.c_int, [G, _dp, _dp, _dp, _dp, _dp]),
         "or_horizontal_velocity": (C.c_int, [G, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
         "or_vertical_velocity": (C.c_int, [G, _dp, _dp, _dp, _dp, _dp]),
+        "or_hardness": (d, [P, d, d]),
+        "or_strain_heating": (C.c_int, [G, P, _dp, _dp, _dp, _dp, _dp, _dp]),
         "or_sia_update": (C.c_int, [G, P, C.c_void_p, C.c_int]),
         "or_update_mask_surface": (C.c_int, [G, P, _dp, _dp, _dp, _dp]),
         "or_mass_continuity": (C.c_int, [G, P, d, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -106,7 +108,7 @@ class OrFields(C.Structure):
         "topgsmooth", "maxtl", "C2", "C3", "C4",
         "theta", "thksmooth", "h_x", "h_y", "flux", "D_stag",
         "delta0", "delta1", "I0", "I1", "u", "v", "w")] + [
-        ("D_max", C.c_double), ("Nx", C.c_int), ("Ny", C.c_int)]
+        ("D_max", C.c_double), ("Nx", C.c_int), ("Ny", C.c_int), ("Sigma", C.c_void_p)]
 
 
 # ---------------------------------------------------------------- scalars
@@ -205,19 +207,33 @@ def sia_update(grid, params, H, b, h, mask, E, full=True, age=None, ub=None, vb=
     out = {k: np.zeros(n2) for k in ("topgsmooth", "maxtl", "C2", "C3", "C4", "theta", "thksmooth")}
     out.update({k: np.zeros(2 * n2) for k in ("h_x", "h_y", "flux", "D_stag")})
     n3w = n3 if full else Mz  # oracle touches delta only when full
-    out.update({k: np.zeros(n3w) for k in ("delta0", "delta1", "I0", "I1", "u", "v", "w")})
+    out.update({k: np.zeros(n3w) for k in ("delta0", "delta1", "I0", "I1", "u", "v", "w", "Sigma")})
     f = OrFields()
     f.H, f.b, f.h, f.mask, f.E = inp("H", H), inp("b", b), inp("h", h), inp("mask", mask), inp("E", E)
     f.age, f.ub, f.vb, f.bmr = inp("age", age), inp("ub", ub), inp("vb", vb), inp("bmr", bmr)
     for k in ("topgsmooth", "maxtl", "C2", "C3", "C4", "theta", "thksmooth", "h_x", "h_y",
               "flux", "D_stag", "delta0", "delta1", "I0", "I1", "u", "v", "w"):
         setattr(f, k, out[k].ctypes.data)
+    f.Sigma = out["Sigma"].ctypes.data if full else None
     _check(lib().or_sia_update(C.byref(grid), C.byref(params), C.byref(f), 1 if full else 0))
     out["D_max"] = f.D_max
     out["Nx"], out["Ny"] = f.Nx, f.Ny
     if not full:
-        for k in ("delta0", "delta1", "I0", "I1", "u", "v", "w"):
+        for k in ("delta0", "delta1", "I0", "I1", "u", "v", "w", "Sigma"):
             del out[k]
+    return out
+
+
+def hardness(params, E, p):
+    """FlowLaw::hardness_parameter of the selected flow law."""
+    return lib().or_hardness(C.byref(params), E, p)
+
+
+def strain_heating(grid, params, H, mask, E, u, v):
+    """StressBalance::compute_volumetric_strain_heating from given u, v."""
+    arr = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (H, mask, E, u, v)]
+    out = np.zeros(grid.Mx * grid.My * grid.Mz)
+    _check(lib().or_strain_heating(C.byref(grid), C.byref(params), *[_ptr(a) for a in arr], _ptr(out)))
     return out
 
 

@@ -252,6 +252,39 @@ double or_flow(const sia_params *P, double stress, double E, double p, double gs
   }
 }
 
+/* FlowLaw::hardness_parameter = softness^(-1/n) (flowlaws.cc:76, 103-105) with
+   each law's softness_parameter: PatersonBudd and its cold/warm variants use
+   the converter's pressure-adjusted temperature (flowlaws.cc:226-229; hooke
+   267-270; arr/arrwarm via flowlaws.hh:281-306), GPBLD its own
+   (flowlaws.cc:209-221), isothermal Glen the constant B (flowlaws.hh:235-237,
+   flowlaws.cc:249-253), Goldsby-Kohlstedt the Paterson-Budd relation
+   (flowlaws.cc:336-351). */
+double or_hardness(const sia_params *P, double E, double p) {
+  const double hp = -1.0 / P->sia_Glen_exponent;
+  const double R = P->ideal_gas_constant;
+  switch (P->sia_flow_law) {
+  case SIA_FLOW_ISOTHERMAL_GLEN:
+    return pow(P->ice_softness, hp);
+  case SIA_FLOW_GPBLD:
+    return pow(gpbld_softness(P, E, p), hp);
+  case SIA_FLOW_PB:
+  case SIA_FLOW_GK:
+    return pow(softness_pb(P, or_ec_pressure_adjusted_temperature(P, E, p)), hp);
+  case SIA_FLOW_HOOKE:
+    return pow(hooke_softness(P, or_ec_pressure_adjusted_temperature(P, E, p)), hp);
+  case SIA_FLOW_ARR: {
+    const double T_pa = or_ec_pressure_adjusted_temperature(P, E, p);
+    return pow(P->Paterson_Budd_A_cold * exp(-P->Paterson_Budd_Q_cold / (R * T_pa)), hp);
+  }
+  case SIA_FLOW_ARRWARM: {
+    const double T_pa = or_ec_pressure_adjusted_temperature(P, E, p);
+    return pow(P->Paterson_Budd_A_warm * exp(-P->Paterson_Budd_Q_warm / (R * T_pa)), hp);
+  }
+  default:
+    return NAN;
+  }
+}
+
 /* FlowLawUsesGrainSize, flowlaws.cc:33-42 */
 int or_flow_uses_grain_size(const sia_params *P) {
   static const double gs[] = {1e-4, 1e-3, 1e-2, 1}, s = 1e4, E = 400000, p = 1e6;
@@ -877,7 +910,67 @@ int or_vertical_velocity(const sia_grid *g, const double *mask, const double *u,
   return SIA_OK;
 }
 
-/* SIAFD::update (SIAFD.cc:119-148) + StressBalance::update's w (PISMStressBalance.cc:98-110) */
+/* D^2 of the shallow strain-rate approximation, PISMStressBalance.cc:309-311 */
+static double D2(double u_x, double u_y, double u_z, double v_x, double v_y, double v_z) {
+  return 0.5 * ((u_x + v_y) * (u_x + v_y) + u_x * u_x + v_y * v_y +
+                0.5 * ((u_y + v_x) * (u_y + v_x) + u_z * u_z + v_z * v_z));
+}
+
+/* StressBalance::compute_volumetric_strain_heating, PISMStressBalance.cc:370-500 */
+int or_strain_heating(const sia_grid *g, const sia_params *P, const double *H, const double *mask,
+                      const double *E, const double *u, const double *v, double *Sigma) {
+  const int Mx = g->Mx, My = g->My, Mz = g->Mz;
+  const double *z = g->z;
+  const double n = P->sia_Glen_exponent, exponent = 0.5 * (1.0 / n + 1.0),
+               e_to_a_power = pow(P->sia_enhancement_factor, -1.0 / n);
+#define MM(a, c) as_int(mask[I2(g, (a), (c))])
+  for (int i = 0; i < Mx; i++) {
+    for (int j = 0; j < My; j++) {
+      const double Hij = H[I2(g, i, j)];
+      const int ks = or_kbelow(z, Mz, Hij);
+      if (ks < 0) return fail(SIA_ERR_THICKNESS_ABOVE_LZ, "strain heating: thickness outside the grid");
+      double west = 1, east = 1, south = 1, north = 1, D_x = 0, D_y = 0;
+      const int m = MM(i, j);
+      if ((m_icy(m) && m_ice_free(MM(i + 1, j))) || (m_ice_free(m) && m_icy(MM(i + 1, j)))) east = 0;
+      if ((m_icy(m) && m_ice_free(MM(i - 1, j))) || (m_ice_free(m) && m_icy(MM(i - 1, j)))) west = 0;
+      D_x = (east + west > 0) ? 1.0 / (g->dx * (east + west)) : 0.0;
+      if ((m_icy(m) && m_ice_free(MM(i, j + 1))) || (m_ice_free(m) && m_icy(MM(i, j + 1)))) north = 0;
+      if ((m_icy(m) && m_ice_free(MM(i, j - 1))) || (m_ice_free(m) && m_icy(MM(i, j - 1)))) south = 0;
+      D_y = (north + south > 0) ? 1.0 / (g->dy * (north + south)) : 0.0;
+      const double *u_ij = u + I3(g, i, j), *u_w = u + I3(g, i - 1, j), *u_e = u + I3(g, i + 1, j),
+                   *u_s = u + I3(g, i, j - 1), *u_n = u + I3(g, i, j + 1);
+      const double *v_ij = v + I3(g, i, j), *v_w = v + I3(g, i - 1, j), *v_e = v + I3(g, i + 1, j),
+                   *v_s = v + I3(g, i, j - 1), *v_n = v + I3(g, i, j + 1);
+      const double *E_ij = E + I3(g, i, j);
+      double *S = Sigma + I3(g, i, j);
+      for (int k = 0; k <= ks; ++k) {
+        const double pressure = or_ec_pressure(P, Hij - z[k]);
+        const double B = or_hardness(P, E_ij[k], pressure);
+        double dz, u_z, v_z;
+        const double u_x = D_x * (west * (u_ij[k] - u_w[k]) + east * (u_e[k] - u_ij[k])),
+                     u_y = D_y * (south * (u_ij[k] - u_s[k]) + north * (u_n[k] - u_ij[k])),
+                     v_x = D_x * (west * (v_ij[k] - v_w[k]) + east * (v_e[k] - v_ij[k])),
+                     v_y = D_y * (south * (v_ij[k] - v_s[k]) + north * (v_n[k] - v_ij[k]));
+        if (k > 0) {
+          dz = z[k + 1] - z[k - 1];
+          u_z = (u_ij[k + 1] - u_ij[k - 1]) / dz;
+          v_z = (v_ij[k + 1] - v_ij[k - 1]) / dz;
+        } else {
+          dz = z[1] - z[0];
+          u_z = (u_ij[1] - u_ij[0]) / dz;
+          v_z = (v_ij[1] - v_ij[0]) / dz;
+        }
+        S[k] = 2.0 * e_to_a_power * B * pow(D2(u_x, u_y, u_z, v_x, v_y, v_z), exponent);
+      }
+      for (int k = ks + 1; k < Mz; ++k) S[k] = 0.0;
+    }
+  }
+#undef MM
+  return SIA_OK;
+}
+
+/* SIAFD::update (SIAFD.cc:119-148) + StressBalance::update's strain heating and
+   w (PISMStressBalance.cc:98-110) */
 int or_sia_update(const sia_grid *g, const sia_params *P, or_fields *f, int full) {
   int rc;
   rc = or_bed_smoother(g, P, f->b, f->topgsmooth, f->maxtl, f->C2, f->C3, f->C4, &f->Nx, &f->Ny);
@@ -896,6 +989,10 @@ int or_sia_update(const sia_grid *g, const sia_params *P, or_fields *f, int full
     if (rc) return rc;
     rc = or_horizontal_velocity(g, f->h_x, f->h_y, f->I0, f->I1, f->ub, f->vb, f->u, f->v);
     if (rc) return rc;
+    if (f->Sigma) {
+      rc = or_strain_heating(g, P, f->H, f->mask, f->E, f->u, f->v, f->Sigma);
+      if (rc) return rc;
+    }
     rc = or_vertical_velocity(g, f->mask, f->u, f->v, f->bmr, f->w);
     if (rc) return rc;
   }

@@ -94,11 +94,19 @@ typedef struct or_fields {
   double *u, *v, *w;                   /* required when full */
   double D_max;
   int Nx, Ny;
+  double *Sigma;                       /* [Mx*My*Mz] strain heating when full, or NULL */
 } or_fields;
 
 /* SIAFD::update + (if full) StressBalance::compute_vertical_velocity. The bed
    smoother always runs (the oracle keeps no state). */
 int or_sia_update(const sia_grid *g, const sia_params *P, or_fields *f, int full);
+
+/* FlowLaw::hardness_parameter of the selected law (flowlaws.cc:103-105, 336-351) */
+double or_hardness(const sia_params *P, double E, double p);
+
+/* StressBalance::compute_volumetric_strain_heating (PISMStressBalance.cc:370-500) */
+int or_strain_heating(const sia_grid *g, const sia_params *P, const double *H, const double *mask,
+                      const double *E, const double *u, const double *v, double *Sigma);
 
 /* GeometryCalculator::compute over the grid (Mask.hh:67-118, iMgeometry.cc:80-149) */
 int or_update_mask_surface(const sia_grid *g, const sia_params *P, const double *bed,

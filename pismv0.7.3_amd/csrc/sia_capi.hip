@@ -35,7 +35,7 @@ struct sia_handle {
   // SIAFD work and output fields
   double *hx = nullptr, *hy = nullptr, *theta = nullptr, *thks = nullptr;
   double *flux = nullptr, *Dstag = nullptr;
-  double *I0 = nullptr, *I1 = nullptr, *u = nullptr, *v = nullptr, *w = nullptr;
+  double *I0 = nullptr, *I1 = nullptr, *u = nullptr, *v = nullptr, *w = nullptr, *Sigma = nullptr;
   int *ks = nullptr;
   // scalars / scratch
   unsigned long long *dmax_bits = nullptr, *cfl_bits = nullptr;
@@ -199,6 +199,10 @@ void fill_dev_params(sia_handle *h) {
   d.hooke_k = p.Hooke_k;
   d.hooke_Tr = p.Hooke_Tr;
   d.sec_to_ka = 1.0 / (1000.0 * p.seconds_per_year);
+  d.hard_pow = -1.0 / p.sia_Glen_exponent;                         // flowlaws.cc:76
+  d.e_pow = std::pow(p.sia_enhancement_factor, -1.0 / p.sia_Glen_exponent);  // PISMStressBalance.cc:388
+  d.heat_exp = 0.5 * (1.0 / p.sia_Glen_exponent + 1.0);             // PISMStressBalance.cc:387
+  d.B_glen = std::pow(p.ice_softness, d.hard_pow);                  // flowlaws.cc:252
   d.inv_ci = 1.0 / p.ice_specific_heat_capacity;
   d.inv_L = 1.0 / p.water_latent_heat_fusion;
   d.alpha_float = 1 - p.ice_density / p.sea_water_density;
@@ -285,7 +289,8 @@ int update_impl(sia_handle *h, const sia_inputs *in, int fast, bool allow_sync) 
   int rc;
   if (full) {
     if ((rc = dalloc(h, &h->I0, n3)) || (rc = dalloc(h, &h->I1, n3)) || (rc = dalloc(h, &h->u, n3)) ||
-        (rc = dalloc(h, &h->v, n3)) || (rc = dalloc(h, &h->w, n3)) || (rc = dalloc(h, &h->ks, 2 * n2))) {
+        (rc = dalloc(h, &h->v, n3)) || (rc = dalloc(h, &h->w, n3)) || (rc = dalloc(h, &h->Sigma, n3)) ||
+        (rc = dalloc(h, &h->ks, 2 * n2))) {
       return rc;
     }
   }
@@ -349,6 +354,21 @@ int update_impl(sia_handle *h, const sia_inputs *in, int fast, bool allow_sync) 
     V.v = h->v;
     V.w = h->w;
     launch_velocity_uvw(P, V, st);
+  }
+  if (full) {
+    // StressBalance::update computes the strain heating on every full update
+    // (PISMStressBalance.cc:102-105)
+    Stage stage(h, SIA_EV_STRAIN_HEAT);
+    HeatArgs S{};
+    S.z = h->d_z;
+    S.H = in->land_ice_thickness;
+    S.mask = in->mask;
+    S.E = in->enthalpy;
+    S.u = h->u;
+    S.v = h->v;
+    S.Sigma = h->Sigma;
+    S.err = h->err;
+    launch_strain_heating(P, S, st);
   }
   if (in->vel_input_u && in->vel_input_v) {
     launch_cfl2d(P, in->mask, in->vel_input_u, in->vel_input_v, h->cfl_bits, st);
@@ -611,7 +631,7 @@ int sia_destroy(sia_handle *h) {
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   else (void)hipDeviceSynchronize();
   void *bufs[] = {h->d_z, h->topgs, h->maxtl, h->C2, h->C3, h->C4, h->hx, h->hy, h->theta, h->thks,
-                  h->flux, h->Dstag, h->I0, h->I1, h->u, h->v, h->w, h->ks, h->dmax_bits, h->cfl_bits, h->err,
+                  h->flux, h->Dstag, h->I0, h->I1, h->u, h->v, h->w, h->Sigma, h->ks, h->dmax_bits, h->cfl_bits, h->err,
                   h->dt_dev, h->acc_dev, h->partials, h->H_tmp, h->dmax_part, h->sg_status,
                   h->sg_block_count, h->sg_list, h->sg_nlist, h->sg_counts, h->sg_F, h->sg_CQb, h->sg_CSt};
   for (void *b : bufs) dfree(b);
@@ -678,6 +698,9 @@ static int get3d(sia_handle *h, const double **out, double *buf) {
 int sia_velocity_u(sia_handle *h, const double **u) { return get3d(h, u, h ? h->u : nullptr); }
 int sia_velocity_v(sia_handle *h, const double **v) { return get3d(h, v, h ? h->v : nullptr); }
 int sia_velocity_w(sia_handle *h, const double **w) { return get3d(h, w, h ? h->w : nullptr); }
+int sia_volumetric_strain_heating(sia_handle *h, const double **Sigma) {
+  return get3d(h, Sigma, h ? h->Sigma : nullptr);
+}
 
 int sia_surface_gradient(sia_handle *h, const double **h_x, const double **h_y) {
   if (!h || !h_x || !h_y) return SIA_ERR_ARGUMENT;
@@ -827,7 +850,7 @@ int sia_profiling_read(sia_handle *h, double *ms, int64_t *count) {
 const char *sia_event_name(int event) {
   static const char *names[SIA_EV_COUNT] = {"SIA bed smoother", "SIA gradient",       "SIA flux",
                                             "SIA 3D hor. vel.", "SB vert. vel.",      "mass transport",
-                                            "surrogate features", "surrogate MLP"};
+                                            "surrogate features", "surrogate MLP",    "SB strain heat"};
   return (event >= 0 && event < SIA_EV_COUNT) ? names[event] : "unknown";
 }
 

@@ -42,6 +42,11 @@ struct DevParams {
   double hooke_A, hooke_Q, hooke_C, hooke_k, hooke_Tr;
   double sec_to_ka;
   double inv_ci, inv_L;      // 1 / c_i, 1 / L (the surrogate's per-level pass)
+  // strain heating (PISMStressBalance.cc:384-388; flowlaws.cc:76, 249-253)
+  double hard_pow;           // -1 / n
+  double e_pow;              // e^(-1/n)
+  double heat_exp;           // (1/n + 1) / 2
+  double B_glen;             // ice_softness^(-1/n)
   // geometry / time stepping
   double alpha_float, sea_level, icefree_thk, floating_thk, max_dt, adapt_ratio;
 };
@@ -184,6 +189,40 @@ __device__ __forceinline__ double flow(const DevParams &P, double stress, double
     return P.A_warm * exp(-P.Q_warm / (P.R * temp)) * stress_pow(P, stress);
   } else {  // LAW_GK
     return gk_flow_from_temp(P, stress, ec_temperature(P, E, p), p, gs);
+  }
+}
+
+// FlowLaw::hardness_parameter = softness^(-1/n) with each law's
+// softness_parameter (flowlaws.cc:103-105, 209-229, 249-253, 267-270, 336-351;
+// flowlaws.hh:235-237, 281-306): the Paterson-Budd family uses the
+// converter's pressure-adjusted temperature here, not temp + beta p as in flow
+template <int LAW>
+__device__ __forceinline__ double hardness(const DevParams &P, double E, double p) {
+  if (LAW == LAW_ISOGLEN) {
+    return P.B_glen;
+  } else if (LAW == LAW_GPBLD) {
+    const double E_s = ec_enthalpy_cts(P, p);
+    double A;
+    if (E < E_s) {
+      A = softness_pb(P, ec_pressure_adjusted_temperature(P, E, p));
+    } else {
+      double omega = ec_water_fraction(P, E, p);
+      if (P.wf_limit < omega) omega = P.wf_limit;
+      A = P.A_T0 * (1.0 + P.wf_coeff * omega);
+    }
+    return pow(A, P.hard_pow);
+  } else if (LAW == LAW_PB || LAW == LAW_GK) {
+    return pow(softness_pb(P, ec_pressure_adjusted_temperature(P, E, p)), P.hard_pow);
+  } else if (LAW == LAW_HOOKE) {
+    const double T_pa = ec_pressure_adjusted_temperature(P, E, p);
+    return pow(P.hooke_A * exp(-P.hooke_Q / (P.R * T_pa) + 3.0 * P.hooke_C * pow(P.hooke_Tr - T_pa, -P.hooke_k)),
+               P.hard_pow);
+  } else if (LAW == LAW_ARR) {
+    const double T_pa = ec_pressure_adjusted_temperature(P, E, p);
+    return pow(P.A_cold * exp(-P.Q_cold / (P.R * T_pa)), P.hard_pow);
+  } else {  // LAW_ARRWARM
+    const double T_pa = ec_pressure_adjusted_temperature(P, E, p);
+    return pow(P.A_warm * exp(-P.Q_warm / (P.R * T_pa)), P.hard_pow);
   }
 }
 

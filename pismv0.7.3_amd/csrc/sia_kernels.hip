@@ -711,6 +711,76 @@ __global__ __launch_bounds__(256) void k_velocity_uvw(DevParams P, VelArgs A, St
   }
 }
 
+// ---------------------------------------------------------------- strain heating
+// StressBalance::compute_volumetric_strain_heating (PISMStressBalance.cc:370-500)
+// from the stored u, v. One wavefront per map column in the velocity kernel's
+// XCD-interleaved strips (rows i-1..i+1 of u and v stay in the XCD's L2), one
+// lane per level; Sigma = 0 above kBelowHeight(H).
+__device__ __forceinline__ double strain_D2(double u_x, double u_y, double u_z, double v_x, double v_y,
+                                            double v_z) {  // PISMStressBalance.cc:309-311
+  return 0.5 * ((u_x + v_y) * (u_x + v_y) + u_x * u_x + v_y * v_y +
+                0.5 * ((u_y + v_x) * (u_y + v_x) + u_z * u_z + v_z * v_z));
+}
+
+template <int LAW>
+__global__ __launch_bounds__(256) void k_strain_heating(DevParams P, HeatArgs A, StripMap sm) {
+  extern __shared__ double zs[];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int Mx = P.Mx, My = P.My, Mz = P.Mz;
+  for (int k = threadIdx.x; k < Mz; k += blockDim.x) zs[k] = A.z[k];
+  __syncthreads();
+  const int x = blockIdx.x % N_XCD;
+  const int Lp = __builtin_amdgcn_readfirstlane((blockIdx.x / N_XCD) * WAVES_PER_BLOCK + wv);
+  int i, j;
+  if (!strip_column(sm, x, Lp, i, j)) return;
+  const long q = (long)i * My + j;
+  const double Hij = A.H[q];
+  int ks = kbelow(zs, Mz, P.Lz, Hij);
+  if (ks < 0 && lane == 0) atomicOr(A.err, ERR_THK_LZ);  // kBelowHeight throws (IceGrid.cc:381-389)
+  const int im = wrapi(i - 1, Mx), ip = wrapi(i + 1, Mx), jm = wrapi(j - 1, My), jp = wrapi(j + 1, My);
+  const long qw = (long)im * My + j, qe = (long)ip * My + j, qs = (long)i * My + jm, qn = (long)i * My + jp;
+  // one-sided differences at ice margins (PISMStressBalance.cc:411-437)
+  const int m = as_int(A.mask[q]), me = as_int(A.mask[qe]), mw = as_int(A.mask[qw]), mn = as_int(A.mask[qn]),
+            ms = as_int(A.mask[qs]);
+  double west = 1, east = 1, south = 1, north = 1;
+  if ((m_icy(m) && m_ice_free(me)) || (m_ice_free(m) && m_icy(me))) east = 0;
+  if ((m_icy(m) && m_ice_free(mw)) || (m_ice_free(m) && m_icy(mw))) west = 0;
+  const double D_x = (east + west > 0) ? 1.0 / (P.dx * (east + west)) : 0.0;
+  if ((m_icy(m) && m_ice_free(mn)) || (m_ice_free(m) && m_icy(mn))) north = 0;
+  if ((m_icy(m) && m_ice_free(ms)) || (m_ice_free(m) && m_icy(ms))) south = 0;
+  const double D_y = (north + south > 0) ? 1.0 / (P.dy * (north + south)) : 0.0;
+  const double *u_ij = A.u + q * Mz, *u_w = A.u + qw * Mz, *u_e = A.u + qe * Mz, *u_s = A.u + qs * Mz,
+               *u_n = A.u + qn * Mz;
+  const double *v_ij = A.v + q * Mz, *v_w = A.v + qw * Mz, *v_e = A.v + qe * Mz, *v_s = A.v + qs * Mz,
+               *v_n = A.v + qn * Mz;
+  const double *E_ij = A.E + q * Mz;
+  double *Sg = A.Sigma + q * Mz;
+  for (int k0 = 0; k0 < Mz; k0 += 64) {
+    const int k = k0 + lane;
+    double sig = 0.0;
+    if (k <= ks) {
+      const double pressure = ec_pressure(P, Hij - zs[k]);
+      const double B = hardness<LAW>(P, E_ij[k], pressure);
+      const double u_x = D_x * (west * (u_ij[k] - u_w[k]) + east * (u_e[k] - u_ij[k])),
+                   u_y = D_y * (south * (u_ij[k] - u_s[k]) + north * (u_n[k] - u_ij[k])),
+                   v_x = D_x * (west * (v_ij[k] - v_w[k]) + east * (v_e[k] - v_ij[k])),
+                   v_y = D_y * (south * (v_ij[k] - v_s[k]) + north * (v_n[k] - v_ij[k]));
+      double u_z, v_z;
+      if (k > 0) {  // ks <= Mz - 2, so level k + 1 exists
+        const double dz = zs[k + 1] - zs[k - 1];
+        u_z = (u_ij[k + 1] - u_ij[k - 1]) / dz;
+        v_z = (v_ij[k + 1] - v_ij[k - 1]) / dz;
+      } else {
+        const double dz = zs[1] - zs[0];
+        u_z = (u_ij[1] - u_ij[0]) / dz;
+        v_z = (v_ij[1] - v_ij[0]) / dz;
+      }
+      sig = 2.0 * P.e_pow * B * pow(strain_D2(u_x, u_y, u_z, v_x, v_y, v_z), P.heat_exp);
+    }
+    if (k < Mz) Sg[k] = sig;
+  }
+}
+
 // ---------------------------------------------------------------- time step
 // 2-D CFL over icy cells (iMadaptive.cc:118-144); the min is taken on the bit
 // patterns of positive doubles, which is order independent.
@@ -978,6 +1048,25 @@ void launch_velocity_uvw(const DevParams &P, const VelArgs &A, hipStream_t st) {
   const StripMap sm = strip_map(P.Mx, P.My, WAVES_PER_BLOCK);
   hipLaunchKernelGGL(k_velocity_uvw, dim3(sm.per_xcd * N_XCD), dim3(256), sizeof(double) * (size_t)P.Mz, st, P,
                      A, sm);
+}
+
+template <int LAW>
+static void launch_heat_law(const DevParams &P, const HeatArgs &A, hipStream_t st) {
+  const StripMap sm = strip_map(P.Mx, P.My, WAVES_PER_BLOCK);
+  hipLaunchKernelGGL((k_strain_heating<LAW>), dim3(sm.per_xcd * N_XCD), dim3(256), sizeof(double) * (size_t)P.Mz,
+                     st, P, A, sm);
+}
+
+void launch_strain_heating(const DevParams &P, const HeatArgs &A, hipStream_t st) {
+  switch (P.law) {
+  case LAW_ISOGLEN: launch_heat_law<LAW_ISOGLEN>(P, A, st); break;
+  case LAW_PB: launch_heat_law<LAW_PB>(P, A, st); break;
+  case LAW_GPBLD: launch_heat_law<LAW_GPBLD>(P, A, st); break;
+  case LAW_HOOKE: launch_heat_law<LAW_HOOKE>(P, A, st); break;
+  case LAW_ARR: launch_heat_law<LAW_ARR>(P, A, st); break;
+  case LAW_ARRWARM: launch_heat_law<LAW_ARRWARM>(P, A, st); break;
+  default: launch_heat_law<LAW_GK>(P, A, st); break;
+  }
 }
 
 void launch_cfl2d(const DevParams &P, const double *mask, const double *ub, const double *vb,

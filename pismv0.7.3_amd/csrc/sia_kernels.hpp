@@ -35,6 +35,12 @@ struct VelArgs {
   double *u, *v, *w;
 };
 
+struct HeatArgs {
+  const double *z, *H, *mask, *E, *u, *v;
+  double *Sigma;             // [Mx][My][Mz]
+  int *err;
+};
+
 struct MassArgs {
   const double *dt;          // device scalar
   const double *H, *flux, *mask, *ub, *vb, *smb, *bmr;
@@ -58,6 +64,7 @@ int column_blocks(const DevParams &P);
 // after A.dmax_part[column_blocks(P) - 1]; all are reduced into A.dmax_bits
 void launch_column(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st, int extra_partials = 0);
 void launch_velocity_uvw(const DevParams &P, const VelArgs &A, hipStream_t st);
+void launch_strain_heating(const DevParams &P, const HeatArgs &A, hipStream_t st);
 void launch_cfl2d(const DevParams &P, const double *mask, const double *ub, const double *vb,
                   unsigned long long *cfl_bits, hipStream_t st);
 void launch_dt(const DevParams &P, const unsigned long long *dmax_bits, const unsigned long long *cfl_bits,

@@ -94,6 +94,10 @@ public:
   const double *velocity_v() { return field3(sia_velocity_v, "velocity_v"); }
   //! StressBalance::velocity_w() (PISMStressBalance.cc:138-140).
   const double *velocity_w() { return field3(sia_velocity_w, "velocity_w"); }
+  // StressBalance::volumetric_strain_heating() (PISMStressBalance.cc:146-148)
+  const double *volumetric_strain_heating() {
+    return field3(sia_volumetric_strain_heating, "volumetric_strain_heating");
+  }
 
   //! SSB_Modifier::stdout_report() (SSB_Modifier.hh:58).
   std::string stdout_report() { return ""; }

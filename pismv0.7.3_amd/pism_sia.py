@@ -47,6 +47,7 @@ SIGNATURES = {
     "sia_velocity_u": (C.c_int, [_vp, _PP]),
     "sia_velocity_v": (C.c_int, [_vp, _PP]),
     "sia_velocity_w": (C.c_int, [_vp, _PP]),
+    "sia_volumetric_strain_heating": (C.c_int, [_vp, _PP]),
     "sia_surface_gradient": (C.c_int, [_vp, _PP, _PP]),
     "sia_diffusivity_staggered": (C.c_int, [_vp, _PP]),
     "sia_theta_thksmooth": (C.c_int, [_vp, _PP, _PP]),
@@ -92,7 +93,7 @@ SIGNATURES = {
     "sia_ensemble_last_error": (C.c_char_p, [_vp]),
 }
 
-N_EVENTS = 8  # SIA_EV_COUNT
+N_EVENTS = 9  # SIA_EV_COUNT
 SG_NPARAM = 58144
 SG_KIN = 64
 SG_SAMPLE_DTYPE = np.dtype([("f", "<f4", 36), ("r", "<f4", 32), ("share", "<f4", 32), ("has", "<u4"),
@@ -238,6 +239,10 @@ class SIAFD:
 
     def velocity_w(self):
         return device_to_numpy(self._get(lib().sia_velocity_w), self.n3)
+
+    def volumetric_strain_heating(self):
+        """StressBalance::volumetric_strain_heating() of the last full update, W m-3."""
+        return device_to_numpy(self._get(lib().sia_volumetric_strain_heating), self.n3)
 
     def diffusivity_staggered(self):
         return device_to_numpy(self._get(lib().sia_diffusivity_staggered), 2 * self.n2)

@@ -25,6 +25,8 @@ def run_gpu(w, full=True, ub=None, vb=None, bmr=None, age=None):
     out.update({k: v for k, v in sia.bed_smoother_fields().items()})
     if full:
         out["u"], out["v"], out["w"] = sia.velocity_u(), sia.velocity_v(), sia.velocity_w()
+        out["Sigma"] = sia.volumetric_strain_heating()
+        out["_heat_inputs"] = (w.grid, w.params, f["H"], f["mask"], f["E"])
     sia.close()
     return out
 
@@ -112,4 +114,13 @@ def check_against_oracle(g, o, grid, full=True, tol=1e-12, ub=None, vb=None, bmr
         d = np.abs(g["w"] - o["w"])
         errs["w"] = float(np.max(d / np.maximum(sw, 1e-300)))
         assert np.all(d <= tol * sw + 1e-300), (label, "w", errs["w"])
+        if "Sigma" in g:
+            # strain heating of the library's own u, v against the oracle's
+            # restatement on the same u, v: only libm ulps may differ
+            import binding as oracle
+            gr, P, H, mask, E = g["_heat_inputs"]
+            ref = oracle.strain_heating(gr, P, H, mask, E, g["u"], g["v"])
+            errs["Sigma"] = rel_elementwise(g["Sigma"], ref, floor=1e-300)
+            assert errs["Sigma"] <= tol, (label, "Sigma", errs["Sigma"])
+            assert np.all(g["Sigma"][ref == 0.0] == 0.0)
     return errs

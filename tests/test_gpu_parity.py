@@ -34,6 +34,26 @@ def test_dome_full_update(S, oracle):
     assert 1.40 < g["D_max"] < 1.50
 
 
+def test_strain_heating_end_to_end(S, oracle):
+    """Sigma of a polythermal window against the oracle's whole full update
+    (its own u, v). u and v agree to 1e-12 of their summands, and Sigma is
+    built from their differences, so the bar is 1e-9 of the column's largest
+    Sigma; the exact check on the library's own u, v is in every full-update
+    test (parity_util.check_against_oracle)."""
+    say("[gpu] strain heating, polythermal window 48x48x401")
+    w = W.greenland_like(700, 748, 1180, 1228)
+    g = run_gpu(w)
+    o = run_oracle(oracle, w)
+    Mx, My, Mz = w.grid.Mx, w.grid.My, w.grid.Mz
+    a = g["Sigma"].reshape(Mx, My, Mz)
+    b = o["Sigma"].reshape(Mx, My, Mz)
+    scale = np.max(np.abs(b), axis=2, keepdims=True)
+    err = np.max(np.abs(a - b) / np.maximum(scale, 1e-300))
+    say("  max |dSigma| / column max: %.2e; Sigma max %.3e W m-3" % (err, b.max()))
+    assert b.max() > 0 and err < 1e-9
+    assert np.array_equal(a == 0.0, b == 0.0)
+
+
 @pytest.mark.parametrize("range_", [5e3, 0.0])
 def test_greenland_window_full_update(S, oracle, range_):
     say("[gpu] greenland_like window 64x80x401, bed_smoother_range", range_)
