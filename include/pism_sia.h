@@ -20,6 +20,10 @@
  *   SSB_Modifier::velocity_u()/velocity_v() (SSB_Modifier.hh:52-54)
  *                                                              sia_velocity_u/_v
  *   StressBalance::velocity_w() (PISMStressBalance.cc:138-140) sia_velocity_w
+ *   StressBalance::volumetric_strain_heating() (PISMStressBalance.cc:146-148,
+ *     computed on !fast updates, 102-104, 370-500)            sia_volumetric_strain_heating
+ *   SIAFD_Regional::compute_surface_gradient (regional.cc:45-109)
+ *                                                              sia_inputs.no_model_mask + usurfstore
  *   BedSmoother::preprocess_bed (PISMBedSmoother.cc:96-149),
  *     triggered by the bed state counter (SIAFD.cc:128-133)   sia_update (bed_state_counter)
  *   SIAFD diagnostics h_x, h_y, diffusivity_staggered, schoofs_theta,
@@ -163,6 +167,8 @@ typedef struct sia_inputs {      /* PISMVars names: SIAFD.cc:122, 220-222, 292, 
   const double *basal_melt_rate;      /* [Mx][My] m s-1 for w at the base, or NULL (w_0 = 0) */
   int64_t bed_state_counter;          /* IceModelVec::get_state_counter() of the bed; the
                                          smoother re-runs when it exceeds the stored one */
+  const double *no_model_mask;        /* [Mx][My] or NULL; with usurfstore: SIAFD_Regional's */
+  const double *usurfstore;           /* gradient override (regional.cc:45-109) */
 } sia_inputs;
 
 typedef struct sia_handle sia_handle;

@@ -60,6 +60,7 @@ def _load():
         "or_horizontal_velocity": (C.c_int, [G, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
         "or_vertical_velocity": (C.c_int, [G, _dp, _dp, _dp, _dp, _dp]),
         "or_hardness": (d, [P, d, d]),
+        "or_regional_gradient": (C.c_int, [G, _dp, _dp, _dp, _dp]),
         "or_strain_heating": (C.c_int, [G, P, _dp, _dp, _dp, _dp, _dp, _dp]),
         "or_sia_update": (C.c_int, [G, P, C.c_void_p, C.c_int]),
         "or_update_mask_surface": (C.c_int, [G, P, _dp, _dp, _dp, _dp]),
@@ -108,7 +109,8 @@ class OrFields(C.Structure):
         "topgsmooth", "maxtl", "C2", "C3", "C4",
         "theta", "thksmooth", "h_x", "h_y", "flux", "D_stag",
         "delta0", "delta1", "I0", "I1", "u", "v", "w")] + [
-        ("D_max", C.c_double), ("Nx", C.c_int), ("Ny", C.c_int), ("Sigma", C.c_void_p)]
+        ("D_max", C.c_double), ("Nx", C.c_int), ("Ny", C.c_int), ("Sigma", C.c_void_p),
+        ("no_model_mask", C.c_void_p), ("usurfstore", C.c_void_p)]
 
 
 # ---------------------------------------------------------------- scalars
@@ -188,7 +190,8 @@ def update_mask_surface(grid, params, bed, H):
     return mask, h
 
 
-def sia_update(grid, params, H, b, h, mask, E, full=True, age=None, ub=None, vb=None, bmr=None):
+def sia_update(grid, params, H, b, h, mask, E, full=True, age=None, ub=None, vb=None, bmr=None,
+               no_model_mask=None, usurfstore=None):
     """One SIAFD::update (+ w when full). Arrays are flat float64 in PISM layout.
 
     Returns a dict of all outputs and intermediates.
@@ -211,6 +214,7 @@ def sia_update(grid, params, H, b, h, mask, E, full=True, age=None, ub=None, vb=
     f = OrFields()
     f.H, f.b, f.h, f.mask, f.E = inp("H", H), inp("b", b), inp("h", h), inp("mask", mask), inp("E", E)
     f.age, f.ub, f.vb, f.bmr = inp("age", age), inp("ub", ub), inp("vb", vb), inp("bmr", bmr)
+    f.no_model_mask, f.usurfstore = inp("no_model_mask", no_model_mask), inp("usurfstore", usurfstore)
     for k in ("topgsmooth", "maxtl", "C2", "C3", "C4", "theta", "thksmooth", "h_x", "h_y",
               "flux", "D_stag", "delta0", "delta1", "I0", "I1", "u", "v", "w"):
         setattr(f, k, out[k].ctypes.data)

@@ -910,6 +910,54 @@ int or_vertical_velocity(const sia_grid *g, const double *mask, const double *u,
   return SIA_OK;
 }
 
+/* SIAFD_Regional::compute_surface_gradient (regional.cc:45-109): where the
+   no-model mask is set on any point of a staggered gradient's stencil, the
+   gradient comes from the stored surface usurfstore instead (zero where the
+   stencil leaves the domain). Mask and usurfstore neighbours wrap
+   periodically, as PISM's ghosts do. */
+int or_regional_gradient(const sia_grid *g, const double *nmm, const double *hst, double *h_x,
+                         double *h_y) {
+  const int Mx = g->Mx, My = g->My;
+  const double dx = g->dx, dy = g->dy;
+#define NM(a, c) nmm[I2(g, (a), (c))]
+#define HS(a, c) hst[I2(g, (a), (c))]
+  for (int i = 0; i < Mx; i++) {
+    for (int j = 0; j < My; j++) {
+      if (NM(i, j) > 0.5 || NM(i + 1, j) > 0.5) {
+        if (i < 0 || i + 1 > Mx - 1) {
+          h_x[IS(g, i, j, 0)] = 0.0;
+        } else {
+          h_x[IS(g, i, j, 0)] = (HS(i + 1, j) - HS(i, j)) / dx;
+        }
+      }
+      if (NM(i - 1, j + 1) > 0.5 || NM(i + 1, j + 1) > 0.5 || NM(i - 1, j) > 0.5 || NM(i + 1, j) > 0.5) {
+        if (i - 1 < 0 || j + 1 > My - 1 || i + 1 > Mx - 1) {
+          h_x[IS(g, i, j, 1)] = 0.0;
+        } else {
+          h_x[IS(g, i, j, 1)] = (+HS(i + 1, j + 1) + HS(i + 1, j) - HS(i - 1, j + 1) - HS(i - 1, j)) / (4.0 * dx);
+        }
+      }
+      if (NM(i, j + 1) > 0.5 || NM(i + 1, j + 1) > 0.5 || NM(i, j - 1) > 0.5 || NM(i + 1, j - 1) > 0.5) {
+        if (i < 0 || j + 1 > My - 1 || i + 1 > Mx - 1 || j - 1 < 0) {
+          h_y[IS(g, i, j, 0)] = 0.0;
+        } else {
+          h_y[IS(g, i, j, 0)] = (+HS(i + 1, j + 1) + HS(i, j + 1) - HS(i + 1, j - 1) - HS(i, j - 1)) / (4.0 * dy);
+        }
+      }
+      if (NM(i, j) > 0.5 || NM(i, j + 1) > 0.5) {
+        if (j < 0 || j + 1 > My - 1) {
+          h_y[IS(g, i, j, 1)] = 0.0;
+        } else {
+          h_y[IS(g, i, j, 1)] = (HS(i, j + 1) - HS(i, j)) / dy;
+        }
+      }
+    }
+  }
+#undef NM
+#undef HS
+  return SIA_OK;
+}
+
 /* D^2 of the shallow strain-rate approximation, PISMStressBalance.cc:309-311 */
 static double D2(double u_x, double u_y, double u_z, double v_x, double v_y, double v_z) {
   return 0.5 * ((u_x + v_y) * (u_x + v_y) + u_x * u_x + v_y * v_y +
@@ -977,6 +1025,10 @@ int or_sia_update(const sia_grid *g, const sia_params *P, or_fields *f, int full
   if (rc) return rc;
   rc = or_surface_gradient(g, P, f->H, f->b, f->h, f->mask, f->h_x, f->h_y);
   if (rc) return rc;
+  if (f->no_model_mask && f->usurfstore) {  /* SIAFD_Regional */
+    rc = or_regional_gradient(g, f->no_model_mask, f->usurfstore, f->h_x, f->h_y);
+    if (rc) return rc;
+  }
   rc = or_theta(g, P, f->Nx, f->Ny, f->h, f->topgsmooth, f->maxtl, f->C2, f->C3, f->C4, f->theta);
   if (rc) return rc;
   rc = or_thksmooth(g, f->h, f->H, f->mask, f->maxtl, f->topgsmooth, f->thksmooth);

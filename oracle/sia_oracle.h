@@ -95,11 +95,16 @@ typedef struct or_fields {
   double D_max;
   int Nx, Ny;
   double *Sigma;                       /* [Mx*My*Mz] strain heating when full, or NULL */
+  const double *no_model_mask, *usurfstore; /* SIAFD_Regional inputs, or NULL */
 } or_fields;
 
 /* SIAFD::update + (if full) StressBalance::compute_vertical_velocity. The bed
    smoother always runs (the oracle keeps no state). */
 int or_sia_update(const sia_grid *g, const sia_params *P, or_fields *f, int full);
+
+/* SIAFD_Regional::compute_surface_gradient's override (regional.cc:45-109) */
+int or_regional_gradient(const sia_grid *g, const double *nmm, const double *hst, double *h_x,
+                         double *h_y);
 
 /* FlowLaw::hardness_parameter of the selected law (flowlaws.cc:103-105, 336-351) */
 double or_hardness(const sia_params *P, double E, double p);

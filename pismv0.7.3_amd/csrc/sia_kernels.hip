@@ -299,6 +299,37 @@ __global__ void k_gradient(DevParams P, Fields2D F, double *__restrict__ hx, dou
   gradient_point(P, F, hx, hy, q);
 }
 
+// SIAFD_Regional::compute_surface_gradient's override (regional.cc:45-109): where
+// the no-model mask touches a staggered gradient's stencil, the gradient of the
+// stored surface usurfstore (zero where the stencil leaves the domain)
+__global__ void k_regional_gradient(DevParams P, const double *__restrict__ nmm, const double *__restrict__ hst,
+                                    double *__restrict__ hx, double *__restrict__ hy) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int Mx = P.Mx, My = P.My;
+  if (q >= (long)Mx * My) return;
+  const int i = (int)(q / My), j = (int)(q % My);
+#define NM(a, c) nmm[idx2(P, (a), (c))]
+#define HS(a, c) hst[idx2(P, (a), (c))]
+  if (NM(i, j) > 0.5 || NM(i + 1, j) > 0.5) {
+    hx[2 * q] = (i + 1 > Mx - 1) ? 0.0 : (HS(i + 1, j) - HS(i, j)) / P.dx;
+  }
+  if (NM(i - 1, j + 1) > 0.5 || NM(i + 1, j + 1) > 0.5 || NM(i - 1, j) > 0.5 || NM(i + 1, j) > 0.5) {
+    hx[2 * q + 1] = (i - 1 < 0 || j + 1 > My - 1 || i + 1 > Mx - 1)
+                        ? 0.0
+                        : (+HS(i + 1, j + 1) + HS(i + 1, j) - HS(i - 1, j + 1) - HS(i - 1, j)) / (4.0 * P.dx);
+  }
+  if (NM(i, j + 1) > 0.5 || NM(i + 1, j + 1) > 0.5 || NM(i, j - 1) > 0.5 || NM(i + 1, j - 1) > 0.5) {
+    hy[2 * q] = (j + 1 > My - 1 || i + 1 > Mx - 1 || j - 1 < 0)
+                    ? 0.0
+                    : (+HS(i + 1, j + 1) + HS(i, j + 1) - HS(i + 1, j - 1) - HS(i, j - 1)) / (4.0 * P.dy);
+  }
+  if (NM(i, j) > 0.5 || NM(i, j + 1) > 0.5) {
+    hy[2 * q + 1] = (j + 1 > My - 1) ? 0.0 : (HS(i, j + 1) - HS(i, j)) / P.dy;
+  }
+#undef NM
+#undef HS
+}
+
 // ---------------------------------------------------------------- theta, thksmooth
 // get_theta (PISMBedSmoother.cc:360-420) and get_smoothed_thk (286-336)
 __device__ __forceinline__ void theta_thk_point(const DevParams &P, const Fields2D &F, const double *__restrict__ topgs,
@@ -1032,6 +1063,12 @@ void launch_gradient(const DevParams &P, const double *H, const double *b, const
   const long n = (long)P.Mx * P.My;
   Fields2D F{H, b, h, mask};
   hipLaunchKernelGGL(k_gradient, dim3(blocks_for(n, 256)), dim3(256), 0, st, P, F, hx, hy);
+}
+
+void launch_regional_gradient(const DevParams &P, const double *nmm, const double *hst, double *hx, double *hy,
+                              hipStream_t st) {
+  const long n = (long)P.Mx * P.My;
+  hipLaunchKernelGGL(k_regional_gradient, dim3(blocks_for(n, 256)), dim3(256), 0, st, P, nmm, hst, hx, hy);
 }
 
 void launch_theta_thk(const DevParams &P, const double *H, const double *b, const double *h,

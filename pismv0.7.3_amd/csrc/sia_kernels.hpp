@@ -55,6 +55,8 @@ void launch_bed_smoother(const DevParams &P, double s2, double s3, double s4, co
                          double *mt, double *c2, double *c3, double *c4, hipStream_t st);
 void launch_gradient(const DevParams &P, const double *H, const double *b, const double *h,
                      const double *mask, double *hx, double *hy, hipStream_t st);
+void launch_regional_gradient(const DevParams &P, const double *nmm, const double *hst, double *hx, double *hy,
+                              hipStream_t st);
 void launch_theta_thk(const DevParams &P, const double *H, const double *b, const double *h,
                       const double *mask, const double *topgs, const double *maxtl, const double *C2,
                       const double *C3, const double *C4, double *theta, double *thks, int *err,

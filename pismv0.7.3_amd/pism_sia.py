@@ -211,11 +211,13 @@ class SIAFD:
         self._check(lib().sia_set_sync_check(self.h, 1 if on else 0))
 
     def update(self, H, b, h, mask, E, fast=False, age=None, ub=None, vb=None, bmr=None,
-               bed_state_counter=0):
-        """SIAFD::update(vel_input, fast) (SIAFD.cc:119-148) + w when not fast."""
+               bed_state_counter=0, no_model_mask=None, usurfstore=None):
+        """SIAFD::update(vel_input, fast) (SIAFD.cc:119-148) + strain heating and
+        w when not fast; SIAFD_Regional's gradient when no_model_mask and
+        usurfstore are given."""
         inp = T.SiaInputs(ptr(H), ptr(b), ptr(h), ptr(mask), ptr(E), ptr(age), ptr(ub), ptr(vb),
-                          ptr(bmr), bed_state_counter)
-        self._keep = (H, b, h, mask, E, age, ub, vb, bmr)
+                          ptr(bmr), bed_state_counter, ptr(no_model_mask), ptr(usurfstore))
+        self._keep = (H, b, h, mask, E, age, ub, vb, bmr, no_model_mask, usurfstore)
         self._check(lib().sia_update(self.h, C.byref(inp), 1 if fast else 0))
 
     def _get(self, fn):

@@ -182,7 +182,8 @@ class SiaInputs(C.Structure):
                 ("surface_altitude", C.c_void_p), ("mask", C.c_void_p),
                 ("enthalpy", C.c_void_p), ("age", C.c_void_p),
                 ("vel_input_u", C.c_void_p), ("vel_input_v", C.c_void_p),
-                ("basal_melt_rate", C.c_void_p), ("bed_state_counter", C.c_int64)]
+                ("basal_melt_rate", C.c_void_p), ("bed_state_counter", C.c_int64),
+                ("no_model_mask", C.c_void_p), ("usurfstore", C.c_void_p)]
 
 
 class SiaGeometry(C.Structure):

@@ -11,12 +11,13 @@ def say(*a):
     sys.stdout.flush()
 
 
-def run_gpu(w, full=True, ub=None, vb=None, bmr=None, age=None):
+def run_gpu(w, full=True, ub=None, vb=None, bmr=None, age=None, no_model_mask=None, usurfstore=None):
     import pism_sia as S
     f = w.fields
     dev = {k: S.to_device(v) for k, v in f.items()}
     extra = {k: (S.to_device(v) if v is not None else None) for k, v in
-             (("ub", ub), ("vb", vb), ("bmr", bmr), ("age", age))}
+             (("ub", ub), ("vb", vb), ("bmr", bmr), ("age", age), ("no_model_mask", no_model_mask),
+              ("usurfstore", usurfstore))}
     sia = S.SIAFD(w.grid, w.params)
     sia.update(dev["H"], dev["b"], dev["h"], dev["mask"], dev["E"], fast=not full, **extra)
     out = dict(flux=sia.diffusive_flux(), D_max=sia.max_diffusivity(), D_stag=sia.diffusivity_staggered())
@@ -31,10 +32,11 @@ def run_gpu(w, full=True, ub=None, vb=None, bmr=None, age=None):
     return out
 
 
-def run_oracle(oracle, w, full=True, ub=None, vb=None, bmr=None, age=None):
+def run_oracle(oracle, w, full=True, ub=None, vb=None, bmr=None, age=None, no_model_mask=None, usurfstore=None):
     f = w.fields
     return oracle.sia_update(w.grid, w.params, f["H"], f["b"], f["h"], f["mask"], f["E"], full=full,
-                             ub=ub, vb=vb, bmr=bmr, age=age)
+                             ub=ub, vb=vb, bmr=bmr, age=age, no_model_mask=no_model_mask,
+                             usurfstore=usurfstore)
 
 
 def rel_elementwise(a, b, floor=1e-300):

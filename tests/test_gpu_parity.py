@@ -54,6 +54,26 @@ def test_strain_heating_end_to_end(S, oracle):
     assert np.array_equal(a == 0.0, b == 0.0)
 
 
+def test_regional_gradient_override(S, oracle):
+    """SIAFD_Regional (regional.cc:45-109): a no-model strip and block with a
+    stored surface that differs from h; full update against the oracle."""
+    say("[gpu] SIAFD_Regional gradient override, window 40x44x401")
+    w = W.greenland_like(700, 740, 1180, 1224)
+    Mx, My = w.grid.Mx, w.grid.My
+    nmm = np.zeros((Mx, My))
+    nmm[:5, :] = 1.0
+    nmm[25:30, 20:26] = 1.0
+    rng = np.random.default_rng(4)
+    hst = w.fields["h"].reshape(Mx, My) + rng.uniform(-20.0, 20.0, (Mx, My))
+    kw = dict(no_model_mask=nmm.ravel(), usurfstore=hst.ravel())
+    g = run_gpu(w, **kw)
+    o = run_oracle(oracle, w, **kw)
+    errs = check_against_oracle(g, o, w.grid, label="regional")
+    say("  max rel errors:", {k: "%.1e" % v for k, v in errs.items()})
+    base = run_oracle(oracle, w, full=False)
+    assert not np.array_equal(o["h_x"], base["h_x"])  # the override took effect
+
+
 @pytest.mark.parametrize("range_", [5e3, 0.0])
 def test_greenland_window_full_update(S, oracle, range_):
     say("[gpu] greenland_like window 64x80x401, bed_smoother_range", range_)
