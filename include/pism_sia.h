@@ -24,6 +24,7 @@
  *     computed on !fast updates, 102-104, 370-500)            sia_volumetric_strain_heating
  *   SIAFD_Regional::compute_surface_gradient (regional.cc:45-109)
  *                                                              sia_inputs.no_model_mask + usurfstore
+ *   SIA_Sliding::update (SIA_Sliding.cc:106-206)              sia_sliding_update
  *   BedSmoother::preprocess_bed (PISMBedSmoother.cc:96-149),
  *     triggered by the bed state counter (SIAFD.cc:128-133)   sia_update (bed_state_counter)
  *   SIAFD diagnostics h_x, h_y, diffusivity_staggered, schoofs_theta,
@@ -249,6 +250,34 @@ int sia_mass_continuity_step(sia_handle *h, const sia_geometry *g, double dt,
    t_end <= 0 means no end. */
 int sia_step(sia_handle *h, const sia_geometry *g, const double *enthalpy,
              int fast, double t_end, double *t_device, double *dt_device);
+
+/* ---- SIA-type basal sliding: SIA_Sliding (SIA_Sliding.cc) --------------- */
+enum sia_sliding_mode {
+  SIA_SLIDING_PRESSURE_MELTING = 0, /* mu_sliding p_over where the base is within beta_CC rho g H of
+                                       minimum_temperature_for_sliding (SIA_Sliding.cc:260-266) */
+  SIA_SLIDING_TEST_E = 1,           /* sia_sliding_verification_mode (SIA_Sliding.cc:217-238) */
+  SIA_SLIDING_EISMINT2_G = 2,       /* EISMINT_II_experiment "G" (SIA_Sliding.cc:240-258) */
+  SIA_SLIDING_EISMINT2_H = 3        /* EISMINT_II_experiment "H" */
+};
+
+typedef struct sia_sliding_params {
+  int32_t mode;                          /* enum sia_sliding_mode */
+  int32_t reserved;
+  double mu_sliding;                     /* cdl:549 */
+  double minimum_temperature_for_sliding;/* cdl:477, K */
+  const double *x, *y;                   /* host: IceGrid::x(), y() (Mx and My values, m);
+                                            read for SIA_SLIDING_TEST_E only */
+} sia_sliding_params;
+
+/* SIA_Sliding::update (SIA_Sliding.cc:106-206), the ShallowStressBalance of
+   pismv's test E and of EISMINT II G/H: its own surface gradient (the
+   handle's surface_gradient_method), the sliding velocity U_b = -C grad h and
+   the basal frictional heating, written to the caller's device arrays
+   [Mx][My]. Pass u_b, v_b as vel_input_u/_v of the next sia_update, as
+   StressBalance::update does (PISMStressBalance.cc:89-96). Reads H, b, h, mask
+   and the basal enthalpy from in. */
+int sia_sliding_update(sia_handle *h, const sia_inputs *in, const sia_sliding_params *sp, double *u_b,
+                       double *v_b, double *basal_frictional_heating);
 
 /* ---- profiling: the reference's Profiling events (Profiling.hh:29-39) --------
    Event names as in SIAFD.cc:129-146, PISMStressBalance.cc:103-109 and

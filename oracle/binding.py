@@ -61,6 +61,8 @@ def _load():
         "or_vertical_velocity": (C.c_int, [G, _dp, _dp, _dp, _dp, _dp]),
         "or_hardness": (d, [P, d, d]),
         "or_regional_gradient": (C.c_int, [G, _dp, _dp, _dp, _dp]),
+        "or_sia_sliding": (C.c_int, [G, P, C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                     _dp, _dp, _dp, _dp, _dp]),
         "or_strain_heating": (C.c_int, [G, P, _dp, _dp, _dp, _dp, _dp, _dp]),
         "or_sia_update": (C.c_int, [G, P, C.c_void_p, C.c_int]),
         "or_update_mask_surface": (C.c_int, [G, P, _dp, _dp, _dp, _dp]),
@@ -226,6 +228,25 @@ def sia_update(grid, params, H, b, h, mask, E, full=True, age=None, ub=None, vb=
         for k in ("delta0", "delta1", "I0", "I1", "u", "v", "w", "Sigma"):
             del out[k]
     return out
+
+
+class OrSliding(C.Structure):
+    _fields_ = [("mode", C.c_int), ("mu_sliding", C.c_double), ("minimum_temperature_for_sliding", C.c_double)]
+
+
+SLIDING_MODES = {"pressure_melting": 0, "test_E": 1, "eismint2_G": 2, "eismint2_H": 3}
+
+
+def sia_sliding(grid, params, mode, x, y, H, h, b, mask, E, mu_sliding=0.0, min_T=273.0):
+    """SIA_Sliding::update: (u_b, v_b, basal frictional heating, h_x, h_y)."""
+    n2 = grid.Mx * grid.My
+    arr = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (x, y, H, h, b, mask, E)]
+    hx, hy = np.zeros(2 * n2), np.zeros(2 * n2)
+    ub, vb, heat = np.zeros(n2), np.zeros(n2), np.zeros(n2)
+    sp = OrSliding(SLIDING_MODES[mode], mu_sliding, min_T)
+    _check(lib().or_sia_sliding(C.byref(grid), C.byref(params), C.byref(sp), *[_ptr(a) for a in arr],
+                                _ptr(hx), _ptr(hy), _ptr(ub), _ptr(vb), _ptr(heat)))
+    return ub, vb, heat, hx, hy
 
 
 def hardness(params, E, p):

@@ -123,6 +123,16 @@ double or_ec_enthalpy(const sia_params *P, double T, double omega, double p) {
   return or_ec_enthalpy_cts(P, p) + omega * ec_L(P, T_melting);
 }
 
+/* EnthalpyConverter::enthalpy_permissive (enthalpyConverter.cc:307-320; cold: 331-335) */
+double or_ec_enthalpy_permissive(const sia_params *P, double T, double omega, double p) {
+  if (P->enthalpy_converter == SIA_EC_COLD) {
+    return P->ice_specific_heat_capacity * (T - P->enthalpy_converter_reference_temperature);
+  }
+  const double T_m = or_ec_melting_temperature(P, p);
+  if (T < T_m) return or_ec_enthalpy(P, T, 0.0, p);
+  return or_ec_enthalpy(P, T_m, fmax(0.0, fmin(omega, 1.0)), p);
+}
+
 /* is_temperate_impl with do_cold_ice_methods = no: enthalpyConverter.cc:156-162; cold: 351-353 */
 int or_ec_is_temperate(const sia_params *P, double E, double p) {
   if (P->enthalpy_converter == SIA_EC_COLD) {
@@ -907,6 +917,154 @@ int or_vertical_velocity(const sia_grid *g, const double *mask, const double *u,
     }
   }
 #undef MM
+  return SIA_OK;
+}
+
+/* SIA_Sliding::surface_gradient_haseloff (SIA_Sliding.cc:374-498): the
+   thickness-threshold form (H <= 0 is ice-free) with pseudo-surface
+   elevations; its mahaffy and eta variants equal SIAFD's (SIA_Sliding.cc:
+   292-372, 501-531). */
+static int sliding_gradient_haseloff(const sia_grid *g, const double *H, const double *b, const double *h,
+                                     double *h_x, double *h_y) {
+  const double Hicefree = 0.0, dx = g->dx, dy = g->dy;
+#define HH(a, c) H[I2(g, (a), (c))]
+#define BB(a, c) b[I2(g, (a), (c))]
+#define SS(a, c) h[I2(g, (a), (c))]
+  for (int i = 0; i < g->Mx; i++) {
+    for (int j = 0; j < g->My; j++) {
+      { /* i-offset */
+        const int icefreeP = HH(i, j) <= Hicefree, icefreeE = HH(i + 1, j) <= Hicefree,
+                  icefreeN = HH(i, j + 1) <= Hicefree, icefreeS = HH(i, j - 1) <= Hicefree,
+                  icefreeNE = HH(i + 1, j + 1) <= Hicefree, icefreeSE = HH(i + 1, j - 1) <= Hicefree;
+        double hhE = SS(i + 1, j);
+        if (icefreeE && (BB(i + 1, j) > SS(i, j))) hhE = SS(i, j);
+        if (icefreeP && (BB(i, j) > SS(i + 1, j))) hhE = SS(i, j);
+        h_x[IS(g, i, j, 0)] = (hhE - SS(i, j)) / dx;
+        double hhN = SS(i, j + 1);
+        if (icefreeN && (BB(i, j + 1) > SS(i, j))) hhN = SS(i, j);
+        if (icefreeP && (BB(i, j) > SS(i, j + 1))) hhN = SS(i, j);
+        double hhS = SS(i, j - 1);
+        if (icefreeS && (BB(i, j - 1) > SS(i, j))) hhS = SS(i, j);
+        if (icefreeP && (BB(i, j) > SS(i, j - 1))) hhS = SS(i, j);
+        double hhNE = SS(i + 1, j + 1);
+        if (icefreeNE && (BB(i + 1, j + 1) > SS(i + 1, j))) hhNE = SS(i + 1, j);
+        if (icefreeE && (BB(i + 1, j) > SS(i + 1, j + 1))) hhNE = SS(i + 1, j);
+        double hhSE = SS(i + 1, j - 1);
+        if (icefreeSE && (BB(i + 1, j - 1) > SS(i + 1, j))) hhSE = SS(i + 1, j);
+        if (icefreeE && (BB(i + 1, j) > SS(i + 1, j - 1))) hhSE = SS(i + 1, j);
+        h_y[IS(g, i, j, 0)] = (hhNE + hhN - hhSE - hhS) / (4.0 * dy);
+      }
+      { /* j-offset */
+        const int icefreeP = HH(i, j) <= Hicefree, icefreeN = HH(i, j + 1) <= Hicefree,
+                  icefreeE = HH(i + 1, j) <= Hicefree, icefreeW = HH(i - 1, j) <= Hicefree,
+                  icefreeNE = HH(i + 1, j + 1) <= Hicefree, icefreeNW = HH(i - 1, j + 1) <= Hicefree;
+        double hhN = SS(i, j + 1);
+        if (icefreeN && (BB(i, j + 1) > SS(i, j))) hhN = SS(i, j);
+        if (icefreeP && (BB(i, j) > SS(i, j + 1))) hhN = SS(i, j);
+        h_y[IS(g, i, j, 1)] = (hhN - SS(i, j)) / dy;
+        double hhE = SS(i + 1, j);
+        if (icefreeE && (BB(i + 1, j) > SS(i, j))) hhE = SS(i, j);
+        if (icefreeP && (BB(i, j) > SS(i + 1, j))) hhE = SS(i, j);
+        double hhW = SS(i - 1, j);
+        if (icefreeW && (BB(i - 1, j) > SS(i, j))) hhW = SS(i, j);
+        if (icefreeP && (BB(i, j) > SS(i - 1, j))) hhW = SS(i, j);
+        double hhNE = SS(i + 1, j + 1);
+        if (icefreeNE && (BB(i + 1, j + 1) > SS(i, j + 1))) hhNE = SS(i, j + 1);
+        if (icefreeN && (BB(i, j + 1) > SS(i + 1, j + 1))) hhNE = SS(i, j + 1);
+        double hhNW = SS(i - 1, j + 1);
+        if (icefreeNW && (BB(i - 1, j + 1) > SS(i, j + 1))) hhNW = SS(i, j + 1);
+        if (icefreeN && (BB(i, j + 1) > SS(i - 1, j + 1))) hhNW = SS(i, j + 1);
+        h_x[IS(g, i, j, 1)] = (hhNE + hhE - hhNW - hhW) / (4.0 * dx);
+      }
+    }
+  }
+#undef HH
+#undef BB
+#undef SS
+  return SIA_OK;
+}
+
+/* SIA_Sliding::basalVelocitySIA (SIA_Sliding.cc:209-267): the coefficient C of
+   U_b = -C grad h */
+static double basal_velocity_sia(const sia_params *P, const or_sliding *sp, double x_in, double y_in, double H,
+                                 double T) {
+  const double ice_rho = P->ice_density, g = P->standard_gravity,
+               beta_CC_grad = P->beta_CC * ice_rho * P->standard_gravity, secpera = P->seconds_per_year;
+  if (sp->mode == OR_SLIDING_TEST_E) {
+    const double r1 = 200e3, r2 = 700e3, theta1 = 10 * (M_PI / 180), theta2 = 40 * (M_PI / 180);
+    const double x = fabs(x_in), y = fabs(y_in);
+    const double r = sqrt(x * x + y * y);
+    double theta;
+    if (x < 1.0) {
+      theta = M_PI / 2.0;
+    } else {
+      theta = atan(y / x);
+    }
+    if ((r > r1) && (r < r2) && (theta > theta1) && (theta < theta2)) {
+      const double rbot = (r2 - r1) * (r2 - r1), thetabot = (theta2 - theta1) * (theta2 - theta1);
+      const double mu_max = 2.5e-11;
+      const double muE = mu_max * (4.0 * (r - r1) * (r2 - r) / rbot) *
+                         (4.0 * (theta - theta1) * (theta2 - theta) / thetabot);
+      return muE * ice_rho * g * H;
+    }
+    return 0.0;
+  }
+  if (sp->mode == OR_SLIDING_EISMINT2_G || sp->mode == OR_SLIDING_EISMINT2_H) {
+    const double Bfactor = 1e-3 / secpera;
+    const double pressure = or_ec_pressure(P, H);
+    const double E = or_ec_enthalpy_permissive(P, T, 0.0, pressure);
+    if (sp->mode == OR_SLIDING_EISMINT2_G) return Bfactor * ice_rho * g * H;
+    return or_ec_is_temperate(P, E, pressure) ? Bfactor * ice_rho * g * H : 0.0;
+  }
+  if (T + beta_CC_grad * H > sp->minimum_temperature_for_sliding) {
+    const double p_over = ice_rho * g * H;
+    return sp->mu_sliding * p_over;
+  }
+  return 0;
+}
+
+/* SIA_Sliding::update (SIA_Sliding.cc:106-206) */
+int or_sia_sliding(const sia_grid *g, const sia_params *P, const or_sliding *sp, const double *x, const double *y,
+                   const double *H, const double *h, const double *b, const double *mask, const double *E,
+                   double *h_x, double *h_y, double *ub, double *vb, double *heat) {
+  int rc;
+  switch (P->surface_gradient_method) {
+  case SIA_GRADIENT_ETA:
+    rc = gradient_eta(g, P, H, b, h_x, h_y);
+    break;
+  case SIA_GRADIENT_HASELOFF:
+    rc = sliding_gradient_haseloff(g, H, b, h, h_x, h_y);
+    break;
+  case SIA_GRADIENT_MAHAFFY:
+    rc = gradient_mahaffy(g, h, h_x, h_y);
+    break;
+  default:
+    return fail(SIA_ERR_CONFIG, "value of surface_gradient_method is not valid");
+  }
+  if (rc) return rc;
+  const double ice_rho = P->ice_density;
+  for (int i = 0; i < g->Mx; i++) {
+    for (int j = 0; j < g->My; j++) {
+      const size_t q = I2(g, i, j);
+      if (m_ocean(as_int(mask[q]))) {
+        ub[q] = 0.0;
+        vb[q] = 0.0;
+        heat[q] = 0.0;
+        continue;
+      }
+      const double myhx = 0.25 * (h_x[IS(g, i, j, 0)] + h_x[IS(g, i - 1, j, 0)] + h_x[IS(g, i, j, 1)] +
+                                  h_x[IS(g, i, j - 1, 1)]),
+                   myhy = 0.25 * (h_y[IS(g, i, j, 0)] + h_y[IS(g, i - 1, j, 0)] + h_y[IS(g, i, j, 1)] +
+                                  h_y[IS(g, i, j - 1, 1)]);
+      const double Hq = h[q] - b[q];
+      const double T = or_ec_temperature(P, E[q * (size_t)g->Mz], or_ec_pressure(P, Hq));
+      const double basalC = basal_velocity_sia(P, sp, x[i], y[j], Hq, T);
+      ub[q] = -basalC * myhx;
+      vb[q] = -basalC * myhy;
+      const double Pov = ice_rho * P->standard_gravity * Hq;
+      heat[q] = -(Pov * myhx) * ub[q] - (Pov * myhy) * vb[q];
+    }
+  }
   return SIA_OK;
 }
 

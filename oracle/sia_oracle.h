@@ -102,6 +102,24 @@ typedef struct or_fields {
    smoother always runs (the oracle keeps no state). */
 int or_sia_update(const sia_grid *g, const sia_params *P, or_fields *f, int full);
 
+/* SIA_Sliding (SIA_Sliding.cc): the sliding law's mode and parameters */
+enum { OR_SLIDING_PRESSURE_MELTING = 0, OR_SLIDING_TEST_E = 1, OR_SLIDING_EISMINT2_G = 2,
+       OR_SLIDING_EISMINT2_H = 3 };
+typedef struct or_sliding {
+  int mode;
+  double mu_sliding;                       /* cdl:549 */
+  double minimum_temperature_for_sliding;  /* cdl:477 */
+} or_sliding;
+
+/* SIA_Sliding::update (SIA_Sliding.cc:106-206): its own surface gradient
+   (h_x, h_y staggered work arrays), the sliding velocity and the basal
+   frictional heating; x[Mx], y[My] are the grid coordinates */
+int or_sia_sliding(const sia_grid *g, const sia_params *P, const or_sliding *sp, const double *x, const double *y,
+                   const double *H, const double *h, const double *b, const double *mask, const double *E,
+                   double *h_x, double *h_y, double *ub, double *vb, double *heat);
+
+double or_ec_enthalpy_permissive(const sia_params *P, double T, double omega, double p);
+
 /* SIAFD_Regional::compute_surface_gradient's override (regional.cc:45-109) */
 int or_regional_gradient(const sia_grid *g, const double *nmm, const double *hst, double *h_x,
                          double *h_y);

@@ -42,6 +42,7 @@ struct sia_handle {
   int *err = nullptr;
   double *dt_dev = nullptr, *acc_dev = nullptr, *partials = nullptr, *H_tmp = nullptr;
   double *dmax_part = nullptr;
+  double *sl_hx = nullptr, *sl_hy = nullptr, *sl_x = nullptr, *sl_y = nullptr;  // SIA_Sliding work
   const double *last_ub = nullptr, *last_vb = nullptr, *last_mask = nullptr;
   bool have_flux = false, have_3d = false;
   // learned surrogate for fast updates (borrowed) and its per-update buffers
@@ -635,7 +636,8 @@ int sia_destroy(sia_handle *h) {
   else (void)hipDeviceSynchronize();
   void *bufs[] = {h->d_z, h->topgs, h->maxtl, h->C2, h->C3, h->C4, h->hx, h->hy, h->theta, h->thks,
                   h->flux, h->Dstag, h->I0, h->I1, h->u, h->v, h->w, h->Sigma, h->ks, h->dmax_bits, h->cfl_bits, h->err,
-                  h->dt_dev, h->acc_dev, h->partials, h->H_tmp, h->dmax_part, h->sg_status,
+                  h->dt_dev, h->acc_dev, h->partials, h->H_tmp, h->dmax_part, h->sl_hx, h->sl_hy,
+                  h->sl_x, h->sl_y, h->sg_status,
                   h->sg_block_count, h->sg_list, h->sg_nlist, h->sg_counts, h->sg_F, h->sg_CQb, h->sg_CSt};
   for (void *b : bufs) dfree(b);
   for (auto &r : h->recs) {
@@ -823,6 +825,47 @@ int sia_step(sia_handle *h, const sia_geometry *g, const double *enthalpy, int f
   launch_dt(h->dp, h->dmax_bits, h->cfl_bits, t_end, t_device, dt_out, h->stream);
   HIP_TRY(h, hipGetLastError());
   return mass_step_impl(h, g, dt_out, nullptr);
+}
+
+int sia_sliding_update(sia_handle *h, const sia_inputs *in, const sia_sliding_params *sp, double *u_b,
+                       double *v_b, double *basal_frictional_heating) {
+  if (!h || !in || !sp || !u_b || !v_b || !basal_frictional_heating) return set_err(h, SIA_ERR_ARGUMENT, "NULL argument");
+  if (!in->land_ice_thickness || !in->bedrock_altitude || !in->surface_altitude || !in->mask || !in->enthalpy) {
+    return set_err(h, SIA_ERR_ARGUMENT, "sia_sliding_update: a required input field is NULL");
+  }
+  if (sp->mode < SIA_SLIDING_PRESSURE_MELTING || sp->mode > SIA_SLIDING_EISMINT2_H) {
+    return set_err(h, SIA_ERR_CONFIG, "unknown SIA sliding mode");
+  }
+  if (sp->mode == SIA_SLIDING_TEST_E && (!sp->x || !sp->y)) {
+    return set_err(h, SIA_ERR_ARGUMENT, "test E sliding needs the grid coordinates x, y");
+  }
+  const size_t n2 = (size_t)h->grid.Mx * h->grid.My;
+  int rc;
+  if ((rc = dalloc(h, &h->sl_hx, 2 * n2)) || (rc = dalloc(h, &h->sl_hy, 2 * n2))) return rc;
+  if (sp->mode == SIA_SLIDING_TEST_E) {
+    if ((rc = dalloc(h, &h->sl_x, (size_t)h->grid.Mx)) || (rc = dalloc(h, &h->sl_y, (size_t)h->grid.My))) return rc;
+    HIP_TRY(h, hipMemcpyAsync(h->sl_x, sp->x, sizeof(double) * h->grid.Mx, hipMemcpyHostToDevice, h->stream));
+    HIP_TRY(h, hipMemcpyAsync(h->sl_y, sp->y, sizeof(double) * h->grid.My, hipMemcpyHostToDevice, h->stream));
+  }
+  SlideArgs A{};
+  A.F = Fields2D{in->land_ice_thickness, in->bedrock_altitude, in->surface_altitude, in->mask};
+  A.E = in->enthalpy;
+  A.x = sp->mode == SIA_SLIDING_TEST_E ? h->sl_x : nullptr;
+  A.y = sp->mode == SIA_SLIDING_TEST_E ? h->sl_y : nullptr;
+  A.hx = h->sl_hx;
+  A.hy = h->sl_hy;
+  A.ub = u_b;
+  A.vb = v_b;
+  A.heat = basal_frictional_heating;
+  A.mode = sp->mode;
+  A.mu = sp->mu_sliding;
+  A.min_T = sp->minimum_temperature_for_sliding;
+  A.secpera = h->params.seconds_per_year;
+  launch_sia_sliding(h->dp, A, h->stream);
+  HIP_TRY(h, hipGetLastError());
+  // the host coordinates must stay valid until the copies ran
+  if (sp->mode == SIA_SLIDING_TEST_E) HIP_TRY(h, hipStreamSynchronize(h->stream));
+  return SIA_OK;
 }
 
 int sia_profiling(sia_handle *h, int on) {

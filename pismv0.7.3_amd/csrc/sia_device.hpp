@@ -104,6 +104,25 @@ __device__ __forceinline__ double ec_water_fraction(const DevParams &P, double E
   return (E - E_s) / ec_L(P, ec_melting_temperature(P, p));
 }
 
+// enthalpy_impl (enthalpyConverter.cc:256-281; cold: 337-340)
+__device__ __forceinline__ double ec_enthalpy(const DevParams &P, double T, double omega, double p) {
+  if (P.ec == EC_COLD) return P.ci * (T - P.T0);
+  const double T_m = ec_melting_temperature(P, p);
+  if (T < T_m) return P.ci * (T - P.T0);
+  return ec_enthalpy_cts(P, p) + omega * ec_L(P, T_m);
+}
+// enthalpy_permissive (enthalpyConverter.cc:307-320; cold: 331-335)
+__device__ __forceinline__ double ec_enthalpy_permissive(const DevParams &P, double T, double omega, double p) {
+  if (P.ec == EC_COLD) return P.ci * (T - P.T0);
+  const double T_m = ec_melting_temperature(P, p);
+  if (T < T_m) return ec_enthalpy(P, T, 0.0, p);
+  return ec_enthalpy(P, T_m, fmax(0.0, fmin(omega, 1.0)), p);
+}
+// is_temperate_impl (enthalpyConverter.cc:156-162; cold: 351-353)
+__device__ __forceinline__ bool ec_is_temperate(const DevParams &P, double E, double p) {
+  return P.ec != EC_COLD && E >= ec_enthalpy_cts(P, p);
+}
+
 // ---------------------------------------------------------------- flow laws
 // FlowLaw::softness_parameter_paterson_budd (flowlaws.cc:90-95)
 __device__ __forceinline__ double softness_pb(const DevParams &P, double T_pa) {

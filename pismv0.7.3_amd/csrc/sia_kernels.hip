@@ -742,6 +742,124 @@ __global__ __launch_bounds__(256) void k_velocity_uvw(DevParams P, VelArgs A, St
   }
 }
 
+// ---------------------------------------------------------------- SIA_Sliding
+// SIA_Sliding::surface_gradient_haseloff (SIA_Sliding.cc:374-498): thickness
+// threshold H <= 0 marks ice-free points, whose pseudo-surface is the
+// neighbour's where the bed stands above it. Its mahaffy and eta variants are
+// SIAFD's (SIA_Sliding.cc:292-372, 501-531).
+__global__ void k_sliding_gradient(DevParams P, SlideArgs A) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)P.Mx * P.My) return;
+  if (P.grad != 0) {
+    gradient_point(P, A.F, A.hx, A.hy, q);
+    return;
+  }
+  const int i = (int)(q / P.My), j = (int)(q % P.My);
+#define HH(a, c) A.F.H[idx2(P, (a), (c))]
+#define BB(a, c) A.F.b[idx2(P, (a), (c))]
+#define SS(a, c) A.F.h[idx2(P, (a), (c))]
+  {  // i-offset
+    const bool icefreeP = HH(i, j) <= 0.0, icefreeE = HH(i + 1, j) <= 0.0, icefreeN = HH(i, j + 1) <= 0.0,
+               icefreeS = HH(i, j - 1) <= 0.0, icefreeNE = HH(i + 1, j + 1) <= 0.0,
+               icefreeSE = HH(i + 1, j - 1) <= 0.0;
+    double hhE = SS(i + 1, j);
+    if (icefreeE && (BB(i + 1, j) > SS(i, j))) hhE = SS(i, j);
+    if (icefreeP && (BB(i, j) > SS(i + 1, j))) hhE = SS(i, j);
+    A.hx[2 * q] = (hhE - SS(i, j)) / P.dx;
+    double hhN = SS(i, j + 1);
+    if (icefreeN && (BB(i, j + 1) > SS(i, j))) hhN = SS(i, j);
+    if (icefreeP && (BB(i, j) > SS(i, j + 1))) hhN = SS(i, j);
+    double hhS = SS(i, j - 1);
+    if (icefreeS && (BB(i, j - 1) > SS(i, j))) hhS = SS(i, j);
+    if (icefreeP && (BB(i, j) > SS(i, j - 1))) hhS = SS(i, j);
+    double hhNE = SS(i + 1, j + 1);
+    if (icefreeNE && (BB(i + 1, j + 1) > SS(i + 1, j))) hhNE = SS(i + 1, j);
+    if (icefreeE && (BB(i + 1, j) > SS(i + 1, j + 1))) hhNE = SS(i + 1, j);
+    double hhSE = SS(i + 1, j - 1);
+    if (icefreeSE && (BB(i + 1, j - 1) > SS(i + 1, j))) hhSE = SS(i + 1, j);
+    if (icefreeE && (BB(i + 1, j) > SS(i + 1, j - 1))) hhSE = SS(i + 1, j);
+    A.hy[2 * q] = (hhNE + hhN - hhSE - hhS) / (4.0 * P.dy);
+  }
+  {  // j-offset
+    const bool icefreeP = HH(i, j) <= 0.0, icefreeN = HH(i, j + 1) <= 0.0, icefreeE = HH(i + 1, j) <= 0.0,
+               icefreeW = HH(i - 1, j) <= 0.0, icefreeNE = HH(i + 1, j + 1) <= 0.0,
+               icefreeNW = HH(i - 1, j + 1) <= 0.0;
+    double hhN = SS(i, j + 1);
+    if (icefreeN && (BB(i, j + 1) > SS(i, j))) hhN = SS(i, j);
+    if (icefreeP && (BB(i, j) > SS(i, j + 1))) hhN = SS(i, j);
+    A.hy[2 * q + 1] = (hhN - SS(i, j)) / P.dy;
+    double hhE = SS(i + 1, j);
+    if (icefreeE && (BB(i + 1, j) > SS(i, j))) hhE = SS(i, j);
+    if (icefreeP && (BB(i, j) > SS(i + 1, j))) hhE = SS(i, j);
+    double hhW = SS(i - 1, j);
+    if (icefreeW && (BB(i - 1, j) > SS(i, j))) hhW = SS(i, j);
+    if (icefreeP && (BB(i, j) > SS(i - 1, j))) hhW = SS(i, j);
+    double hhNE = SS(i + 1, j + 1);
+    if (icefreeNE && (BB(i + 1, j + 1) > SS(i, j + 1))) hhNE = SS(i, j + 1);
+    if (icefreeN && (BB(i, j + 1) > SS(i + 1, j + 1))) hhNE = SS(i, j + 1);
+    double hhNW = SS(i - 1, j + 1);
+    if (icefreeNW && (BB(i - 1, j + 1) > SS(i, j + 1))) hhNW = SS(i, j + 1);
+    if (icefreeN && (BB(i, j + 1) > SS(i - 1, j + 1))) hhNW = SS(i, j + 1);
+    A.hx[2 * q + 1] = (hhNE + hhE - hhNW - hhW) / (4.0 * P.dx);
+  }
+#undef HH
+#undef BB
+#undef SS
+}
+
+// SIA_Sliding::basalVelocitySIA (SIA_Sliding.cc:209-267): C in U_b = -C grad h
+__device__ __forceinline__ double basal_velocity_sia(const DevParams &P, const SlideArgs &A, double x_in,
+                                                     double y_in, double H, double T) {
+  const double ice_rho = P.rho_i, beta_CC_grad = P.beta * ice_rho * P.g;
+  if (A.mode == 1) {  // verification test E
+    const double r1 = 200e3, r2 = 700e3, theta1 = 10 * (M_PI / 180), theta2 = 40 * (M_PI / 180);
+    const double x = fabs(x_in), y = fabs(y_in);
+    const double r = sqrt(x * x + y * y);
+    const double theta = (x < 1.0) ? M_PI / 2.0 : atan(y / x);
+    if ((r > r1) && (r < r2) && (theta > theta1) && (theta < theta2)) {
+      const double rbot = (r2 - r1) * (r2 - r1), thetabot = (theta2 - theta1) * (theta2 - theta1);
+      const double mu_max = 2.5e-11;
+      const double muE = mu_max * (4.0 * (r - r1) * (r2 - r) / rbot) *
+                         (4.0 * (theta - theta1) * (theta2 - theta) / thetabot);
+      return muE * ice_rho * P.g * H;
+    }
+    return 0.0;
+  }
+  if (A.mode == 2 || A.mode == 3) {  // EISMINT II experiments G, H
+    const double Bfactor = 1e-3 / A.secpera;
+    const double pressure = ec_pressure(P, H);
+    const double E = ec_enthalpy_permissive(P, T, 0.0, pressure);
+    if (A.mode == 2) return Bfactor * ice_rho * P.g * H;
+    return ec_is_temperate(P, E, pressure) ? Bfactor * ice_rho * P.g * H : 0.0;
+  }
+  if (T + beta_CC_grad * H > A.min_T) return A.mu * (ice_rho * P.g * H);
+  return 0.0;
+}
+
+// SIA_Sliding::update (SIA_Sliding.cc:106-206)
+__global__ void k_sia_sliding(DevParams P, SlideArgs A) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)P.Mx * P.My) return;
+  if (m_ocean(as_int(A.F.mask[q]))) {
+    A.ub[q] = 0.0;
+    A.vb[q] = 0.0;
+    A.heat[q] = 0.0;
+    return;
+  }
+  const int i = (int)(q / P.My), j = (int)(q % P.My);
+  const long qw = idx2(P, i - 1, j), qs = idx2(P, i, j - 1);
+  const double myhx = 0.25 * (A.hx[2 * q] + A.hx[2 * qw] + A.hx[2 * q + 1] + A.hx[2 * qs + 1]),
+               myhy = 0.25 * (A.hy[2 * q] + A.hy[2 * qw] + A.hy[2 * q + 1] + A.hy[2 * qs + 1]);
+  const double H = A.F.h[q] - A.F.b[q];
+  const double T = ec_temperature(P, A.E[q * P.Mz], ec_pressure(P, H));
+  const double basalC = basal_velocity_sia(P, A, A.x ? A.x[i] : 0.0, A.y ? A.y[j] : 0.0, H, T);
+  const double u = -basalC * myhx, v = -basalC * myhy;
+  A.ub[q] = u;
+  A.vb[q] = v;
+  const double Pov = P.rho_i * P.g * H;
+  A.heat[q] = -(Pov * myhx) * u - (Pov * myhy) * v;
+}
+
 // ---------------------------------------------------------------- strain heating
 // StressBalance::compute_volumetric_strain_heating (PISMStressBalance.cc:370-500)
 // from the stored u, v. One wavefront per map column in the velocity kernel's
@@ -1104,6 +1222,12 @@ void launch_strain_heating(const DevParams &P, const HeatArgs &A, hipStream_t st
   case LAW_ARRWARM: launch_heat_law<LAW_ARRWARM>(P, A, st); break;
   default: launch_heat_law<LAW_GK>(P, A, st); break;
   }
+}
+
+void launch_sia_sliding(const DevParams &P, const SlideArgs &A, hipStream_t st) {
+  const long n = (long)P.Mx * P.My;
+  hipLaunchKernelGGL(k_sliding_gradient, dim3(blocks_for(n, 256)), dim3(256), 0, st, P, A);
+  hipLaunchKernelGGL(k_sia_sliding, dim3(blocks_for(n, 256)), dim3(256), 0, st, P, A);
 }
 
 void launch_cfl2d(const DevParams &P, const double *mask, const double *ub, const double *vb,

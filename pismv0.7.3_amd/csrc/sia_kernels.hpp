@@ -35,6 +35,18 @@ struct VelArgs {
   double *u, *v, *w;
 };
 
+// SIA_Sliding (SIA_Sliding.cc:106-267): mode 0 pressure-melting-activated
+// mu_sliding, 1 verification test E, 2 EISMINT II G, 3 EISMINT II H
+struct SlideArgs {
+  Fields2D F;                // H, b, h, mask
+  const double *E;           // [Mx][My][Mz] (the basal level is read)
+  const double *x, *y;       // [Mx], [My] grid coordinates (test E)
+  double *hx, *hy;           // [Mx][My][2] the sliding law's own surface gradient
+  double *ub, *vb, *heat;    // [Mx][My]
+  int mode;
+  double mu, min_T, secpera;
+};
+
 struct HeatArgs {
   const double *z, *H, *mask, *E, *u, *v;
   double *Sigma;             // [Mx][My][Mz]
@@ -67,6 +79,7 @@ int column_blocks(const DevParams &P);
 void launch_column(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st, int extra_partials = 0);
 void launch_velocity_uvw(const DevParams &P, const VelArgs &A, hipStream_t st);
 void launch_strain_heating(const DevParams &P, const HeatArgs &A, hipStream_t st);
+void launch_sia_sliding(const DevParams &P, const SlideArgs &A, hipStream_t st);
 void launch_cfl2d(const DevParams &P, const double *mask, const double *ub, const double *vb,
                   unsigned long long *cfl_bits, hipStream_t st);
 void launch_dt(const DevParams &P, const unsigned long long *dmax_bits, const unsigned long long *cfl_bits,

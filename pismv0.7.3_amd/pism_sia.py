@@ -48,6 +48,7 @@ SIGNATURES = {
     "sia_velocity_v": (C.c_int, [_vp, _PP]),
     "sia_velocity_w": (C.c_int, [_vp, _PP]),
     "sia_volumetric_strain_heating": (C.c_int, [_vp, _PP]),
+    "sia_sliding_update": (C.c_int, [_vp, C.POINTER(T.SiaInputs), C.c_void_p, _vp, _vp, _vp]),
     "sia_surface_gradient": (C.c_int, [_vp, _PP, _PP]),
     "sia_diffusivity_staggered": (C.c_int, [_vp, _PP]),
     "sia_theta_thksmooth": (C.c_int, [_vp, _PP, _PP]),
@@ -94,6 +95,12 @@ SIGNATURES = {
 }
 
 N_EVENTS = 9  # SIA_EV_COUNT
+SLIDING_MODES = {"pressure_melting": 0, "test_E": 1, "eismint2_G": 2, "eismint2_H": 3}
+
+
+class SiaSlidingParams(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("reserved", C.c_int32), ("mu_sliding", C.c_double),
+                ("minimum_temperature_for_sliding", C.c_double), ("x", C.c_void_p), ("y", C.c_void_p)]
 SG_NPARAM = 58144
 SG_KIN = 64
 SG_SAMPLE_DTYPE = np.dtype([("f", "<f4", 36), ("r", "<f4", 32), ("share", "<f4", 32), ("has", "<u4"),
@@ -291,6 +298,22 @@ class SIAFD:
     def step(self, geom, E, fast=False, t_end=0.0, t_dev=None, dt_dev=None):
         self._check(lib().sia_step(self.h, C.byref(geom), ptr(E), 1 if fast else 0, t_end,
                                    ptr(t_dev), ptr(dt_dev)))
+
+    def sliding(self, H, b, h, mask, E, mode="pressure_melting", x=None, y=None, mu_sliding=0.0,
+                min_T=273.0):
+        """SIA_Sliding::update (SIA_Sliding.cc:106-206): (u_b, v_b, basal
+        frictional heating) as device tensors; mode one of SLIDING_MODES."""
+        torch = _torch()
+        n2 = self.grid.Mx * self.grid.My
+        ub, vb, heat = (torch.empty(n2, dtype=torch.float64, device="cuda") for _ in range(3))
+        inp = T.SiaInputs(ptr(H), ptr(b), ptr(h), ptr(mask), ptr(E), None, None, None, None, 0, None, None)
+        xs = np.ascontiguousarray(x, dtype=np.float64) if x is not None else None
+        ys = np.ascontiguousarray(y, dtype=np.float64) if y is not None else None
+        sp = SiaSlidingParams(SLIDING_MODES[mode], 0, mu_sliding, min_T,
+                              xs.ctypes.data if xs is not None else None,
+                              ys.ctypes.data if ys is not None else None)
+        self._check(lib().sia_sliding_update(self.h, C.byref(inp), C.byref(sp), ptr(ub), ptr(vb), ptr(heat)))
+        return ub, vb, heat
 
     def use_surrogate(self, sg):
         """Fast updates use the learned surrogate `sg` (None: FP64 again)."""

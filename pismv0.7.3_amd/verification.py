@@ -1,5 +1,6 @@
-"""Verification test C of pismv (isothermal SIA with time-dependent SMB) on the
-HIP path: the known-answer end-to-end check of SIA + mass continuity.
+"""Verification tests C and E of pismv on the HIP path: the known-answer
+end-to-end checks of SIA + mass continuity (C: isothermal SIA with
+time-dependent SMB) and of the SIA-type sliding path (E: test_14.sh).
 
 Restated from the reference (paths relative to its root):
 * exactC, the Bueler et al. (2005) similarity solution with lambda = 5
@@ -69,6 +70,99 @@ def test_c_setup(Mx, My, Mz=31, Lx=1000e3, Lz=4000.0):
     return grid, P, r.ravel(), E0
 
 
+def exactE(x, y):
+    """exactTestsABCDE.c:183-251 (vectorised). Returns (H, M, mu, ub, vb) in
+    SI units; the closed form's year is SperA = 31556926 s."""
+    L, M0 = 750000.0, 0.3 / SPERA_EXACT
+    g, rho, n = 9.81, 910.0, 3.0
+    A = 1.0e-16 / SPERA_EXACT
+    Gamma = 2 * (rho * g) ** n * A / (n + 2)
+    mu_max, r1, r2 = 2.5e-11, 200e3, 700e3
+    theta1, theta2 = 10 * (math.pi / 180), 40 * (math.pi / 180)
+    rbot, thetabot = (r2 - r1) * (r2 - r1), (theta2 - theta1) * (theta2 - theta1)
+    xa, ya = np.abs(x), np.abs(y)
+    sgnx = np.where(np.asarray(x) < 0, -1.0, 1.0)
+    sgny = np.where(np.asarray(y) < 0, -1.0, 1.0)
+    r = np.sqrt(xa * xa + ya * ya)
+    m, q = 2.0 * n + 2.0, 1.0 + 1.0 / n
+    C = ((2.0 ** (n - 1)) * M0 / Gamma) ** (1.0 / m)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        chi = L ** q - r ** q
+        H = np.where(r < L, C * np.power(np.maximum(chi, 0.0), n / m), 0.0)
+        theta = np.where(xa < 1.0, math.pi / 2.0, np.arctan(ya / np.where(xa < 1.0, 1.0, xa)))
+        inside = (r < L) & ~((r <= r1) | (r >= r2) | (theta <= theta1) | (theta >= theta2))
+        mufactor = mu_max * (4.0 * (theta - theta1) * (theta2 - theta) / thetabot)
+        mu = np.where(inside, mufactor * (4.0 * (r - r1) * (r2 - r) / rbot), 0.0)
+        dchidr = -q * np.power(r, 1.0 / n)
+        dhdr = ((n * C) / m) * np.power(np.where(inside, chi, 1.0), (n - m) / m) * dchidr
+        P = rho * g * H
+        hx, hy = dhdr * np.cos(theta), dhdr * np.sin(theta)
+        ub = np.where(inside, sgnx * (-mu * P * hx), 0.0)
+        vb = np.where(inside, sgny * (-mu * P * hy), 0.0)
+        d2hdr2 = dhdr * (-((n + 2.0) / m) * dchidr / np.where(inside, chi, 1.0) + 1.0 / (n * r))
+        dmudr = mufactor * 4.0 * (r1 + r2 - 2.0 * r) / rbot
+        Mb = -P * ((mu / r) * H * dhdr + dmudr * H * dhdr + 2.0 * mu * dhdr * dhdr + mu * H * d2hdr2)
+        M = np.where(inside, M0 + Mb, M0)
+    return H, M, mu, ub, vb
+
+
+def test_e_setup(Mx, My, Mz=3, Lx=800e3, Lz=4000.0):
+    """pismv -test E (pismv.cc:84-88; iceCompModel.cc:196-213, 302-377): the
+    test C settings on an 800 km domain, sliding from SIA_Sliding in
+    verification mode; returns (grid, params, x, y, r, E0)."""
+    x = T.horizontal_coordinates(Lx, Mx)
+    y = T.horizontal_coordinates(Lx, My)
+    z = T.vertical_levels(Lz, Mz, "quadratic", 4.0)
+    grid = T.make_grid(Mx, My, T.horizontal_spacing(Lx, Mx), T.horizontal_spacing(Lx, My), z)
+    P = T.make_params(sia_flow_law="arr", sia_enhancement_factor=1.0, bed_smoother_range=0.0,
+                      is_dry_simulation=1, include_bmr_in_continuity=0, enthalpy_converter="cold")
+    A0 = 1.0e-16 / SECPERA
+    T0 = -P.Paterson_Budd_Q_cold / (P.ideal_gas_constant * (math.log(A0) - math.log(P.Paterson_Budd_A_cold)))
+    E0 = P.ice_specific_heat_capacity * (T0 - P.enthalpy_converter_reference_temperature)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    return grid, P, x, y, np.sqrt(X * X + Y * Y).ravel(), E0
+
+
+def geometry_errors_E(grid, H, x, y):
+    """computeGeometryErrors with exactE (iceCompModel.cc:595-744)."""
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    Hexact = exactE(X.ravel(), Y.ravel())[0]
+    return _geometry_errors(grid, H, Hexact)
+
+
+def basal_velocity_errors(grid, H, ub, vb, x, y):
+    """IceCompModel::computeBasalVelocityErrors (iceCompModel.cc:744-796) and the
+    printed line (iceCompModel.cc:1115-1127): maxvector, avvector (m/a),
+    prcntavvec, maxub, maxvb (m/a); the year of m/a is the model's."""
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    _, _, _, ube, vbe = exactE(X.ravel(), Y.ravel())
+    icy = H > 0.0
+    uberr = np.abs(ub - ube)[icy]
+    vberr = np.abs(vb - vbe)[icy]
+    vecerr = np.sqrt(uberr * uberr + vberr * vberr)
+    avvecerr = float(np.sum(vecerr)) / (grid.Mx * grid.My)
+    xpeak, ypeak = 450e3 * math.cos(25.0 * (math.pi / 180.0)), 450e3 * math.sin(25.0 * (math.pi / 180.0))
+    _, _, _, ubp, vbp = exactE(np.array([xpeak]), np.array([ypeak]))
+    exactmaxspeed = math.sqrt(ubp[0] ** 2 + vbp[0] ** 2)
+    return (float(vecerr.max()) * SECPERA, avvecerr * SECPERA, (avvecerr / exactmaxspeed) * 100.0,
+            float(uberr.max()) * SECPERA, float(vberr.max()) * SECPERA)
+
+
+def _geometry_errors(grid, H, Hexact):
+    Mx, My = grid.Mx, grid.My
+    a = grid.dx * grid.dy * 1e-3 * 1e-3
+    m = (2.0 * 3.0 + 2.0) / 3.0
+    vol = float(np.sum(a * H[H > 0] * 1e-3))
+    volexact = float(np.sum(a * Hexact[Hexact > 0] * 1e-3))
+    Herr = float(np.max(np.abs(H - Hexact)))
+    etaerr = float(np.max(np.abs(np.power(H, m) - np.power(Hexact, m))))
+    avHerr = float(np.sum(np.abs(H - Hexact))) / (Mx * My)
+    c = ((Mx - 1) // 2) * My + (My - 1) // 2
+    domeHexact = float(Hexact[c])
+    volerr = abs(vol - volexact)
+    return (100 * volerr / volexact, Herr, avHerr, etaerr / math.pow(domeHexact, m))
+
+
 def geometry_errors(grid, H, r, t):
     """IceCompModel::computeGeometryErrors for test C (iceCompModel.cc:595-744)
     and the printed ratios (965-972): prcntVOL, maxH, avH, relmaxETA."""
@@ -127,3 +221,41 @@ def run_test_c_gpu(Mx, My, Mz=31, years=5000.0):
     t, steps = run_time_loop(step, years)
     Hf = dev["H"].cpu().numpy()
     return geometry_errors(grid, Hf, r, t), steps, Hf
+
+
+def run_test_e_gpu(Mx, years=100.0):
+    """Test E through the HIP library (C ABI): SIA_Sliding (sia_sliding_update),
+    SIAFD with the sliding velocity, dt, mass continuity, then H = 0 beyond
+    750 km (iceCompModel.cc:478-497). Returns (geometry errors, base-velocity
+    errors, steps)."""
+    import pism_sia as S
+    import torch
+    import workloads as W
+    grid, P, x, y, r, E0 = test_e_setup(Mx, Mx)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    H0, M, _, _, _ = exactE(X.ravel(), Y.ravel())
+    b = np.zeros(Mx * Mx)
+    mask, h = W.geometry_mask_surface(P, b, H0)
+    dev = dict(H=S.to_device(H0), b=S.to_device(b), h=S.to_device(h), mask=S.to_device(mask),
+               E=S.to_device(np.full(Mx * Mx * grid.Mz, E0)))
+    smb = S.to_device(M * P.ice_density)
+    outside = torch.from_numpy(r > 750e3).to("cuda")
+    sia = S.SIAFD(grid, P)
+    geom = S.geometry(dev["H"], dev["h"], dev["mask"], dev["b"], smb=smb)
+    last = {}
+
+    def step(t, tte):
+        ub, vb, _ = sia.sliding(dev["H"], dev["b"], dev["h"], dev["mask"], dev["E"], mode="test_E", x=x, y=y)
+        sia.update(dev["H"], dev["b"], dev["h"], dev["mask"], dev["E"], fast=False, ub=ub, vb=vb)
+        dt = sia.max_timestep(tte)
+        sia.mass_continuity_step(geom, dt, accounting=False)
+        dev["H"][outside] = 0.0
+        last["ub"], last["vb"] = ub, vb
+        return dt
+
+    t, steps = run_time_loop(step, years)
+    Hf = dev["H"].cpu().numpy()
+    geo = geometry_errors_E(grid, Hf, x, y)
+    vel = basal_velocity_errors(grid, Hf, last["ub"].cpu().numpy(), last["vb"].cpu().numpy(), x, y)
+    sia.close()
+    return geo, vel, steps

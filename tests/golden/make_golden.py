@@ -1,6 +1,6 @@
 """Extract the expected outputs of PISM's regression scripts into JSON fixtures.
 
-Reads the heredoc blocks of test/regression/test_{15,20,21,22,24,25,26}.sh
+Reads the heredoc blocks of test/regression/test_{14,15,20,21,22,24,25,26}.sh
 under the reference tree (text only, nothing is executed) and writes
 tests/golden/regression_outputs.json. Run from the repository root:
 
@@ -48,6 +48,13 @@ def main(ref):
     out["test_C"] = {"script": "test/regression/test_15.sh",
                      "columns": ["prcntVOL", "maxH", "avH", "relmaxETA"],
                      "Mx31": rows[0], "Mx41": rows[1]}
+    te = heredoc(os.path.join(reg, "test_14.sh"))
+    rows = [ln.split() for ln in te if re.match(r"\s+[0-9]", ln)]
+    out["test_E"] = {"script": "test/regression/test_14.sh",
+                     "geometry": ["prcntVOL", "maxH", "avH", "relmaxETA"],
+                     "base_vels": ["maxvector", "avvector", "prcntavvec", "maxub", "maxvb"],
+                     "Mx21": {"geometry": rows[0], "base_vels": rows[1]},
+                     "Mx41": {"geometry": rows[2], "base_vels": rows[3]}}
     dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "regression_outputs.json")
     with open(dst, "w") as f:
         json.dump(out, f, indent=1)

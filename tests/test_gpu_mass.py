@@ -151,3 +151,20 @@ def test_gpu_reproduces_test_15_table(S, Mx, key):
     say("  errors", ["%.6f" % e for e in errs], "golden", gold[key])
     assert ["%.6f" % e for e in errs] == gold[key]
     assert steps == 84
+
+
+@pytest.mark.parametrize("Mx,key", [(21, "Mx21"), (41, "Mx41")])
+def test_gpu_reproduces_test_14_table(S, Mx, key):
+    """The HIP path (SIA_Sliding + SIAFD with sliding + mass continuity)
+    reproduces every printed digit of test_14's error tables
+    (test/regression/test_14.sh:17-35): 100 years of verification test E."""
+    import json
+    import verification as V
+    say("[gpu] verification test E", Mx)
+    gold = json.load(open(os.path.join(REPO, "tests", "golden", "regression_outputs.json")))["test_E"][key]
+    geo, vel, steps = V.run_test_e_gpu(Mx)
+    got_geo = ["%.6f" % e for e in geo]
+    got_vel = ["%.4f" % vel[0], "%.5f" % vel[1], "%.5f" % vel[2], "%.4f" % vel[3], "%.4f" % vel[4]]
+    say("  geometry", got_geo, "golden", gold["geometry"])
+    say("  base vels", got_vel, "golden", gold["base_vels"])
+    assert got_geo == gold["geometry"] and got_vel == gold["base_vels"]

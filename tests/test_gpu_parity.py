@@ -11,7 +11,7 @@ import pytest
 
 import sia_types as T
 import workloads as W
-from parity_util import check_against_oracle, run_gpu, run_oracle, say
+from parity_util import check_against_oracle, rel_elementwise, run_gpu, run_oracle, say
 
 pytestmark = pytest.mark.gpu
 
@@ -72,6 +72,31 @@ def test_regional_gradient_override(S, oracle):
     say("  max rel errors:", {k: "%.1e" % v for k, v in errs.items()})
     base = run_oracle(oracle, w, full=False)
     assert not np.array_equal(o["h_x"], base["h_x"])  # the override took effect
+
+
+@pytest.mark.parametrize("method", ["haseloff", "mahaffy", "eta"])
+def test_sia_sliding_modes(S, oracle, method):
+    """SIA_Sliding (SIA_Sliding.cc:106-267) in all four modes against the
+    oracle, on a margin window (ice-free cells exercise the thickness-threshold
+    haseloff gradient); 1e-12 relative."""
+    say("[gpu] SIA_Sliding, gradient", method)
+    w = W.greenland_like(150, 190, 1380, 1430, Mz=201, Lz=4000.0, surface_gradient_method=method)
+    Mx, My = w.grid.Mx, w.grid.My
+    f = w.fields
+    dev = {k: S.to_device(v) for k, v in f.items()}
+    x = 300e3 + np.arange(Mx) * w.grid.dx   # inside test E's sliding sector
+    y = 120e3 + np.arange(My) * w.grid.dy
+    sia = S.SIAFD(w.grid, w.params)
+    for mode in ("pressure_melting", "test_E", "eismint2_G", "eismint2_H"):
+        g = [t.cpu().numpy() for t in sia.sliding(dev["H"], dev["b"], dev["h"], dev["mask"], dev["E"], mode=mode,
+                                                  x=x, y=y, mu_sliding=1e-10, min_T=270.0)]
+        o = oracle.sia_sliding(w.grid, w.params, mode, x, y, f["H"], f["h"], f["b"], f["mask"], f["E"],
+                               mu_sliding=1e-10, min_T=270.0)[:3]
+        for name, a, b in zip(("ub", "vb", "heat"), g, o):
+            err = rel_elementwise(a, b, floor=1e-300)
+            assert err <= 1e-12, (mode, name, err)
+        assert np.abs(o[0]).max() > 0 or mode == "eismint2_H", mode
+    sia.close()
 
 
 @pytest.mark.parametrize("range_", [5e3, 0.0])
