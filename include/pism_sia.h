@@ -214,6 +214,17 @@ int sia_bed_smoother_fields(sia_handle *h, const double **topgsmooth, const doub
                             const double **C2, const double **C3, const double **C4,
                             int32_t *Nx, int32_t *Ny);
 
+/* SIAFD diagnostic "diffusivity" (SIAFD_diagnostics.cc:108-127): the staggered
+   D of the last update averaged to the regular grid (SIAFD.cc:736-744,
+   iceModelVec2.cc:679-689), written to the caller's device array [Mx][My]. */
+int sia_diffusivity(sia_handle *h, double *out2d);
+
+/* IceModelVec3::getSurfaceValues (iceModelVec3.cc:131-155, 206-222): a 3-D
+   device field [Mx][My][Mz] on the handle's levels interpolated linearly to
+   z = thickness[i][j] (end levels beyond the column), as sia.py's
+   computeSIASurfaceVelocities uses it (site-packages/PISM/sia.py:24-55). */
+int sia_surface_values(sia_handle *h, const double *field3d, const double *thickness, double *out2d);
+
 /* Maximum time step (s) after sia_update: min(maximum_time_step, time_to_end,
    diffusivity limit) as in IceModel::max_timestep with ZeroSliding
    (iMadaptive.cc:157-171, 214-388). time_to_end <= 0 means "no end". */

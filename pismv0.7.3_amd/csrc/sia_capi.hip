@@ -191,6 +191,7 @@ void fill_dev_params(sia_handle *h) {
     d.pb_coeff = beta_CC_grad / (p.ice_density * p.standard_gravity);            // flowlaws.cc:242
   }
   d.A_T0 = softness_pb_host(p, p.water_melting_point_temperature);  // flowlaws.cc:219
+  d.ln_A_T0 = std::log(d.A_T0);
   d.wf_coeff = p.gpbld_water_frac_coeff;
   d.wf_limit = p.gpbld_water_frac_observed_limit;
   d.A_glen = p.ice_softness;
@@ -204,6 +205,10 @@ void fill_dev_params(sia_handle *h) {
   d.e_pow = std::pow(p.sia_enhancement_factor, -1.0 / p.sia_Glen_exponent);  // PISMStressBalance.cc:388
   d.heat_exp = 0.5 * (1.0 / p.sia_Glen_exponent + 1.0);             // PISMStressBalance.cc:387
   d.B_glen = std::pow(p.ice_softness, d.hard_pow);                  // flowlaws.cc:252
+  d.ln_A_cold = std::log(p.Paterson_Budd_A_cold);
+  d.ln_A_warm = std::log(p.Paterson_Budd_A_warm);
+  d.ln_A_glen = std::log(p.ice_softness);
+  d.ln_hooke_A = std::log(p.Hooke_A);
   d.inv_ci = 1.0 / p.ice_specific_heat_capacity;
   d.inv_L = 1.0 / p.water_latent_heat_fusion;
   d.alpha_float = 1 - p.ice_density / p.sea_water_density;
@@ -825,6 +830,21 @@ int sia_step(sia_handle *h, const sia_geometry *g, const double *enthalpy, int f
   launch_dt(h->dp, h->dmax_bits, h->cfl_bits, t_end, t_device, dt_out, h->stream);
   HIP_TRY(h, hipGetLastError());
   return mass_step_impl(h, g, dt_out, nullptr);
+}
+
+int sia_surface_values(sia_handle *h, const double *field3d, const double *thickness, double *out2d) {
+  if (!h || !field3d || !thickness || !out2d) return set_err(h, SIA_ERR_ARGUMENT, "NULL argument");
+  launch_surface_values(h->dp, h->d_z, field3d, thickness, out2d, h->stream);
+  HIP_TRY(h, hipGetLastError());
+  return SIA_OK;
+}
+
+int sia_diffusivity(sia_handle *h, double *out2d) {
+  if (!h || !out2d) return set_err(h, SIA_ERR_ARGUMENT, "NULL argument");
+  if (!h->have_flux) return set_err(h, SIA_ERR_STATE, "no update has run");
+  launch_diffusivity_regular(h->dp, h->Dstag, out2d, h->stream);
+  HIP_TRY(h, hipGetLastError());
+  return SIA_OK;
 }
 
 int sia_sliding_update(sia_handle *h, const sia_inputs *in, const sia_sliding_params *sp, double *u_b,

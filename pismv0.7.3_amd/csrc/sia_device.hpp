@@ -47,6 +47,7 @@ struct DevParams {
   double e_pow;              // e^(-1/n)
   double heat_exp;           // (1/n + 1) / 2
   double B_glen;             // ice_softness^(-1/n)
+  double ln_A_cold, ln_A_warm, ln_A_T0, ln_A_glen, ln_hooke_A;  // logs of the softness constants
   // geometry / time stepping
   double alpha_float, sea_level, icefree_thk, floating_thk, max_dt, adapt_ratio;
 };
@@ -242,6 +243,38 @@ __device__ __forceinline__ double hardness(const DevParams &P, double E, double 
   } else {  // LAW_ARRWARM
     const double T_pa = ec_pressure_adjusted_temperature(P, E, p);
     return pow(P.A_warm * exp(-P.Q_warm / (P.R * T_pa)), P.hard_pow);
+  }
+}
+
+// ln of each law's softness_parameter (the A whose -1/n power is the hardness,
+// flowlaws.cc:103-105): for the Paterson-Budd family ln A_{c,w} - Q / (R T_pa)
+// in closed form, so that B = exp(-ln A / n) takes one exp instead of an exp
+// and a pow (equal to B to a few ulp)
+template <int LAW>
+__device__ __forceinline__ double log_softness(const DevParams &P, double E, double p) {
+  if (LAW == LAW_ISOGLEN) {
+    return P.ln_A_glen;
+  } else if (LAW == LAW_GPBLD) {
+    const double E_s = ec_enthalpy_cts(P, p);
+    if (E < E_s) {
+      const double T_pa = ec_pressure_adjusted_temperature(P, E, p);
+      const bool cold = T_pa < P.T_crit;
+      return (cold ? P.ln_A_cold : P.ln_A_warm) - (cold ? P.Q_cold : P.Q_warm) / (P.R * T_pa);
+    }
+    double omega = ec_water_fraction(P, E, p);
+    if (P.wf_limit < omega) omega = P.wf_limit;
+    return P.ln_A_T0 + log(1.0 + P.wf_coeff * omega);
+  } else if (LAW == LAW_PB || LAW == LAW_GK) {
+    const double T_pa = ec_pressure_adjusted_temperature(P, E, p);
+    const bool cold = T_pa < P.T_crit;
+    return (cold ? P.ln_A_cold : P.ln_A_warm) - (cold ? P.Q_cold : P.Q_warm) / (P.R * T_pa);
+  } else if (LAW == LAW_HOOKE) {
+    const double T_pa = ec_pressure_adjusted_temperature(P, E, p);
+    return P.ln_hooke_A - P.hooke_Q / (P.R * T_pa) + 3.0 * P.hooke_C * pow(P.hooke_Tr - T_pa, -P.hooke_k);
+  } else if (LAW == LAW_ARR) {
+    return P.ln_A_cold - P.Q_cold / (P.R * ec_pressure_adjusted_temperature(P, E, p));
+  } else {  // LAW_ARRWARM
+    return P.ln_A_warm - P.Q_warm / (P.R * ec_pressure_adjusted_temperature(P, E, p));
   }
 }
 

@@ -742,6 +742,45 @@ __global__ __launch_bounds__(256) void k_velocity_uvw(DevParams P, VelArgs A, St
   }
 }
 
+// ---------------------------------------------------------------- diagnostics
+// IceModelVec3::getSurfaceValues (iceModelVec3.cc:206-222) = getValZ at z = H
+// (iceModelVec3.cc:131-155): linear interpolation between the levels that
+// bracket z (GSL bsearch: largest m <= Mz - 2 with z_m <= z), the end levels
+// beyond the column.
+__global__ void k_surface_values(DevParams P, const double *__restrict__ z, const double *__restrict__ f,
+                                 const double *__restrict__ H, double *__restrict__ out) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)P.Mx * P.My) return;
+  const int Mz = P.Mz;
+  const double *col = f + q * Mz;
+  const double zq = H[q];
+  double r;
+  if (zq >= z[Mz - 1]) {
+    r = col[Mz - 1];
+  } else if (zq <= z[0]) {
+    r = col[0];
+  } else {
+    int lo = 0, hi = Mz - 1;
+    while (hi > lo + 1) {
+      const int mid = (lo + hi) >> 1;
+      if (z[mid] > zq) hi = mid; else lo = mid;
+    }
+    const double incr = (zq - z[lo]) / (z[lo + 1] - z[lo]);
+    const double valm = col[lo];
+    r = valm + incr * (col[lo + 1] - valm);
+  }
+  out[q] = r;
+}
+
+// SIAFD::compute_diffusivity (SIAFD.cc:736-744): the staggered D averaged to
+// the regular grid by IceModelVec2Stag::staggered_to_regular (iceModelVec2.cc:679-689)
+__global__ void k_diffusivity_regular(DevParams P, const double *__restrict__ D, double *__restrict__ out) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)P.Mx * P.My) return;
+  const int i = (int)(q / P.My), j = (int)(q % P.My);
+  out[q] = 0.25 * (D[2 * q] + D[2 * q + 1] + D[2 * idx2(P, i, j - 1) + 1] + D[2 * idx2(P, i - 1, j)]);
+}
+
 // ---------------------------------------------------------------- SIA_Sliding
 // SIA_Sliding::surface_gradient_haseloff (SIA_Sliding.cc:374-498): thickness
 // threshold H <= 0 marks ice-free points, whose pseudo-surface is the
@@ -871,12 +910,19 @@ __device__ __forceinline__ double strain_D2(double u_x, double u_y, double u_z, 
                 0.5 * ((u_y + v_x) * (u_y + v_x) + u_z * u_z + v_z * v_z));
 }
 
+// Sigma = 2 e^(-1/n) B D^(1+1/n) is evaluated as 2 e^(-1/n) exp(-ln A / n +
+// (1/n + 1)/2 ln D^2): one exp and one log a level instead of the reference's
+// exp and two pow; 1 / (z_{k+1} - z_{k-1}) is staged with the levels.
 template <int LAW>
 __global__ __launch_bounds__(256) void k_strain_heating(DevParams P, HeatArgs A, StripMap sm) {
-  extern __shared__ double zs[];
+  extern __shared__ double zs[];  // z [Mz], then 1 / (z_{k+1} - z_{k-1}) [Mz] (k = 0: 1 / (z_1 - z_0))
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int Mx = P.Mx, My = P.My, Mz = P.Mz;
-  for (int k = threadIdx.x; k < Mz; k += blockDim.x) zs[k] = A.z[k];
+  double *idz = zs + Mz;
+  for (int k = threadIdx.x; k < Mz; k += blockDim.x) {
+    zs[k] = A.z[k];
+    idz[k] = (k == 0) ? 1.0 / (A.z[1] - A.z[0]) : ((k + 1 < Mz) ? 1.0 / (A.z[k + 1] - A.z[k - 1]) : 0.0);
+  }
   __syncthreads();
   const int x = blockIdx.x % N_XCD;
   const int Lp = __builtin_amdgcn_readfirstlane((blockIdx.x / N_XCD) * WAVES_PER_BLOCK + wv);
@@ -909,22 +955,16 @@ __global__ __launch_bounds__(256) void k_strain_heating(DevParams P, HeatArgs A,
     double sig = 0.0;
     if (k <= ks) {
       const double pressure = ec_pressure(P, Hij - zs[k]);
-      const double B = hardness<LAW>(P, E_ij[k], pressure);
+      const double lnA = log_softness<LAW>(P, E_ij[k], pressure);
       const double u_x = D_x * (west * (u_ij[k] - u_w[k]) + east * (u_e[k] - u_ij[k])),
                    u_y = D_y * (south * (u_ij[k] - u_s[k]) + north * (u_n[k] - u_ij[k])),
                    v_x = D_x * (west * (v_ij[k] - v_w[k]) + east * (v_e[k] - v_ij[k])),
                    v_y = D_y * (south * (v_ij[k] - v_s[k]) + north * (v_n[k] - v_ij[k]));
-      double u_z, v_z;
-      if (k > 0) {  // ks <= Mz - 2, so level k + 1 exists
-        const double dz = zs[k + 1] - zs[k - 1];
-        u_z = (u_ij[k + 1] - u_ij[k - 1]) / dz;
-        v_z = (v_ij[k + 1] - v_ij[k - 1]) / dz;
-      } else {
-        const double dz = zs[1] - zs[0];
-        u_z = (u_ij[1] - u_ij[0]) / dz;
-        v_z = (v_ij[1] - v_ij[0]) / dz;
-      }
-      sig = 2.0 * P.e_pow * B * pow(strain_D2(u_x, u_y, u_z, v_x, v_y, v_z), P.heat_exp);
+      // ks <= Mz - 2, so level k + 1 exists; one-sided at the base
+      const int kl = (k > 0) ? k - 1 : 0, ku = (k > 0) ? k + 1 : 1;
+      const double u_z = (u_ij[ku] - u_ij[kl]) * idz[k], v_z = (v_ij[ku] - v_ij[kl]) * idz[k];
+      const double d2 = strain_D2(u_x, u_y, u_z, v_x, v_y, v_z);
+      sig = (d2 > 0.0) ? 2.0 * P.e_pow * exp(P.hard_pow * lnA + P.heat_exp * log(d2)) : 0.0;
     }
     if (k < Mz) Sg[k] = sig;
   }
@@ -1208,8 +1248,8 @@ void launch_velocity_uvw(const DevParams &P, const VelArgs &A, hipStream_t st) {
 template <int LAW>
 static void launch_heat_law(const DevParams &P, const HeatArgs &A, hipStream_t st) {
   const StripMap sm = strip_map(P.Mx, P.My, WAVES_PER_BLOCK);
-  hipLaunchKernelGGL((k_strain_heating<LAW>), dim3(sm.per_xcd * N_XCD), dim3(256), sizeof(double) * (size_t)P.Mz,
-                     st, P, A, sm);
+  hipLaunchKernelGGL((k_strain_heating<LAW>), dim3(sm.per_xcd * N_XCD), dim3(256),
+                     2 * sizeof(double) * (size_t)P.Mz, st, P, A, sm);
 }
 
 void launch_strain_heating(const DevParams &P, const HeatArgs &A, hipStream_t st) {
@@ -1222,6 +1262,17 @@ void launch_strain_heating(const DevParams &P, const HeatArgs &A, hipStream_t st
   case LAW_ARRWARM: launch_heat_law<LAW_ARRWARM>(P, A, st); break;
   default: launch_heat_law<LAW_GK>(P, A, st); break;
   }
+}
+
+void launch_surface_values(const DevParams &P, const double *z, const double *f, const double *H, double *out,
+                           hipStream_t st) {
+  const long n = (long)P.Mx * P.My;
+  hipLaunchKernelGGL(k_surface_values, dim3(blocks_for(n, 256)), dim3(256), 0, st, P, z, f, H, out);
+}
+
+void launch_diffusivity_regular(const DevParams &P, const double *D, double *out, hipStream_t st) {
+  const long n = (long)P.Mx * P.My;
+  hipLaunchKernelGGL(k_diffusivity_regular, dim3(blocks_for(n, 256)), dim3(256), 0, st, P, D, out);
 }
 
 void launch_sia_sliding(const DevParams &P, const SlideArgs &A, hipStream_t st) {

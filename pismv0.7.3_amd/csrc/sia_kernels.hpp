@@ -80,6 +80,9 @@ void launch_column(const DevParams &P, const ColumnArgs &A, bool full, hipStream
 void launch_velocity_uvw(const DevParams &P, const VelArgs &A, hipStream_t st);
 void launch_strain_heating(const DevParams &P, const HeatArgs &A, hipStream_t st);
 void launch_sia_sliding(const DevParams &P, const SlideArgs &A, hipStream_t st);
+void launch_surface_values(const DevParams &P, const double *z, const double *f, const double *H, double *out,
+                           hipStream_t st);
+void launch_diffusivity_regular(const DevParams &P, const double *D, double *out, hipStream_t st);
 void launch_cfl2d(const DevParams &P, const double *mask, const double *ub, const double *vb,
                   unsigned long long *cfl_bits, hipStream_t st);
 void launch_dt(const DevParams &P, const unsigned long long *dmax_bits, const unsigned long long *cfl_bits,

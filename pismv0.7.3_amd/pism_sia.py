@@ -49,6 +49,8 @@ SIGNATURES = {
     "sia_velocity_w": (C.c_int, [_vp, _PP]),
     "sia_volumetric_strain_heating": (C.c_int, [_vp, _PP]),
     "sia_sliding_update": (C.c_int, [_vp, C.POINTER(T.SiaInputs), C.c_void_p, _vp, _vp, _vp]),
+    "sia_diffusivity": (C.c_int, [_vp, _vp]),
+    "sia_surface_values": (C.c_int, [_vp, _vp, _vp, _vp]),
     "sia_surface_gradient": (C.c_int, [_vp, _PP, _PP]),
     "sia_diffusivity_staggered": (C.c_int, [_vp, _PP]),
     "sia_theta_thksmooth": (C.c_int, [_vp, _PP, _PP]),
@@ -314,6 +316,29 @@ class SIAFD:
                               ys.ctypes.data if ys is not None else None)
         self._check(lib().sia_sliding_update(self.h, C.byref(inp), C.byref(sp), ptr(ub), ptr(vb), ptr(heat)))
         return ub, vb, heat
+
+    def diffusivity(self):
+        """SIAFD's "diffusivity" diagnostic (regular grid), numpy [Mx*My]."""
+        torch = _torch()
+        out = torch.empty(self.n2, dtype=torch.float64, device="cuda")
+        self._check(lib().sia_diffusivity(self.h, ptr(out)))
+        return out.cpu().numpy()
+
+    def surface_values(self, field3d, thickness):
+        """IceModelVec3::getSurfaceValues of a device 3-D field (tensor or raw
+        device pointer) at z = thickness."""
+        torch = _torch()
+        out = torch.empty(self.n2, dtype=torch.float64, device="cuda")
+        fp = C.c_void_p(field3d) if isinstance(field3d, int) else ptr(field3d)
+        self._check(lib().sia_surface_values(self.h, fp, ptr(thickness), ptr(out)))
+        return out
+
+    def velocity_u_device(self):
+        """Device pointer of u (owned by the library, valid until the next update)."""
+        return self._get(lib().sia_velocity_u)
+
+    def velocity_v_device(self):
+        return self._get(lib().sia_velocity_v)
 
     def use_surrogate(self, sg):
         """Fast updates use the learned surrogate `sg` (None: FP64 again)."""
