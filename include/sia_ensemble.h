@@ -33,7 +33,14 @@ typedef struct sia_ensemble sia_ensemble;
    grain-size/age coupling is off. geometry[i], enthalpy[i]: member i's device
    fields ([Mx][My] and [Mx][My][Mz]; H, h and mask are updated in place by the
    steps). t_end: host array of n end times in seconds (<= 0: no end), or NULL.
-   Member times start at 0. */
+   Member times start at 0.
+   A member bound to a learned surrogate (sia_use_surrogate, include/
+   sia_surrogate.h) takes its fast updates through it, as sia_step would: the
+   surrogate stages run with one launch per stage for all members bound to the
+   same surrogate, and the FP64 column stage takes only their fallback columns.
+   The binding is read here; sia_ensemble_step returns SIA_ERR_STATE once a
+   member's binding differs from it. A surrogate retrained while bound is
+   picked up by the next sia_ensemble_step. */
 int sia_ensemble_create(sia_handle *const *members, const sia_geometry *geometry, const double *const *enthalpy,
                         const double *t_end, int n, sia_ensemble **out);
 int sia_ensemble_destroy(sia_ensemble *e);

@@ -71,6 +71,7 @@ struct sia_surrogate {
   SgNorm norm{};
   bool have_norm = false;
   double glen_n = 0.0;  // Glen exponent of the training samples (0: none yet)
+  int version = 0;      // bumped when the normalization changes (ensembles copy it)
   SgSample *samples = nullptr;
   long long *d_n = nullptr;
   int64_t cap = 0, n = 0;
@@ -1064,6 +1065,7 @@ int sia_surrogate_fit_normalization(sia_surrogate *sg) {
   const double rms = rn > 0.0 ? std::sqrt(r2 / rn) : 0.0;
   sg->norm.rscale = (float)(rms > 0.0 ? 4.0 * rms : (rmax > 0.0 ? rmax : 1.0));
   sg->have_norm = true;
+  sg->version++;
   return SIA_OK;
 }
 
@@ -1165,6 +1167,7 @@ int sia_surrogate_load(sia_surrogate *sg, const void *blob, size_t size) {
   const char *b = (const char *)blob;
   std::memcpy(&sg->norm, b + sizeof(hdr), sizeof(SgNorm));
   sg->have_norm = hdr.have_norm != 0;
+  sg->version++;
   sg->glen_n = hdr.glen_n;
   return sia_surrogate_set_params(sg, (const float *)(b + sizeof(hdr) + sizeof(SgNorm)), SG_NPARAM);
 }
@@ -1220,7 +1223,23 @@ struct sia_ensemble {
   int *d_pt_start = nullptr, *d_col_start[3] = {nullptr, nullptr, nullptr};
   double *d_t = nullptr, *d_dt = nullptr, *d_part = nullptr;
   int *d_err_acc = nullptr;
-  std::vector<size_t> part_off;        // member offsets into d_part (sized for 8 columns per wave)
+  std::vector<size_t> part_off;        // member offsets into d_part (sized for 8 columns per wave + MLP)
+  // members bound to a surrogate (sia_use_surrogate on the member handle before
+  // sia_ensemble_create): one group per surrogate, whose stages run between
+  // the gradient and the (fallback-only) column stage
+  struct SgGroup {
+    sia_surrogate *sg = nullptr;
+    std::vector<int> idx;               // ensemble member indices
+    std::vector<EnsSgMember> table;
+    EnsSgMember *d_table = nullptr;
+    std::vector<int> start[3];          // block prefixes: classify, features, MLP
+    int *d_start[3] = {nullptr, nullptr, nullptr};
+    int feat_waves = 4, nm1 = 2, max_mz = 0;
+    int version = 0;                    // sg->version the table's normalization copies
+  };
+  std::vector<SgGroup> groups;
+  std::vector<sia_surrogate *> bound;   // each member's surrogate at creation (nullptr: FP64)
+  std::vector<int> mlp_part;            // MLP D_max partials of each member
   hipStream_t own = nullptr, user = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   hipGraph_t graph = nullptr;
@@ -1250,8 +1269,17 @@ int ens_err(sia_ensemble *e, int code, const std::string &msg) {
 int ens_upload(sia_ensemble *e) {
   const int c = cpw_slot(e->cpw);
   for (int i = 0; i < e->n; i++) {
+    const int ncol = e->col_start[c][i + 1] - e->col_start[c][i];
     e->table[i].A.dmax_part = e->d_part + e->part_off[i];
-    e->table[i].n_part = e->col_start[c][i + 1] - e->col_start[c][i];
+    e->table[i].n_part = ncol + e->mlp_part[i];
+  }
+  for (auto &g : e->groups) {
+    for (size_t k = 0; k < g.idx.size(); k++) {
+      const int i = g.idx[k];
+      g.table[k].M.dmax_part = e->table[i].A.dmax_part + (e->col_start[c][i + 1] - e->col_start[c][i]);
+    }
+    ENS_TRY(e, hipMemcpyAsync(g.d_table, g.table.data(), sizeof(EnsSgMember) * g.table.size(),
+                              hipMemcpyHostToDevice, e->own));
   }
   ENS_TRY(e, hipMemcpyAsync(e->d_table, e->table.data(), sizeof(EnsMember) * e->n, hipMemcpyHostToDevice, e->own));
   ENS_TRY(e, hipStreamSynchronize(e->own));
@@ -1271,6 +1299,29 @@ EnsLaunch ens_launch(const sia_ensemble *e) {
   L.law = e->law;
   L.max_mz = e->max_mz;
   return L;
+}
+
+// one ensemble step: gradient, the surrogate stages of each group, then the
+// column stage (fallback columns only for surrogate-bound members), dt, mass
+// continuity and the surface/mask update
+void ens_issue_step(const sia_ensemble *e, const EnsLaunch &L, hipStream_t st) {
+  launch_ens_front(L, st);
+  for (const auto &g : e->groups) {
+    EnsSgLaunch G{};
+    G.table = g.d_table;
+    G.cls_start = g.d_start[0];
+    G.feat_start = g.d_start[1];
+    G.mlp_start = g.d_start[2];
+    G.nm = (int)g.idx.size();
+    G.cls_blocks = g.start[0].back();
+    G.feat_blocks = g.start[1].back();
+    G.mlp_blocks = g.start[2].back();
+    G.feat_waves = g.feat_waves;
+    G.nm1 = g.nm1;
+    G.max_mz = g.max_mz;
+    launch_ens_sg(G, st);
+  }
+  launch_ens_back(L, st);
 }
 
 void ens_drop_graph(sia_ensemble *e) {
@@ -1312,6 +1363,8 @@ int sia_ensemble_create(sia_handle *const *members, const sia_geometry *geometry
   e->pt_start.assign(n + 1, 0);
   for (int c = 0; c < 3; c++) e->col_start[c].assign(n + 1, 0);
   e->part_off.assign(n, 0);
+  e->bound.assign(n, nullptr);
+  e->mlp_part.assign(n, 0);
   size_t part_total = 0;
   long long pt_total = 0, col_total[3] = {0, 0, 0};
   for (int i = 0; i < n; i++) {
@@ -1337,14 +1390,45 @@ int sia_ensemble_create(sia_handle *const *members, const sia_geometry *geometry
       return SIA_ERR_STATE;
     }
     e->max_mz = std::max(e->max_mz, h->grid.Mz);
+    e->bound[i] = h->sg;
+    if (h->sg) {
+      size_t gi = 0;
+      while (gi < e->groups.size() && e->groups[gi].sg != h->sg) gi++;
+      if (gi == e->groups.size()) {
+        e->groups.emplace_back();
+        e->groups.back().sg = h->sg;
+        e->groups.back().nm1 = h->dp.nm1_int;
+        e->groups.back().version = h->sg->version;
+      }
+      e->groups[gi].idx.push_back(i);
+      e->groups[gi].max_mz = std::max(e->groups[gi].max_mz, h->grid.Mz);
+    }
     e->pt_start[i] = (int)pt_total;
     pt_total += ens_point_blocks(h->dp);
     for (int c = 0; c < 3; c++) {
       e->col_start[c][i] = (int)col_total[c];
       col_total[c] += ens_column_blocks(h->dp, ENS_CPW[c]);
     }
+  }
+  // surrogate groups: per-member block shares of the feature pass and the MLP
+  // in proportion to the member's staggered columns (at least one block each)
+  for (auto &g : e->groups) {
+    long long tot = 0;
+    for (int i : g.idx) tot += 2LL * members[i]->grid.Mx * members[i]->grid.My;
+    g.feat_waves = sg_features_waves(g.max_mz);
+    const int feat_total = 8192 / g.feat_waves;
+    for (int k = 0; k < 3; k++) g.start[k].assign(1, 0);
+    for (int i : g.idx) {
+      const double share = (double)(2LL * members[i]->grid.Mx * members[i]->grid.My) / (double)tot;
+      const int nb[3] = {sg_classify_blocks(members[i]->dp), std::max(1, (int)std::lround(feat_total * share)),
+                         std::max(1, (int)std::lround(sg_mlp_blocks() * share))};
+      for (int k = 0; k < 3; k++) g.start[k].push_back(g.start[k].back() + nb[k]);
+      e->mlp_part[i] = nb[2];
+    }
+  }
+  for (int i = 0; i < n; i++) {
     e->part_off[i] = part_total;
-    part_total += (size_t)ens_column_blocks(h->dp, 8);
+    part_total += (size_t)ens_column_blocks(members[i]->dp, 8) + (size_t)e->mlp_part[i];
   }
   if (pt_total >= 2147483647LL || col_total[0] >= 2147483647LL) {
     delete e;
@@ -1427,6 +1511,53 @@ int sia_ensemble_create(sia_handle *const *members, const sia_geometry *geometry
     M.dt = e->d_dt + i;
     M.err_acc = e->d_err_acc + i;
     M.t_end = t_end ? t_end[i] : 0.0;
+    if (h->sg) M.A.only = h->sg_status;  // the column stage takes the fallback columns only
+  }
+  for (auto &g : e->groups) {
+    for (int i : g.idx) {
+      sia_handle *h = members[i];
+      if (ensure_sg_buffers(h) != SIA_OK) {
+        e->last_error = h->last_error;
+        sia_ensemble_destroy(e);
+        return SIA_ERR_OUT_OF_MEMORY;
+      }
+      EnsSgMember S{};
+      S.P = h->dp;
+      S.C = SgColumnArgs{h->d_z, enthalpy[i], h->thks, h->theta, h->hx, h->hy, h->err};
+      S.L = sg_list_args(h);
+      S.F.list = h->sg_list;
+      S.F.n_list = h->sg_nlist;
+      S.F.F = h->sg_F;
+      S.F.CQb = h->sg_CQb;
+      S.F.CSt = h->sg_CSt;
+      S.F.status = h->sg_status;
+      S.F.stride = 1;
+      S.M.list = h->sg_list;
+      S.M.n_list = h->sg_nlist;
+      S.M.F = h->sg_F;
+      S.M.CQb = h->sg_CQb;
+      S.M.CSt = h->sg_CSt;
+      S.M.frag = g.sg->frag;
+      S.M.bias = g.sg->master;
+      S.M.norm = g.sg->norm;
+      S.M.hx = h->hx;
+      S.M.hy = h->hy;
+      S.M.status = h->sg_status;
+      S.M.flux = h->flux;
+      S.M.Dstag = h->Dstag;
+      S.M.counts = h->sg_counts;
+      g.table.push_back(S);
+    }
+    if ((err = hipMalloc((void **)&g.d_table, sizeof(EnsSgMember) * g.table.size())) != hipSuccess) {
+      return fail(err, "hipMalloc");
+    }
+    for (int k = 0; k < 3; k++) {
+      const size_t b = sizeof(int) * g.start[k].size();
+      if ((err = hipMalloc((void **)&g.d_start[k], b)) != hipSuccess ||
+          (err = hipMemcpy(g.d_start[k], g.start[k].data(), b, hipMemcpyHostToDevice)) != hipSuccess) {
+        return fail(err, "hipMalloc");
+      }
+    }
   }
   int rc = ens_upload(e);
   if (rc) {
@@ -1444,6 +1575,10 @@ int sia_ensemble_destroy(sia_ensemble *e) {
   void *bufs[] = {e->d_table, e->d_pt_start, e->d_col_start[0], e->d_col_start[1], e->d_col_start[2],
                   e->d_t, e->d_dt, e->d_err_acc, e->d_part};
   for (void *b : bufs) dfree(b);
+  for (auto &g : e->groups) {
+    dfree(g.d_table);
+    for (int k = 0; k < 3; k++) dfree(g.d_start[k]);
+  }
   if (e->ev_in) (void)hipEventDestroy(e->ev_in);
   if (e->ev_out) (void)hipEventDestroy(e->ev_out);
   if (e->own) (void)hipStreamDestroy(e->own);
@@ -1472,6 +1607,25 @@ int sia_ensemble_set_tile(sia_ensemble *e, int cols_per_wave) {
 int sia_ensemble_step(sia_ensemble *e, int n_steps, int use_graph) {
   if (!e || n_steps < 0) return SIA_ERR_ARGUMENT;
   if (n_steps == 0) return SIA_OK;
+  for (int i = 0; i < e->n; i++) {
+    if (e->members[i]->sg != e->bound[i]) {
+      return ens_err(e, SIA_ERR_STATE, "member " + std::to_string(i) +
+                                           ": its surrogate binding changed after sia_ensemble_create");
+    }
+  }
+  bool renorm = false;
+  for (auto &g : e->groups) {
+    if (g.version != g.sg->version) {
+      for (auto &S : g.table) S.M.norm = g.sg->norm;
+      g.version = g.sg->version;
+      renorm = true;
+    }
+  }
+  if (renorm) {
+    ENS_TRY(e, hipStreamSynchronize(e->own));  // earlier steps may still read the tables
+    const int rc = ens_upload(e);
+    if (rc) return rc;
+  }
   ENS_TRY(e, hipEventRecord(e->ev_in, e->user));
   ENS_TRY(e, hipStreamWaitEvent(e->own, e->ev_in, 0));
   const EnsLaunch L = ens_launch(e);
@@ -1479,7 +1633,7 @@ int sia_ensemble_step(sia_ensemble *e, int n_steps, int use_graph) {
     if (!e->exec || e->graph_steps != n_steps || e->graph_cpw != e->cpw) {
       ens_drop_graph(e);
       ENS_TRY(e, hipStreamBeginCapture(e->own, hipStreamCaptureModeThreadLocal));
-      for (int s = 0; s < n_steps; s++) launch_ens_step(L, e->own);
+      for (int s = 0; s < n_steps; s++) ens_issue_step(e, L, e->own);
       hipGraph_t g = nullptr;
       const hipError_t ce = hipStreamEndCapture(e->own, &g);
       if (ce != hipSuccess) return ens_err(e, SIA_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
@@ -1490,7 +1644,7 @@ int sia_ensemble_step(sia_ensemble *e, int n_steps, int use_graph) {
     }
     ENS_TRY(e, hipGraphLaunch(e->exec, e->own));
   } else {
-    for (int s = 0; s < n_steps; s++) launch_ens_step(L, e->own);
+    for (int s = 0; s < n_steps; s++) ens_issue_step(e, L, e->own);
     ENS_TRY(e, hipGetLastError());
   }
   ENS_TRY(e, hipEventRecord(e->ev_out, e->own));
@@ -1551,9 +1705,9 @@ int sia_ensemble_autotune(sia_ensemble *e, int steps, int *chosen, double *ms) {
       break;
     }
     const EnsLaunch L = ens_launch(e);
-    launch_ens_step(L, e->own);  // warm-up
+    ens_issue_step(e, L, e->own);  // warm-up
     err = hipEventRecord(a, e->own);
-    for (int s = 0; err == hipSuccess && s < steps; s++) launch_ens_step(L, e->own);
+    for (int s = 0; err == hipSuccess && s < steps; s++) ens_issue_step(e, L, e->own);
     if (err == hipSuccess) err = hipEventRecord(b, e->own);
     if (err == hipSuccess) err = hipEventSynchronize(b);
     float t = 0.0f;

@@ -1316,14 +1316,7 @@ void launch_mask_surface(const DevParams &P, const double *H_new, const double *
 // parameters and fields read from the device table (a wave-uniform address:
 // scalar loads). The arithmetic is the single-member kernels', so every member's
 // results are bitwise those of stepping it alone.
-__device__ __forceinline__ int ens_member(const int *__restrict__ start, int nm, int b) {
-  int lo = 0, hi = nm;  // start[lo] <= b < start[hi]
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (start[mid] <= b) lo = mid; else hi = mid;
-  }
-  return lo;
-}
+// (ens_member: sia_kernels.hpp)
 
 // per member: D_max accumulators, error flags, 2-D CFL bound
 __global__ void k_ens_reset(const EnsMember *__restrict__ T, int nm) {
@@ -1419,9 +1412,12 @@ static void ens_column_cpw(int law, const EnsMember *T, const int *start, int nm
   }
 }
 
-void launch_ens_step(const EnsLaunch &L, hipStream_t st) {
+void launch_ens_front(const EnsLaunch &L, hipStream_t st) {
   hipLaunchKernelGGL(k_ens_reset, dim3(L.nm), dim3(64), 0, st, L.table, L.nm);
   hipLaunchKernelGGL(k_ens_gradient, dim3(L.pt_blocks), dim3(256), 0, st, L.table, L.pt_start, L.nm);
+}
+
+void launch_ens_back(const EnsLaunch &L, hipStream_t st) {
   switch (L.cpw) {
   case 8: ens_column_cpw<8>(L.law, L.table, L.col_start, L.nm, L.col_blocks, L.max_mz, st); break;
   case 32: ens_column_cpw<32>(L.law, L.table, L.col_start, L.nm, L.col_blocks, L.max_mz, st); break;
@@ -1430,6 +1426,11 @@ void launch_ens_step(const EnsLaunch &L, hipStream_t st) {
   hipLaunchKernelGGL(k_ens_dt, dim3(L.nm), dim3(1024), 0, st, L.table, L.nm);
   hipLaunchKernelGGL(k_ens_mass, dim3(L.pt_blocks), dim3(256), 0, st, L.table, L.pt_start, L.nm);
   hipLaunchKernelGGL(k_ens_mask, dim3(L.pt_blocks), dim3(256), 0, st, L.table, L.pt_start, L.nm);
+}
+
+void launch_ens_step(const EnsLaunch &L, hipStream_t st) {
+  launch_ens_front(L, st);
+  launch_ens_back(L, st);
 }
 
 }  // namespace pism_sia

@@ -109,6 +109,16 @@ struct EnsMember {
   int n_part;
 };
 
+// member of block b of a stage grid: start[m] <= b < start[m + 1] (start: device [nm + 1])
+__device__ __forceinline__ int ens_member(const int *__restrict__ start, int nm, int b) {
+  int lo = 0, hi = nm;  // start[lo] <= b < start[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (start[mid] <= b) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
 struct EnsLaunch {
   const EnsMember *table;           // device [nm]
   const int *pt_start, *col_start;  // device [nm + 1]: first block of each member
@@ -118,5 +128,9 @@ struct EnsLaunch {
 int ens_point_blocks(const DevParams &P);
 int ens_column_blocks(const DevParams &P, int cpw);
 void launch_ens_step(const EnsLaunch &L, hipStream_t st);
+// the step in two halves: reset + gradient, then column + dt + mass + mask
+// (the surrogate stages of surrogate-bound members go in between)
+void launch_ens_front(const EnsLaunch &L, hipStream_t st);
+void launch_ens_back(const EnsLaunch &L, hipStream_t st);
 
 }  // namespace pism_sia

@@ -128,8 +128,9 @@ int sg_classify_blocks(const DevParams &P) { return (2 * P.Mx * P.My + SG_CLS_BL
 // status of staggered column s: SG_ZERO (thk == 0 or zero slope: D = 0,
 // SIAFD.cc:635-641), SG_FALLBACK (thk below the second level, or above Lz:
 // left to the FP64 kernel, which also reports the error), else SG_SURR.
-__global__ __launch_bounds__(SG_CLS_BLOCK) void k_sg_classify(DevParams P, SgColumnArgs C, SgListArgs L) {
-  const int s = blockIdx.x * SG_CLS_BLOCK + threadIdx.x;
+__device__ __forceinline__ void sg_classify_block(const DevParams &P, const SgColumnArgs &C, const SgListArgs &L,
+                                                  int bid) {
+  const int s = bid * SG_CLS_BLOCK + threadIdx.x;
   const int nstag = 2 * P.Mx * P.My;
   int st = -1;
   if (s < nstag) {
@@ -155,17 +156,19 @@ __global__ __launch_bounds__(SG_CLS_BLOCK) void k_sg_classify(DevParams P, SgCol
   const int ns = __syncthreads_count(st == SG_SURR);
   const int nf = __syncthreads_count(st == SG_FALLBACK);
   if (threadIdx.x == 0) {
-    L.block_count[3 * blockIdx.x + 0] = nz;
-    L.block_count[3 * blockIdx.x + 1] = ns;
-    L.block_count[3 * blockIdx.x + 2] = nf;
+    L.block_count[3 * bid + 0] = nz;
+    L.block_count[3 * bid + 1] = ns;
+    L.block_count[3 * bid + 2] = nf;
   }
+}
+
+__global__ __launch_bounds__(SG_CLS_BLOCK) void k_sg_classify(DevParams P, SgColumnArgs C, SgListArgs L) {
+  sg_classify_block(P, C, L, blockIdx.x);
 }
 
 // exclusive scan of the per-block surrogate counts (one block of 1024); the
 // offsets overwrite block_count[3 b + 1]; totals go to counts[0..2], n_list
-__global__ __launch_bounds__(1024) void k_sg_scan(int nb, SgListArgs L) {
-  __shared__ long long part[1024];
-  __shared__ long long tot[3];
+__device__ __forceinline__ void sg_scan_block(int nb, const SgListArgs &L, long long *part, long long *tot) {
   const int t = threadIdx.x;
   const int per = (nb + 1023) / 1024;
   const int b0 = t * per, b1 = min(nb, b0 + per);
@@ -204,10 +207,15 @@ __global__ __launch_bounds__(1024) void k_sg_scan(int nb, SgListArgs L) {
   }
 }
 
+__global__ __launch_bounds__(1024) void k_sg_scan(int nb, SgListArgs L) {
+  __shared__ long long part[1024];
+  __shared__ long long tot[3];
+  sg_scan_block(nb, L, part, tot);
+}
+
 // ordered compaction of the SG_SURR columns
-__global__ __launch_bounds__(SG_CLS_BLOCK) void k_sg_scatter(DevParams P, SgListArgs L) {
-  __shared__ int wsum[SG_CLS_BLOCK / 64];
-  const int s = blockIdx.x * SG_CLS_BLOCK + threadIdx.x;
+__device__ __forceinline__ void sg_scatter_block(const DevParams &P, const SgListArgs &L, int bid, int *wsum) {
+  const int s = bid * SG_CLS_BLOCK + threadIdx.x;
   const int nstag = 2 * P.Mx * P.My;
   const bool flag = (s < nstag) && L.status[s] == SG_SURR;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -215,9 +223,14 @@ __global__ __launch_bounds__(SG_CLS_BLOCK) void k_sg_scatter(DevParams P, SgList
   const int rank = __popcll(m & ((1ull << lane) - 1ull));
   if (lane == 0) wsum[wv] = __popcll(m);
   __syncthreads();
-  int base = L.block_count[3 * blockIdx.x + 1];
+  int base = L.block_count[3 * bid + 1];
   for (int w = 0; w < wv; w++) base += wsum[w];
   if (flag) L.list[base + rank] = s;
+}
+
+__global__ __launch_bounds__(SG_CLS_BLOCK) void k_sg_scatter(DevParams P, SgListArgs L) {
+  __shared__ int wsum[SG_CLS_BLOCK / 64];
+  sg_scatter_block(P, L, blockIdx.x, wsum);
 }
 
 void launch_sg_list(const DevParams &P, const SgColumnArgs &C, const SgListArgs &L, hipStream_t st) {
@@ -316,15 +329,16 @@ __device__ __forceinline__ void sg_cell_sums(const DevParams &P, const double2 *
   }
 }
 
+// Block bid of nblk persistent blocks walking the list. LDS: zsk (dynamic):
+// (z_k, (dz_k [k >= 1] + dz_{k+1}) / 2) per level, then per wavefront the two
+// staged profiles: FP64 enthalpy for the exact training targets, in inference
+// FP32 y_k = E_k / c_i + T0 - T_ref (x_k without the beta p_k term), then the
+// inference output rows; bucket: SG_NBUCKET ints.
 template <bool TARGETS, int NP>
-__global__ __launch_bounds__(256) void k_sg_features(DevParams P, SgColumnArgs C, SgFeatureArgs F) {
-  // LDS: (z_k, (dz_k [k >= 1] + dz_{k+1}) / 2) per level, then per wavefront
-  // the two staged profiles: FP64 enthalpy for the exact training targets, in
-  // inference FP32 y_k = E_k / c_i + T0 - T_ref (x_k without the beta p_k term)
+__device__ __forceinline__ void sg_features_block(const DevParams &P, const SgColumnArgs &C, const SgFeatureArgs &F,
+                                                  int bid, int nblk, double2 *zsk, int *bucket) {
   typedef typename std::conditional<TARGETS, double, float>::type EType;
-  extern __shared__ double2 zsk[];
   const int Mz = P.Mz;
-  __shared__ int bucket[SG_NBUCKET];
   for (int k = threadIdx.x; k < Mz; k += blockDim.x) {
     const double zk = C.z[k];
     zsk[k] = make_double2(zk, 0.5 * (((k >= 1) ? zk - C.z[k - 1] : 0.0) + ((k + 1 < Mz) ? C.z[k + 1] - zk : 0.0)));
@@ -362,7 +376,7 @@ __global__ __launch_bounds__(256) void k_sg_features(DevParams P, SgColumnArgs C
   const double ta = P.A_T0 * P.wf_coeff * P.ci * P.inv_L, tcap = P.wf_limit / (P.ci * P.inv_L);
   const double QR_cold = P.Q_cold / P.R, QR_warm = P.Q_warm / P.R;
   const int hh = lane >> 5, m = lane & (SG_NCELL - 1);
-  for (int t0 = (blockIdx.x * nwb + wv) * SG_FEAT_COLS; t0 < nsel; t0 += gridDim.x * nwb * SG_FEAT_COLS) {
+  for (int t0 = (bid * nwb + wv) * SG_FEAT_COLS; t0 < nsel; t0 += nblk * nwb * SG_FEAT_COLS) {
     // ---- setup: lane c < 16 prepares entry t0 + c (c_ks = -1: no entry)
     int c_ks = -1, c_q = 0, c_q2 = 0, c_s = 0;
     double c_thk = 0.0, c_lgs = 0.0, c_Cf = 0.0;
@@ -559,6 +573,13 @@ __global__ __launch_bounds__(256) void k_sg_features(DevParams P, SgColumnArgs C
   }
 }
 
+template <bool TARGETS, int NP>
+__global__ __launch_bounds__(256) void k_sg_features(DevParams P, SgColumnArgs C, SgFeatureArgs F) {
+  extern __shared__ double2 zsk[];
+  __shared__ int bucket[SG_NBUCKET];
+  sg_features_block<TARGETS, NP>(P, C, F, blockIdx.x, gridDim.x, zsk, bucket);
+}
+
 // dynamic LDS of k_sg_features: (z, half spacing) per level + per wavefront two
 // staged profiles (+ in inference the chunk's output rows)
 static size_t sg_features_lds(int Mz, int waves, bool targets) {
@@ -699,12 +720,12 @@ __device__ __forceinline__ int load_inputs(const SgMlpArgs &A, int slot, bool li
 // each wavefront evaluates 16-sample tiles.
 constexpr int SG_MLP_WAVES = 16;
 
-__global__ __launch_bounds__(64 * SG_MLP_WAVES) void k_sg_mlp(DevParams P, SgMlpArgs A) {
-  extern __shared__ unsigned short wfrag[];  // [SG_NFRAG][64][8], then the GELU table
-  // hidden biases as GELU-table offsets (b - lo) S, then the raw output biases
-  __shared__ __align__(16) float bias[4 * SG_WIDTH + SG_NOUT];
-  __shared__ double wmax[SG_MLP_WAVES];
-  __shared__ int wfall[SG_MLP_WAVES];
+// Block bid of nblk persistent blocks walking the list. LDS: wfrag (dynamic):
+// [SG_NFRAG][64][8] weights, then the GELU table; bias: the hidden biases as
+// GELU-table offsets (b - lo) S, then the raw output biases; wmax, wfall: one
+// entry per wavefront.
+__device__ __forceinline__ void sg_mlp_block(const DevParams &P, const SgMlpArgs &A, int bid, int nblk,
+                                             unsigned short *wfrag, float *bias, double *wmax, int *wfall) {
   float2 *gtab = reinterpret_cast<float2 *>(wfrag + SG_NFRAG * 512);
   {
     const uint4 *src = reinterpret_cast<const uint4 *>(A.frag);
@@ -730,7 +751,7 @@ __global__ __launch_bounds__(64 * SG_MLP_WAVES) void k_sg_mlp(DevParams P, SgMlp
   const int ntile = (n + 15) / 16;
   double dmax = 0.0;
   int nfall = 0;
-  for (int tile = blockIdx.x * SG_MLP_WAVES + wv; tile < ntile; tile += gridDim.x * SG_MLP_WAVES) {
+  for (int tile = bid * SG_MLP_WAVES + wv; tile < ntile; tile += nblk * SG_MLP_WAVES) {
     const int slot = tile * 16 + smp;
     const bool live = slot < n;
     bf16x8 x0, x1;
@@ -809,15 +830,93 @@ __global__ __launch_bounds__(64 * SG_MLP_WAVES) void k_sg_mlp(DevParams P, SgMlp
       mx = fmax(mx, wmax[w]);
       nf += wfall[w];
     }
-    A.dmax_part[blockIdx.x] = mx;
+    A.dmax_part[bid] = mx;
     if (nf) atomicAdd(&A.counts[2], nf);
     if (nf) atomicSub(&A.counts[1], nf);
   }
 }
 
+__global__ __launch_bounds__(64 * SG_MLP_WAVES) void k_sg_mlp(DevParams P, SgMlpArgs A) {
+  extern __shared__ unsigned short wfrag[];
+  __shared__ __align__(16) float bias[4 * SG_WIDTH + SG_NOUT];
+  __shared__ double wmax[SG_MLP_WAVES];
+  __shared__ int wfall[SG_MLP_WAVES];
+  sg_mlp_block(P, A, blockIdx.x, gridDim.x, wfrag, bias, wmax, wfall);
+}
+
+static size_t sg_mlp_lds() { return (size_t)SG_NFRAG * 512 * sizeof(unsigned short) + sizeof(float2) * SG_GTAB; }
+
 void launch_sg_mlp(const DevParams &P, const SgMlpArgs &A, hipStream_t st) {
-  const size_t lds = (size_t)SG_NFRAG * 512 * sizeof(unsigned short) + sizeof(float2) * SG_GTAB;
+  const size_t lds = sg_mlp_lds();
   hipLaunchKernelGGL(k_sg_mlp, dim3(SG_MLP_BLOCKS), dim3(64 * SG_MLP_WAVES), lds, st, P, A);
+}
+
+// ---------------------------------------------------------------- ensembles
+// Stage grids of all members bound to one surrogate: block b runs block
+// b - start[m] of member m's single-member kernel body (the member's table
+// entry is at a wave-uniform address: scalar loads). The feature pass and the
+// MLP give each member a persistent share of blocks (start[m + 1] - start[m])
+// that walks its own list, so each member's rows are those of its own
+// single-member update, whatever the share.
+__global__ __launch_bounds__(SG_CLS_BLOCK) void k_ens_sg_classify(const EnsSgMember *__restrict__ T,
+                                                                   const int *__restrict__ start, int nm) {
+  const int m = ens_member(start, nm, blockIdx.x);
+  sg_classify_block(T[m].P, T[m].C, T[m].L, blockIdx.x - start[m]);
+}
+
+// one block per member
+__global__ __launch_bounds__(1024) void k_ens_sg_scan(const EnsSgMember *__restrict__ T,
+                                                      const int *__restrict__ start, int nm) {
+  __shared__ long long part[1024];
+  __shared__ long long tot[3];
+  const int m = blockIdx.x;
+  if (m >= nm) return;
+  sg_scan_block(start[m + 1] - start[m], T[m].L, part, tot);
+}
+
+__global__ __launch_bounds__(SG_CLS_BLOCK) void k_ens_sg_scatter(const EnsSgMember *__restrict__ T,
+                                                                  const int *__restrict__ start, int nm) {
+  __shared__ int wsum[SG_CLS_BLOCK / 64];
+  const int m = ens_member(start, nm, blockIdx.x);
+  sg_scatter_block(T[m].P, T[m].L, blockIdx.x - start[m], wsum);
+}
+
+template <int NP>
+__global__ __launch_bounds__(256) void k_ens_sg_features(const EnsSgMember *__restrict__ T,
+                                                         const int *__restrict__ start, int nm) {
+  extern __shared__ double2 zsk[];
+  __shared__ int bucket[SG_NBUCKET];
+  const int m = ens_member(start, nm, blockIdx.x);
+  sg_features_block<false, NP>(T[m].P, T[m].C, T[m].F, blockIdx.x - start[m], start[m + 1] - start[m], zsk, bucket);
+}
+
+__global__ __launch_bounds__(64 * SG_MLP_WAVES) void k_ens_sg_mlp(const EnsSgMember *__restrict__ T,
+                                                                  const int *__restrict__ start, int nm) {
+  extern __shared__ unsigned short wfrag[];
+  __shared__ __align__(16) float bias[4 * SG_WIDTH + SG_NOUT];
+  __shared__ double wmax[SG_MLP_WAVES];
+  __shared__ int wfall[SG_MLP_WAVES];
+  const int m = ens_member(start, nm, blockIdx.x);
+  sg_mlp_block(T[m].P, T[m].M, blockIdx.x - start[m], start[m + 1] - start[m], wfrag, bias, wmax, wfall);
+}
+
+int sg_features_waves(int Mz) { return sg_features_lds(Mz, 4, false) <= 64 * 1024 ? 4 : 1; }
+
+void launch_ens_sg(const EnsSgLaunch &L, hipStream_t st) {
+  hipLaunchKernelGGL(k_ens_sg_classify, dim3(L.cls_blocks), dim3(SG_CLS_BLOCK), 0, st, L.table, L.cls_start, L.nm);
+  hipLaunchKernelGGL(k_ens_sg_scan, dim3(L.nm), dim3(1024), 0, st, L.table, L.cls_start, L.nm);
+  hipLaunchKernelGGL(k_ens_sg_scatter, dim3(L.cls_blocks), dim3(SG_CLS_BLOCK), 0, st, L.table, L.cls_start, L.nm);
+  const size_t lds = sg_features_lds(L.max_mz, L.feat_waves, false);
+  const dim3 fg(L.feat_blocks), fb(64 * L.feat_waves);
+  if (L.nm1 == 1) {
+    hipLaunchKernelGGL(k_ens_sg_features<2>, fg, fb, lds, st, L.table, L.feat_start, L.nm);
+  } else if (L.nm1 == 2) {
+    hipLaunchKernelGGL(k_ens_sg_features<3>, fg, fb, lds, st, L.table, L.feat_start, L.nm);
+  } else {
+    hipLaunchKernelGGL(k_ens_sg_features<4>, fg, fb, lds, st, L.table, L.feat_start, L.nm);
+  }
+  hipLaunchKernelGGL(k_ens_sg_mlp, dim3(L.mlp_blocks), dim3(64 * SG_MLP_WAVES), sg_mlp_lds(), st, L.table,
+                     L.mlp_start, L.nm);
 }
 
 // ---------------------------------------------------------------- training

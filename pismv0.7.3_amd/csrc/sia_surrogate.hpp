@@ -119,6 +119,27 @@ int sg_mlp_blocks();
 void launch_sg_mlp(const DevParams &P, const SgMlpArgs &A, hipStream_t st);
 void launch_sg_pack(const float *master, unsigned short *frag, hipStream_t st);
 
+// ---- ensembles: the surrogate stages of every member bound to one surrogate,
+// one launch per stage. Each member keeps its own list, feature rows and
+// status; blocks map to members through block-prefix tables as in the plain
+// ensemble stages, and run the single-member kernels' bodies.
+struct EnsSgMember {
+  DevParams P;
+  SgColumnArgs C;
+  SgListArgs L;
+  SgFeatureArgs F;
+  SgMlpArgs M;  // M.dmax_part: the member's partials after its column-stage ones
+};
+struct EnsSgLaunch {
+  const EnsSgMember *table;                        // device [nm]
+  const int *cls_start, *feat_start, *mlp_start;   // device [nm + 1]
+  int nm, cls_blocks, feat_blocks, mlp_blocks;
+  int feat_waves, nm1, max_mz;                     // wavefronts per feature block, n - 1, largest Mz
+};
+// wavefronts per block of the inference feature pass for columns of Mz levels
+int sg_features_waves(int Mz);
+void launch_ens_sg(const EnsSgLaunch &L, hipStream_t st);
+
 // training
 struct SgTrainBuffers {
   int B;                                   // batch
