@@ -301,11 +301,12 @@ enum sia_event {
   SIA_EV_GRADIENT = 1,       /* "SIA gradient" (+ theta, thksmooth) */
   SIA_EV_FLUX = 2,           /* "SIA flux" */
   SIA_EV_3D_HOR_VEL = 3,     /* "SIA 3D hor. vel." */
-  SIA_EV_VERT_VEL = 4,       /* "SB vert. vel." */
+  SIA_EV_VERT_VEL = 4,       /* "SB vert. vel.": computed in the "SIA 3D hor. vel." kernel, not recorded apart */
   SIA_EV_MASS_TRANSPORT = 5, /* "mass transport" (+ time step, surface and mask) */
   SIA_EV_SG_FEATURES = 6,    /* "surrogate features": the learned surrogate's per-level pass (inside "SIA flux") */
   SIA_EV_SG_MLP = 7,         /* "surrogate MLP": its fused network (inside "SIA flux") */
-  SIA_EV_STRAIN_HEAT = 8,    /* "SB strain heat" (PISMStressBalance.cc:102-104) */
+  SIA_EV_STRAIN_HEAT = 8,    /* "SB strain heat" (PISMStressBalance.cc:102-104): computed in the
+                                "SIA 3D hor. vel." kernel, not recorded apart */
   SIA_EV_COUNT = 9
 };
 int sia_profiling(sia_handle *h, int on);

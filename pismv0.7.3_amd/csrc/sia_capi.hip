@@ -347,7 +347,7 @@ int update_impl(sia_handle *h, const sia_inputs *in, int fast, bool allow_sync) 
     }
   }
   if (full) {
-    // "SIA 3D hor. vel." and "SB vert. vel." run as one fused kernel
+    // "SIA 3D hor. vel.", "SB vert. vel." and the strain heating run as one fused kernel
     Stage stage(h, SIA_EV_3D_HOR_VEL);
     VelArgs V{};
     V.z = h->d_z;
@@ -363,22 +363,13 @@ int update_impl(sia_handle *h, const sia_inputs *in, int fast, bool allow_sync) 
     V.u = h->u;
     V.v = h->v;
     V.w = h->w;
-    launch_velocity_uvw(P, V, st);
-  }
-  if (full) {
     // StressBalance::update computes the strain heating on every full update
-    // (PISMStressBalance.cc:102-105)
-    Stage stage(h, SIA_EV_STRAIN_HEAT);
-    HeatArgs S{};
-    S.z = h->d_z;
-    S.H = in->land_ice_thickness;
-    S.mask = in->mask;
-    S.E = in->enthalpy;
-    S.u = h->u;
-    S.v = h->v;
-    S.Sigma = h->Sigma;
-    S.err = h->err;
-    launch_strain_heating(P, S, st);
+    // (PISMStressBalance.cc:102-105); it runs in the same pass
+    V.H = in->land_ice_thickness;
+    V.E = in->enthalpy;
+    V.Sigma = h->Sigma;
+    V.err = h->err;
+    launch_velocity_uvw(P, V, st);
   }
   if (in->vel_input_u && in->vel_input_v) {
     launch_cfl2d(P, in->mask, in->vel_input_u, in->vel_input_v, h->cfl_bits, st);

@@ -11,8 +11,9 @@
 //                                    integral I (SIAFD.cc:546-722, 819-879); one
 //                                    64-lane wavefront per staggered column, one lane
 //                                    per level, wave-tree reductions and scans
-//   k_velocity_uvw                   u, v (SIAFD.cc:918-958) and w
-//                                    (PISMStressBalance.cc:209-286) in one pass
+//   k_velocity_uvw<LAW>              u, v (SIAFD.cc:918-958), w (PISMStressBalance.cc:209-286)
+//                                    and (LAW >= 0) the volumetric strain heating
+//                                    (PISMStressBalance.cc:370-500) in one pass
 //   k_mass_continuity, k_mask_surface, k_dt, accounting reductions
 //                                    iMgeometry.cc:56-149, 497-822; iMadaptive.cc:118-171
 #include "sia_kernels.hpp"
@@ -32,6 +33,8 @@ __device__ __forceinline__ double dpp_f64(double v) {
 
 // value of lane l-1, 0 in lane 0 (wave_shr:1)
 __device__ __forceinline__ double wave_shift_up1(double v) { return dpp_f64<0x138, 0xf, true>(v); }
+// value of lane l+1, 0 in lane 63 (wave_shl:1)
+__device__ __forceinline__ double wave_shift_down1(double v) { return dpp_f64<0x130, 0xf, true>(v); }
 
 // inclusive prefix sum over the 64 lanes: Hillis-Steele with row_shr 1, 2,
 // 4, 8 inside each row of 16 lanes, then row_bcast:15 adds the total of row
@@ -632,12 +635,37 @@ __device__ __forceinline__ double uv_formula(double Ie, double he, double Iw, do
   return -0.25 * (Ie * he + Iw * hw + In * hn + Is * hs) + basal;
 }
 
+__device__ __forceinline__ double strain_D2(double u_x, double u_y, double u_z, double v_x, double v_y,
+                                            double v_z) {  // PISMStressBalance.cc:309-311
+  return 0.5 * ((u_x + v_y) * (u_x + v_y) + u_x * u_x + v_y * v_y +
+                0.5 * ((u_y + v_x) * (u_y + v_x) + u_z * u_z + v_z * v_z));
+}
+
+// LAW >= 0: the volumetric strain heating of the same column in the same pass
+// (StressBalance::compute_volumetric_strain_heating, PISMStressBalance.cc:370-500).
+// It needs u, v of the column and of its four neighbours at each level; u at
+// (i-1,j), (i+1,j) and v at (i,j-1), (i,j+1) are the w-divergence terms, and
+// u at (i,j-+1), v at (i-+1,j) come from the same 16 I columns with the other
+// gradient component, evaluated exactly as the neighbour's own wavefront
+// evaluates its u, v (same operands, same order), so they equal the stored
+// fields. u_z, v_z take the levels k-+1 from the neighbouring lanes; lane 63's
+// level k+1 is one more (wave-uniform) evaluation at k0 + 64, lane 0's k-1 is
+// carried from the previous chunk. Sigma = 2 e^(-1/n) B D^(1+1/n) is evaluated
+// as 2 e^(-1/n) exp(-ln A / n + (1/n + 1)/2 ln D^2), 0 above kBelowHeight(H).
+template <int LAW>
 __global__ __launch_bounds__(256) void k_velocity_uvw(DevParams P, VelArgs A, StripMap sm) {
-  extern __shared__ double zs[];
+  constexpr bool HEAT = LAW >= 0;
+  extern __shared__ double zs[];  // z [Mz], then (HEAT) 1 / (z_{k+1} - z_{k-1}) [Mz] (k = 0: 1 / (z_1 - z_0))
   __shared__ VelSlot tbl[WAVES_PER_BLOCK][16];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int Mx = P.Mx, My = P.My, Mz = P.Mz;
-  for (int k = threadIdx.x; k < Mz; k += blockDim.x) zs[k] = A.z[k];
+  double *idz = zs + Mz;
+  for (int k = threadIdx.x; k < Mz; k += blockDim.x) {
+    zs[k] = A.z[k];
+    if (HEAT) {
+      idz[k] = (k == 0) ? 1.0 / (A.z[1] - A.z[0]) : ((k + 1 < Mz) ? 1.0 / (A.z[k + 1] - A.z[k - 1]) : 0.0);
+    }
+  }
   __syncthreads();
   // block b runs on XCD b % 8 as the (b / 8)-th block of that XCD
   const int x = blockIdx.x % N_XCD;
@@ -683,6 +711,29 @@ __global__ __launch_bounds__(256) void k_velocity_uvw(DevParams P, VelArgs A, St
   if ((m_icy(m) && m_ice_free(ms)) || (m_ice_free(m) && m_icy(ms))) south = 0;
   const double D_y = (north + south > 0) ? 1.0 / (P.dy * (north + south)) : 0.0;
   const double w0 = A.bmr ? -A.bmr[q] : 0.0;
+  // strain heating: kBelowHeight(H) (throws above Lz, IceGrid.cc:381-389),
+  // the other neighbours' basal velocities, and the levels the formulas run
+  // to (ksH + 1 for u_z at ksH; above kmax every I column is clamped, so the
+  // formulas are constant there and equal the stored values)
+  int ksH = -1;
+  double Hij = 0.0, ub_s = 0.0, ub_n = 0.0, vb_w = 0.0, vb_e = 0.0;
+  if (HEAT) {
+    Hij = A.H[q];
+    ksH = kbelow(zs, Mz, P.Lz, Hij);
+    if (ksH < 0 && lane == 0) atomicOr(A.err, ERR_THK_LZ);
+    if (A.ub) {
+      ub_s = A.ub[i * My + jm];
+      ub_n = A.ub[i * My + jp];
+    }
+    if (A.vb) {
+      vb_w = A.vb[im * My + j];
+      vb_e = A.vb[ip * My + j];
+    }
+  }
+  const int kev = HEAT ? max(kmax, min(ksH + 1, Mz - 1)) : kmax;
+  const double *__restrict__ E_ij = HEAT ? A.E + (long)q * Mz : nullptr;
+  double *__restrict__ So = HEAT ? A.Sigma + (long)q * Mz : nullptr;
+  double carry_u = 0.0, carry_v = 0.0;  // u, v at the previous chunk's last level
   double *__restrict__ uo = A.u + (long)q * Mz;
   double *__restrict__ vo = A.v + (long)q * Mz;
   double *__restrict__ wo = A.w + (long)q * Mz;
@@ -692,8 +743,8 @@ __global__ __launch_bounds__(256) void k_velocity_uvw(DevParams P, VelArgs A, St
   for (int k0 = 0; k0 < Mz; k0 += 64) {
     const int k = k0 + lane;
     const bool act = k < Mz;
-    double uc, vc, integrand;
-    if (k0 <= kmax) {
+    double uc, vc, integrand, sig = 0.0;
+    if (k0 <= kev) {
       const int kk = act ? k : Mz - 1;
       double Iv[16];
       if (kmax == 0) {
@@ -702,7 +753,7 @@ __global__ __launch_bounds__(256) void k_velocity_uvw(DevParams P, VelArgs A, St
         for (int t = 0; t < 16; t++) Iv[t] = 0.0;
       } else {
 #pragma unroll
-        for (int t = 0; t < 16; t++) Iv[t] = T[t].col[min(kk, T[t].ks)];
+        for (int t = 0; t < 16; t++) Iv[t] = glb(T[t].col)[min(kk, T[t].ks)];  // device memory: global loads
       }
       uc = uv_formula(Iv[0], T[0].hx, Iv[1], T[1].hx, Iv[2], T[2].hx, Iv[3], T[3].hx, ub_c);
       vc = uv_formula(Iv[0], T[0].hy, Iv[1], T[1].hy, Iv[2], T[2].hy, Iv[3], T[3].hy, vb_c);
@@ -714,10 +765,54 @@ __global__ __launch_bounds__(256) void k_velocity_uvw(DevParams P, VelArgs A, St
       const double u_x = D_x * (west * (uc - uw) + east * (ue - uc));
       const double v_y = D_y * (south * (vc - vs) + north * (vn - vc));
       integrand = u_x + v_y;
-      if (kmax < k0 + 64) {
-        uc_t = lane_value(uc, kmax - k0);
-        vc_t = lane_value(vc, kmax - k0);
-        int_t = lane_value(integrand, kmax - k0);
+      if (HEAT && k0 <= ksH) {
+        // u, v at level k0 + 64 for lane 63's k + 1 (the next chunk's lane 0)
+        double un_c = 0.0, vn_c = 0.0;
+        if (k0 + 63 <= ksH) {
+          const int kn = min(k0 + 64, Mz - 1);
+          double J[4];
+#pragma unroll
+          for (int t = 0; t < 4; t++) J[t] = (kmax == 0) ? 0.0 : glb(T[t].col)[min(kn, T[t].ks)];
+          un_c = uv_formula(J[0], T[0].hx, J[1], T[1].hx, J[2], T[2].hx, J[3], T[3].hx, ub_c);
+          vn_c = uv_formula(J[0], T[0].hy, J[1], T[1].hy, J[2], T[2].hy, J[3], T[3].hy, vb_c);
+        }
+        double u_up = wave_shift_down1(uc), v_up = wave_shift_down1(vc);
+        double u_dn = wave_shift_up1(uc), v_dn = wave_shift_up1(vc);
+        if (lane == 63) {
+          u_up = un_c;
+          v_up = vn_c;
+        }
+        if (lane == 0) {
+          u_dn = carry_u;
+          v_dn = carry_v;
+        }
+        if (k <= ksH) {
+          // the neighbours' u, v as their own wavefronts evaluate them
+          const double us = uv_formula(Iv[10], T[10].hx, Iv[11], T[11].hx, Iv[3], T[3].hx, Iv[12], T[12].hx, ub_s);
+          const double un = uv_formula(Iv[13], T[13].hx, Iv[14], T[14].hx, Iv[15], T[15].hx, Iv[2], T[2].hx, ub_n);
+          const double vw = uv_formula(Iv[1], T[1].hy, Iv[4], T[4].hy, Iv[5], T[5].hy, Iv[6], T[6].hy, vb_w);
+          const double ve = uv_formula(Iv[7], T[7].hy, Iv[0], T[0].hy, Iv[8], T[8].hy, Iv[9], T[9].hy, vb_e);
+          const double pressure = ec_pressure(P, Hij - zs[k]);
+          const double lnA = log_softness<LAW>(P, E_ij[k], pressure);
+          const double hu_x = D_x * (west * (uc - uw) + east * (ue - uc)),
+                       hu_y = D_y * (south * (uc - us) + north * (un - uc)),
+                       hv_x = D_x * (west * (vc - vw) + east * (ve - vc)),
+                       hv_y = D_y * (south * (vc - vs) + north * (vn - vc));
+          // one-sided at the base (PISMStressBalance.cc:464-476)
+          const double u_z = (k > 0) ? (u_up - u_dn) * idz[k] : (u_up - uc) * idz[0];
+          const double v_z = (k > 0) ? (v_up - v_dn) * idz[k] : (v_up - vc) * idz[0];
+          const double d2 = strain_D2(hu_x, hu_y, u_z, hv_x, hv_y, v_z);
+          sig = (d2 > 0.0) ? 2.0 * P.e_pow * exp(P.hard_pow * lnA + P.heat_exp * log(d2)) : 0.0;
+        }
+      }
+      if (HEAT) {
+        carry_u = lane_value(uc, 63);
+        carry_v = lane_value(vc, 63);
+      }
+      if (kev < k0 + 64) {
+        uc_t = lane_value(uc, kev - k0);
+        vc_t = lane_value(vc, kev - k0);
+        int_t = lane_value(integrand, kev - k0);
       }
     } else {
       uc = uc_t;
@@ -726,8 +821,11 @@ __global__ __launch_bounds__(256) void k_velocity_uvw(DevParams P, VelArgs A, St
     }
     double dz = 0.0;
     if (act) {
-      uo[k] = uc;
-      vo[k] = vc;
+      // u, v, w stream out once per update: non-temporal stores keep them
+      // from displacing the I columns the neighbouring wavefronts read from L2
+      __builtin_nontemporal_store(uc, uo + k);
+      __builtin_nontemporal_store(vc, vo + k);
+      if (HEAT) __builtin_nontemporal_store(sig, So + k);
       if (k > 0) dz = zs[k] - zs[k - 1];
     } else {
       integrand = 0.0;
@@ -736,7 +834,7 @@ __global__ __launch_bounds__(256) void k_velocity_uvw(DevParams P, VelArgs A, St
     if (lane == 0) iprev = carry_int;
     const double t = (act && k > 0) ? 0.5 * (integrand + iprev) * dz : 0.0;  // PISMStressBalance.cc:280-282
     const double S = carry_sum + wave_inclusive_scan(t);
-    if (act) wo[k] = w0 - S;
+    if (act) __builtin_nontemporal_store(w0 - S, wo + k);
     carry_sum = lane_value(S, 63);
     carry_int = lane_value(integrand, 63);
   }
@@ -897,77 +995,6 @@ __global__ void k_sia_sliding(DevParams P, SlideArgs A) {
   A.vb[q] = v;
   const double Pov = P.rho_i * P.g * H;
   A.heat[q] = -(Pov * myhx) * u - (Pov * myhy) * v;
-}
-
-// ---------------------------------------------------------------- strain heating
-// StressBalance::compute_volumetric_strain_heating (PISMStressBalance.cc:370-500)
-// from the stored u, v. One wavefront per map column in the velocity kernel's
-// XCD-interleaved strips (rows i-1..i+1 of u and v stay in the XCD's L2), one
-// lane per level; Sigma = 0 above kBelowHeight(H).
-__device__ __forceinline__ double strain_D2(double u_x, double u_y, double u_z, double v_x, double v_y,
-                                            double v_z) {  // PISMStressBalance.cc:309-311
-  return 0.5 * ((u_x + v_y) * (u_x + v_y) + u_x * u_x + v_y * v_y +
-                0.5 * ((u_y + v_x) * (u_y + v_x) + u_z * u_z + v_z * v_z));
-}
-
-// Sigma = 2 e^(-1/n) B D^(1+1/n) is evaluated as 2 e^(-1/n) exp(-ln A / n +
-// (1/n + 1)/2 ln D^2): one exp and one log a level instead of the reference's
-// exp and two pow; 1 / (z_{k+1} - z_{k-1}) is staged with the levels.
-template <int LAW>
-__global__ __launch_bounds__(256) void k_strain_heating(DevParams P, HeatArgs A, StripMap sm) {
-  extern __shared__ double zs[];  // z [Mz], then 1 / (z_{k+1} - z_{k-1}) [Mz] (k = 0: 1 / (z_1 - z_0))
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int Mx = P.Mx, My = P.My, Mz = P.Mz;
-  double *idz = zs + Mz;
-  for (int k = threadIdx.x; k < Mz; k += blockDim.x) {
-    zs[k] = A.z[k];
-    idz[k] = (k == 0) ? 1.0 / (A.z[1] - A.z[0]) : ((k + 1 < Mz) ? 1.0 / (A.z[k + 1] - A.z[k - 1]) : 0.0);
-  }
-  __syncthreads();
-  const int x = blockIdx.x % N_XCD;
-  const int Lp = __builtin_amdgcn_readfirstlane((blockIdx.x / N_XCD) * WAVES_PER_BLOCK + wv);
-  int i, j;
-  if (!strip_column(sm, x, Lp, i, j)) return;
-  const long q = (long)i * My + j;
-  const double Hij = A.H[q];
-  int ks = kbelow(zs, Mz, P.Lz, Hij);
-  if (ks < 0 && lane == 0) atomicOr(A.err, ERR_THK_LZ);  // kBelowHeight throws (IceGrid.cc:381-389)
-  const int im = wrapi(i - 1, Mx), ip = wrapi(i + 1, Mx), jm = wrapi(j - 1, My), jp = wrapi(j + 1, My);
-  const long qw = (long)im * My + j, qe = (long)ip * My + j, qs = (long)i * My + jm, qn = (long)i * My + jp;
-  // one-sided differences at ice margins (PISMStressBalance.cc:411-437)
-  const int m = as_int(A.mask[q]), me = as_int(A.mask[qe]), mw = as_int(A.mask[qw]), mn = as_int(A.mask[qn]),
-            ms = as_int(A.mask[qs]);
-  double west = 1, east = 1, south = 1, north = 1;
-  if ((m_icy(m) && m_ice_free(me)) || (m_ice_free(m) && m_icy(me))) east = 0;
-  if ((m_icy(m) && m_ice_free(mw)) || (m_ice_free(m) && m_icy(mw))) west = 0;
-  const double D_x = (east + west > 0) ? 1.0 / (P.dx * (east + west)) : 0.0;
-  if ((m_icy(m) && m_ice_free(mn)) || (m_ice_free(m) && m_icy(mn))) north = 0;
-  if ((m_icy(m) && m_ice_free(ms)) || (m_ice_free(m) && m_icy(ms))) south = 0;
-  const double D_y = (north + south > 0) ? 1.0 / (P.dy * (north + south)) : 0.0;
-  const double *u_ij = A.u + q * Mz, *u_w = A.u + qw * Mz, *u_e = A.u + qe * Mz, *u_s = A.u + qs * Mz,
-               *u_n = A.u + qn * Mz;
-  const double *v_ij = A.v + q * Mz, *v_w = A.v + qw * Mz, *v_e = A.v + qe * Mz, *v_s = A.v + qs * Mz,
-               *v_n = A.v + qn * Mz;
-  const double *E_ij = A.E + q * Mz;
-  double *Sg = A.Sigma + q * Mz;
-  for (int k0 = 0; k0 < Mz; k0 += 64) {
-    const int k = k0 + lane;
-    double sig = 0.0;
-    if (k <= ks) {
-      const double pressure = ec_pressure(P, Hij - zs[k]);
-      const double lnA = log_softness<LAW>(P, E_ij[k], pressure);
-      const double u_x = D_x * (west * (u_ij[k] - u_w[k]) + east * (u_e[k] - u_ij[k])),
-                   u_y = D_y * (south * (u_ij[k] - u_s[k]) + north * (u_n[k] - u_ij[k])),
-                   v_x = D_x * (west * (v_ij[k] - v_w[k]) + east * (v_e[k] - v_ij[k])),
-                   v_y = D_y * (south * (v_ij[k] - v_s[k]) + north * (v_n[k] - v_ij[k]));
-      // ks <= Mz - 2, so level k + 1 exists; one-sided at the base
-      const int kl = (k > 0) ? k - 1 : 0, ku = (k > 0) ? k + 1 : 1;
-      const double u_z = (u_ij[ku] - u_ij[kl]) * idz[k], v_z = (v_ij[ku] - v_ij[kl]) * idz[k];
-      const double d2 = strain_D2(u_x, u_y, u_z, v_x, v_y, v_z);
-      sig = (d2 > 0.0) ? 2.0 * P.e_pow * exp(P.hard_pow * lnA + P.heat_exp * log(d2)) : 0.0;
-    }
-    if (k < Mz) Sg[k] = sig;
-  }
 }
 
 // ---------------------------------------------------------------- time step
@@ -1239,28 +1266,26 @@ void launch_theta_thk(const DevParams &P, const double *H, const double *b, cons
                      C4, theta, thks, err);
 }
 
-void launch_velocity_uvw(const DevParams &P, const VelArgs &A, hipStream_t st) {
-  const StripMap sm = strip_map(P.Mx, P.My, WAVES_PER_BLOCK);
-  hipLaunchKernelGGL(k_velocity_uvw, dim3(sm.per_xcd * N_XCD), dim3(256), sizeof(double) * (size_t)P.Mz, st, P,
-                     A, sm);
-}
-
 template <int LAW>
-static void launch_heat_law(const DevParams &P, const HeatArgs &A, hipStream_t st) {
+static void launch_vel_law(const DevParams &P, const VelArgs &A, hipStream_t st) {
   const StripMap sm = strip_map(P.Mx, P.My, WAVES_PER_BLOCK);
-  hipLaunchKernelGGL((k_strain_heating<LAW>), dim3(sm.per_xcd * N_XCD), dim3(256),
-                     2 * sizeof(double) * (size_t)P.Mz, st, P, A, sm);
+  hipLaunchKernelGGL((k_velocity_uvw<LAW>), dim3(sm.per_xcd * N_XCD), dim3(256),
+                     (LAW >= 0 ? 2 : 1) * sizeof(double) * (size_t)P.Mz, st, P, A, sm);
 }
 
-void launch_strain_heating(const DevParams &P, const HeatArgs &A, hipStream_t st) {
+void launch_velocity_uvw(const DevParams &P, const VelArgs &A, hipStream_t st) {
+  if (!A.Sigma) {
+    launch_vel_law<-1>(P, A, st);
+    return;
+  }
   switch (P.law) {
-  case LAW_ISOGLEN: launch_heat_law<LAW_ISOGLEN>(P, A, st); break;
-  case LAW_PB: launch_heat_law<LAW_PB>(P, A, st); break;
-  case LAW_GPBLD: launch_heat_law<LAW_GPBLD>(P, A, st); break;
-  case LAW_HOOKE: launch_heat_law<LAW_HOOKE>(P, A, st); break;
-  case LAW_ARR: launch_heat_law<LAW_ARR>(P, A, st); break;
-  case LAW_ARRWARM: launch_heat_law<LAW_ARRWARM>(P, A, st); break;
-  default: launch_heat_law<LAW_GK>(P, A, st); break;
+  case LAW_ISOGLEN: launch_vel_law<LAW_ISOGLEN>(P, A, st); break;
+  case LAW_PB: launch_vel_law<LAW_PB>(P, A, st); break;
+  case LAW_GPBLD: launch_vel_law<LAW_GPBLD>(P, A, st); break;
+  case LAW_HOOKE: launch_vel_law<LAW_HOOKE>(P, A, st); break;
+  case LAW_ARR: launch_vel_law<LAW_ARR>(P, A, st); break;
+  case LAW_ARRWARM: launch_vel_law<LAW_ARRWARM>(P, A, st); break;
+  default: launch_vel_law<LAW_GK>(P, A, st); break;
   }
 }
 
@@ -1336,9 +1361,10 @@ __global__ __launch_bounds__(256) void k_ens_gradient(const EnsMember *__restric
   const EnsMember &E = T[m];
   const long q = (long)(blockIdx.x - start[m]) * 256 + threadIdx.x;
   if (q >= (long)E.P.Mx * E.P.My) return;
-  gradient_point(E.P, E.F, E.hx, E.hy, q);
-  theta_thk_point(E.P, E.F, E.topgs, E.maxtl, E.C2, E.C3, E.C4, E.theta, E.thks,
-                  E.A.err, q);
+  const Fields2D F = global_view(E.F);
+  gradient_point(E.P, F, glb(E.hx), glb(E.hy), q);
+  theta_thk_point(E.P, F, glb(E.topgs), glb(E.maxtl), glb(E.C2), glb(E.C3), glb(E.C4), glb(E.theta),
+                  glb(E.thks), glb(E.A.err), q);
 }
 
 template <int LAW, bool FULL, int CPW>
@@ -1347,7 +1373,8 @@ __global__ __launch_bounds__(256) void k_ens_column(const EnsMember *__restrict_
   extern __shared__ double zs[];
   __shared__ double wmax[WAVES_PER_BLOCK];
   const int m = ens_member(start, nm, blockIdx.x);
-  column_block<LAW, FULL, CPW>(T[m].P, T[m].A, blockIdx.x - start[m], zs, wmax);
+  const ColumnArgs A = global_view(T[m].A);
+  column_block<LAW, FULL, CPW>(T[m].P, A, blockIdx.x - start[m], zs, wmax);
 }
 
 // per member (one block each): D_max from the column partials, then dt and t += dt
@@ -1357,7 +1384,8 @@ __global__ __launch_bounds__(1024) void k_ens_dt(const EnsMember *__restrict__ T
   if (mb >= nm) return;
   const EnsMember &E = T[mb];
   double m = 0.0;
-  for (int b = threadIdx.x; b < E.n_part; b += blockDim.x) m = (m < E.A.dmax_part[b]) ? E.A.dmax_part[b] : m;
+  const double *part = glb(E.A.dmax_part);
+  for (int b = threadIdx.x; b < E.n_part; b += blockDim.x) m = (m < part[b]) ? part[b] : m;
   red[threadIdx.x] = m;
   __syncthreads();
   for (int off = blockDim.x >> 1; off > 0; off >>= 1) {
@@ -1374,7 +1402,8 @@ __global__ __launch_bounds__(256) void k_ens_mass(const EnsMember *__restrict__ 
                                                   int nm) {
   __shared__ double red[8][256];
   const int m = ens_member(start, nm, blockIdx.x);
-  mass_block(T[m].P, T[m].M, blockIdx.x - start[m], red);
+  const MassArgs M = global_view(T[m].M);
+  mass_block(T[m].P, M, blockIdx.x - start[m], red);
 }
 
 __global__ __launch_bounds__(256) void k_ens_mask(const EnsMember *__restrict__ T, const int *__restrict__ start,
@@ -1383,7 +1412,7 @@ __global__ __launch_bounds__(256) void k_ens_mask(const EnsMember *__restrict__ 
   const EnsMember &E = T[m];
   const long q = (long)(blockIdx.x - start[m]) * 256 + threadIdx.x;
   if (q >= (long)E.P.Mx * E.P.My) return;
-  mask_point(E.P, E.M.H_new, E.F.b, E.H, E.mask, E.h, E.A.err, q);
+  mask_point(E.P, glb(E.M.H_new), glb(E.F.b), glb(E.H), glb(E.mask), glb(E.h), glb(E.A.err), q);
 }
 
 int ens_point_blocks(const DevParams &P) { return (int)(((long)P.Mx * P.My + 255) / 256); }

@@ -33,6 +33,11 @@ struct VelArgs {
   const double *z, *mask, *I0, *I1, *hx, *hy, *ub, *vb, *bmr;
   const int *ks;
   double *u, *v, *w;
+  // the volumetric strain heating in the same pass when Sigma is given:
+  // thickness, enthalpy, Sigma [Mx][My][Mz], error flags
+  const double *H, *E;
+  double *Sigma;
+  int *err;
 };
 
 // SIA_Sliding (SIA_Sliding.cc:106-267): mode 0 pressure-melting-activated
@@ -45,12 +50,6 @@ struct SlideArgs {
   double *ub, *vb, *heat;    // [Mx][My]
   int mode;
   double mu, min_T, secpera;
-};
-
-struct HeatArgs {
-  const double *z, *H, *mask, *E, *u, *v;
-  double *Sigma;             // [Mx][My][Mz]
-  int *err;
 };
 
 struct MassArgs {
@@ -78,7 +77,6 @@ int column_blocks(const DevParams &P);
 // after A.dmax_part[column_blocks(P) - 1]; all are reduced into A.dmax_bits
 void launch_column(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st, int extra_partials = 0);
 void launch_velocity_uvw(const DevParams &P, const VelArgs &A, hipStream_t st);
-void launch_strain_heating(const DevParams &P, const HeatArgs &A, hipStream_t st);
 void launch_sia_sliding(const DevParams &P, const SlideArgs &A, hipStream_t st);
 void launch_surface_values(const DevParams &P, const double *z, const double *f, const double *H, double *out,
                            hipStream_t st);
@@ -91,6 +89,58 @@ long mass_partial_blocks(const DevParams &P);
 void launch_mass_continuity(const DevParams &P, const MassArgs &A, double *acc_out, hipStream_t st);
 void launch_mask_surface(const DevParams &P, const double *H_new, const double *bed, double *H,
                          double *mask, double *h, int *err, hipStream_t st);
+
+// Pointers read from a device table are generic to the compiler, and its
+// accesses through them become flat instructions, which also wait on the LDS
+// counter. Everything the tables point to is device memory: rebuilding the
+// pointer from its integer value as a global-address-space pointer lets the
+// compiler issue global loads and stores (null stays null).
+template <typename T>
+__device__ __forceinline__ T *glb(T *p) {
+  return (T *)(__attribute__((address_space(1))) T *)(uintptr_t)p;
+}
+
+__device__ __forceinline__ Fields2D global_view(Fields2D f) {
+  f.H = glb(f.H);
+  f.b = glb(f.b);
+  f.h = glb(f.h);
+  f.mask = glb(f.mask);
+  return f;
+}
+
+__device__ __forceinline__ ColumnArgs global_view(ColumnArgs a) {
+  a.z = glb(a.z);
+  a.E = glb(a.E);
+  a.age = glb(a.age);
+  a.thks = glb(a.thks);
+  a.theta = glb(a.theta);
+  a.hx = glb(a.hx);
+  a.hy = glb(a.hy);
+  a.flux = glb(a.flux);
+  a.Dstag = glb(a.Dstag);
+  a.I0 = glb(a.I0);
+  a.I1 = glb(a.I1);
+  a.ks = glb(a.ks);
+  a.dmax_bits = glb(a.dmax_bits);
+  a.dmax_part = glb(a.dmax_part);
+  a.err = glb(a.err);
+  a.only = glb(a.only);
+  return a;
+}
+
+__device__ __forceinline__ MassArgs global_view(MassArgs a) {
+  a.dt = glb(a.dt);
+  a.H = glb(a.H);
+  a.flux = glb(a.flux);
+  a.mask = glb(a.mask);
+  a.ub = glb(a.ub);
+  a.vb = glb(a.vb);
+  a.smb = glb(a.smb);
+  a.bmr = glb(a.bmr);
+  a.H_new = glb(a.H_new);
+  a.partials = glb(a.partials);
+  return a;
+}
 
 // ---- ensembles: one fast SIA step + dt + mass continuity of every member, one
 // launch per stage (device table of members; the member's fields are its own)

@@ -690,7 +690,8 @@ __device__ __forceinline__ void act_to_chunks(const f32x4 *acc, const float *ub,
 
 // layer-0 inputs of one sample row: features 8g..8g+7 (chunk 0), 32+8g..+7 (chunk 1);
 // returns nonzero when a feature lies outside the training range
-__device__ __forceinline__ int load_inputs(const SgMlpArgs &A, int slot, bool live, int g, bf16x8 &x0, bf16x8 &x1) {
+__device__ __forceinline__ int load_inputs(const SgMlpArgs &A, const SgNorm &N, int slot, bool live, int g,
+                                           bf16x8 &x0, bf16x8 &x1) {
   f32x8 v0, v1;
   int ood = 0;
   const float *fr = A.F + (long)(live ? slot : 0) * SG_FSTRIDE;
@@ -698,14 +699,14 @@ __device__ __forceinline__ int load_inputs(const SgMlpArgs &A, int slot, bool li
   for (int jj = 0; jj < 8; jj++) {
     const int f = 8 * g + jj;
     const float v = fr[f];
-    ood |= (v < A.norm.lo[f]) | (v > A.norm.hi[f]);
-    v0[jj] = (v - A.norm.mean[f]) * A.norm.inv_std[f];
+    ood |= (v < N.lo[f]) | (v > N.hi[f]);
+    v0[jj] = (v - N.mean[f]) * N.inv_std[f];
     const int f1 = 32 + 8 * g + jj;
     float w1 = 0.0f;
     if (f1 < SG_NIN) {
       const float r = fr[f1];
-      ood |= (r < A.norm.lo[f1]) | (r > A.norm.hi[f1]);
-      w1 = (r - A.norm.mean[f1]) * A.norm.inv_std[f1];
+      ood |= (r < N.lo[f1]) | (r > N.hi[f1]);
+      w1 = (r - N.mean[f1]) * N.inv_std[f1];
     }
     v1[jj] = w1;
   }
@@ -720,12 +721,13 @@ __device__ __forceinline__ int load_inputs(const SgMlpArgs &A, int slot, bool li
 // each wavefront evaluates 16-sample tiles.
 constexpr int SG_MLP_WAVES = 16;
 
-// Block bid of nblk persistent blocks walking the list. LDS: wfrag (dynamic):
-// [SG_NFRAG][64][8] weights, then the GELU table; bias: the hidden biases as
-// GELU-table offsets (b - lo) S, then the raw output biases; wmax, wfall: one
-// entry per wavefront.
-__device__ __forceinline__ void sg_mlp_block(const DevParams &P, const SgMlpArgs &A, int bid, int nblk,
-                                             unsigned short *wfrag, float *bias, double *wmax, int *wfall) {
+// Block bid of nblk persistent blocks walking the list (N: the normalization,
+// A.norm or a copy of it). LDS: wfrag (dynamic): [SG_NFRAG][64][8] weights,
+// then the GELU table; bias: the hidden biases as GELU-table offsets
+// (b - lo) S, then the raw output biases; wmax, wfall: one entry per wavefront.
+__device__ __forceinline__ void sg_mlp_block(const DevParams &P, const SgMlpArgs &A, const SgNorm &N, int bid,
+                                             int nblk, unsigned short *wfrag, float *bias, double *wmax,
+                                             int *wfall) {
   float2 *gtab = reinterpret_cast<float2 *>(wfrag + SG_NFRAG * 512);
   {
     const uint4 *src = reinterpret_cast<const uint4 *>(A.frag);
@@ -755,7 +757,7 @@ __device__ __forceinline__ void sg_mlp_block(const DevParams &P, const SgMlpArgs
     const int slot = tile * 16 + smp;
     const bool live = slot < n;
     bf16x8 x0, x1;
-    const int ood = load_inputs(A, slot, live, g, x0, x1);
+    const int ood = load_inputs(A, N, slot, live, g, x0, x1);
     f32x4 acc[8];
     bf16x8 h[4];
 #pragma unroll
@@ -789,7 +791,7 @@ __device__ __forceinline__ void sg_mlp_block(const DevParams &P, const SgMlpArgs
     if (live) {
       const float4 q0 = *reinterpret_cast<const float4 *>(A.CQb + (long)slot * SG_NCELL + 4 * g);
       const float4 q1 = *reinterpret_cast<const float4 *>(A.CQb + (long)slot * SG_NCELL + 16 + 4 * g);
-      const float rs = A.norm.rscale;
+      const float rs = N.rscale;
       part = q0.x * __expf((o0[0] + bo[4 * g + 0]) * rs) + q0.y * __expf((o0[1] + bo[4 * g + 1]) * rs) +
              q0.z * __expf((o0[2] + bo[4 * g + 2]) * rs) + q0.w * __expf((o0[3] + bo[4 * g + 3]) * rs) +
              q1.x * __expf((o1[0] + bo[16 + 4 * g + 0]) * rs) + q1.y * __expf((o1[1] + bo[16 + 4 * g + 1]) * rs) +
@@ -841,7 +843,7 @@ __global__ __launch_bounds__(64 * SG_MLP_WAVES) void k_sg_mlp(DevParams P, SgMlp
   __shared__ __align__(16) float bias[4 * SG_WIDTH + SG_NOUT];
   __shared__ double wmax[SG_MLP_WAVES];
   __shared__ int wfall[SG_MLP_WAVES];
-  sg_mlp_block(P, A, blockIdx.x, gridDim.x, wfrag, bias, wmax, wfall);
+  sg_mlp_block(P, A, A.norm, blockIdx.x, gridDim.x, wfrag, bias, wmax, wfall);
 }
 
 static size_t sg_mlp_lds() { return (size_t)SG_NFRAG * 512 * sizeof(unsigned short) + sizeof(float2) * SG_GTAB; }
@@ -861,7 +863,9 @@ void launch_sg_mlp(const DevParams &P, const SgMlpArgs &A, hipStream_t st) {
 __global__ __launch_bounds__(SG_CLS_BLOCK) void k_ens_sg_classify(const EnsSgMember *__restrict__ T,
                                                                    const int *__restrict__ start, int nm) {
   const int m = ens_member(start, nm, blockIdx.x);
-  sg_classify_block(T[m].P, T[m].C, T[m].L, blockIdx.x - start[m]);
+  const SgColumnArgs C = global_view(T[m].C);
+  const SgListArgs L = global_view(T[m].L);
+  sg_classify_block(T[m].P, C, L, blockIdx.x - start[m]);
 }
 
 // one block per member
@@ -871,14 +875,16 @@ __global__ __launch_bounds__(1024) void k_ens_sg_scan(const EnsSgMember *__restr
   __shared__ long long tot[3];
   const int m = blockIdx.x;
   if (m >= nm) return;
-  sg_scan_block(start[m + 1] - start[m], T[m].L, part, tot);
+  const SgListArgs L = global_view(T[m].L);
+  sg_scan_block(start[m + 1] - start[m], L, part, tot);
 }
 
 __global__ __launch_bounds__(SG_CLS_BLOCK) void k_ens_sg_scatter(const EnsSgMember *__restrict__ T,
                                                                   const int *__restrict__ start, int nm) {
   __shared__ int wsum[SG_CLS_BLOCK / 64];
   const int m = ens_member(start, nm, blockIdx.x);
-  sg_scatter_block(T[m].P, T[m].L, blockIdx.x - start[m], wsum);
+  const SgListArgs L = global_view(T[m].L);
+  sg_scatter_block(T[m].P, L, blockIdx.x - start[m], wsum);
 }
 
 template <int NP>
@@ -887,7 +893,9 @@ __global__ __launch_bounds__(256) void k_ens_sg_features(const EnsSgMember *__re
   extern __shared__ double2 zsk[];
   __shared__ int bucket[SG_NBUCKET];
   const int m = ens_member(start, nm, blockIdx.x);
-  sg_features_block<false, NP>(T[m].P, T[m].C, T[m].F, blockIdx.x - start[m], start[m + 1] - start[m], zsk, bucket);
+  const SgColumnArgs C = global_view(T[m].C);
+  const SgFeatureArgs F = global_view(T[m].F);
+  sg_features_block<false, NP>(T[m].P, C, F, blockIdx.x - start[m], start[m + 1] - start[m], zsk, bucket);
 }
 
 __global__ __launch_bounds__(64 * SG_MLP_WAVES) void k_ens_sg_mlp(const EnsSgMember *__restrict__ T,
@@ -897,7 +905,9 @@ __global__ __launch_bounds__(64 * SG_MLP_WAVES) void k_ens_sg_mlp(const EnsSgMem
   __shared__ double wmax[SG_MLP_WAVES];
   __shared__ int wfall[SG_MLP_WAVES];
   const int m = ens_member(start, nm, blockIdx.x);
-  sg_mlp_block(T[m].P, T[m].M, blockIdx.x - start[m], start[m + 1] - start[m], wfrag, bias, wmax, wfall);
+  const SgMlpArgs M = global_view(T[m].M);
+  sg_mlp_block(T[m].P, M, *glb(&T[m].M.norm), blockIdx.x - start[m], start[m + 1] - start[m], wfrag, bias, wmax,
+               wfall);
 }
 
 int sg_features_waves(int Mz) { return sg_features_lds(Mz, 4, false) <= 64 * 1024 ? 4 : 1; }

@@ -119,6 +119,60 @@ int sg_mlp_blocks();
 void launch_sg_mlp(const DevParams &P, const SgMlpArgs &A, hipStream_t st);
 void launch_sg_pack(const float *master, unsigned short *frag, hipStream_t st);
 
+// global-address-space views of table-held arguments (see glb)
+__device__ __forceinline__ SgColumnArgs global_view(SgColumnArgs a) {
+  a.z = glb(a.z);
+  a.E = glb(a.E);
+  a.thks = glb(a.thks);
+  a.theta = glb(a.theta);
+  a.hx = glb(a.hx);
+  a.hy = glb(a.hy);
+  a.err = glb(a.err);
+  return a;
+}
+
+__device__ __forceinline__ SgListArgs global_view(SgListArgs a) {
+  a.status = glb(a.status);
+  a.block_count = glb(a.block_count);
+  a.list = glb(a.list);
+  a.n_list = glb(a.n_list);
+  a.counts = glb(a.counts);
+  a.flux = glb(a.flux);
+  a.Dstag = glb(a.Dstag);
+  return a;
+}
+
+__device__ __forceinline__ SgFeatureArgs global_view(SgFeatureArgs a) {
+  a.list = glb(a.list);
+  a.n_list = glb(a.n_list);
+  a.F = glb(a.F);
+  a.CQb = glb(a.CQb);
+  a.CSt = glb(a.CSt);
+  a.status = glb(a.status);
+  a.samples = glb(a.samples);
+  a.n_samples = glb(a.n_samples);
+  return a;
+}
+
+__device__ __forceinline__ SgMlpArgs global_view(const SgMlpArgs &a) {
+  SgMlpArgs g = a;
+  g.list = glb(a.list);
+  g.n_list = glb(a.n_list);
+  g.F = glb(a.F);
+  g.CQb = glb(a.CQb);
+  g.CSt = glb(a.CSt);
+  g.frag = glb(a.frag);
+  g.bias = glb(a.bias);
+  g.hx = glb(a.hx);
+  g.hy = glb(a.hy);
+  g.status = glb(a.status);
+  g.flux = glb(a.flux);
+  g.Dstag = glb(a.Dstag);
+  g.dmax_part = glb(a.dmax_part);
+  g.counts = glb(a.counts);
+  return g;
+}
+
 // ---- ensembles: the surrogate stages of every member bound to one surrogate,
 // one launch per stage. Each member keeps its own list, feature rows and
 // status; blocks map to members through block-prefix tables as in the plain
