@@ -33,8 +33,14 @@ sizes, one kernel launch per stage for all members, 50 steps captured in a
 HIP graph) and reports it under "ensemble", beside the same members stepped
 with per-member launches.
 
+--combined runs the combined workload instead and prints its own line: the
+surrogate inside the 64-member ensemble (the sheet + 63 ensemble members),
+5000 steps in HIP graphs of 100, the surrogate's flux error judged in FP64
+every 100 steps, beside an all-plain FP64 run of the same seeds.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--path surrogate|plain]
                        [--mode full|fast] [--no-cpu-baseline] [--no-ensemble] [--copy-calibration GIB]
+                       [--combined]
 """
 import argparse
 import json
@@ -189,16 +195,16 @@ def time_steps(S, torch, sia, f, steps, warmup, fast):
 ENS_SIZES = (201, 401, 601, 801)
 
 
-def ensemble_members(n=64, seed=2026):
+def ensemble_members(n=64, seed=2026, base=1000):
     """The ensemble workload: n members with seeded sizes Mx, My in ENS_SIZES,
     dx ~ U[2, 5] km, Mz ~ U{101..201} (Lz 4000 m), GPBLD with e ~ U[1, 5] and
-    n in {3, 4}, each a polythermal sheet of its own seed."""
+    n in {3, 4}, each a polythermal sheet of its own seed (base + member)."""
     rng = np.random.default_rng(seed)
     out = []
     for m in range(n):
         Mx, My = (int(v) for v in rng.choice(ENS_SIZES, 2))
         out.append(dict(Mx=Mx, My=My, dx=float(rng.uniform(2e3, 5e3)), Mz=int(rng.integers(101, 202)),
-                        e=float(rng.uniform(1.0, 5.0)), n=float(rng.choice([3.0, 4.0])), seed=1000 + m))
+                        e=float(rng.uniform(1.0, 5.0)), n=float(rng.choice([3.0, 4.0])), seed=base + m))
     return out
 
 
@@ -258,6 +264,233 @@ def run_ensemble(S, torch, steps=50, reps=2):
                         "mass continuity of every member", "dtype": "f64"}
 
 
+COMBINED_STEPS, COMBINED_CHECK = 5000, 100
+# The surrogate's cost per column is the MLP's (fixed) plus a feature pass in
+# proportion to the column's levels, the FP64 kernel's is in proportion to the
+# levels alone: measured on MI355X, the surrogate is ahead for the 401-level
+# sheet (ks ~ 320) and behind for 101..201-level members (ks ~ 100). The
+# depth policy binds it to members of at least this many levels.
+SURROGATE_MIN_MZ = 301
+
+
+def combined_members(S, torch):
+    """The combined workload: the 1501x2801x401 sheet plus the first 63
+    members of the ensemble workload, each with its grid, parameters and
+    device fields (regenerated from the seeds on every call)."""
+    import sia_types as T
+    import workloads as W
+    z = T.vertical_levels(LZ, MZ)
+    out = [dict(grid=T.make_grid(MX, MY, DX, DX, z), params=W.gl_params(), n=3.0, z=z, Mz=MZ)]
+    out[0]["f"] = S.generate_greenland_like(out[0]["grid"], out[0]["params"])
+    for m in ensemble_members()[:63]:
+        zm = T.vertical_levels(LZ, m["Mz"])
+        grid = T.make_grid(m["Mx"], m["My"], m["dx"], m["dx"], zm)
+        params = W.gl_params(sia_enhancement_factor=m["e"], sia_Glen_exponent=m["n"])
+        f = S.generate_greenland_like(grid, params, Mx_full=m["Mx"], My_full=m["My"], seed=m["seed"], dx=m["dx"])
+        out.append(dict(grid=grid, params=params, n=m["n"], z=zm, Mz=m["Mz"], f=f))
+    for m in out:
+        m["sia"] = S.SIAFD(m["grid"], m["params"])
+        m["sia"].set_sync_check(False)
+        f = m["f"]
+        m["sia"].update(f["H"], f["b"], f["h"], f["mask"], f["E"], fast=True)  # bed smoother, buffers
+        m["geom"] = S.geometry(f["H"], f["h"], f["mask"], f["b"])
+        m["cols"] = m["grid"].Mx * m["grid"].My
+    torch.cuda.synchronize()
+    return out
+
+
+def train_combined_surrogate(S, torch, n, train_steps, batch):
+    """A surrogate for Glen exponent n, trained on sheets and ensemble-type
+    members of other seeds than the combined run's."""
+    import sia_types as T
+    import workloads as W
+    t0 = time.perf_counter()
+    sources = []
+    if n == 3.0:
+        z = T.vertical_levels(LZ, MZ)
+        sources += [(T.make_grid(MX, MY, DX, DX, z), W.gl_params(), dict(seed=sd)) for sd in TRAIN_SEEDS[:6]]
+    for m in ensemble_members(n=64, seed=4242, base=50000):
+        if m["n"] == n:
+            zm = T.vertical_levels(LZ, m["Mz"])
+            sources.append((T.make_grid(m["Mx"], m["My"], m["dx"], m["dx"], zm),
+                            W.gl_params(sia_enhancement_factor=m["e"], sia_Glen_exponent=m["n"]),
+                            dict(Mx_full=m["Mx"], My_full=m["My"], seed=m["seed"], dx=m["dx"])))
+    sg = S.Surrogate(seed=int(n) + 5, capacity=45_000_000)
+    total = 0
+    for grid, params, kw in sources:
+        ft = S.generate_greenland_like(grid, params, **kw)
+        c = S.SIAFD(grid, params)
+        c.set_sync_check(False)
+        total = sg.collect(c, ft["H"], ft["b"], ft["h"], ft["mask"], ft["E"], stride=1)
+        c.close()
+        del ft
+    torch.cuda.synchronize()
+    t_collect = time.perf_counter() - t0
+    sg.fit_normalization()
+    t1 = time.perf_counter()
+    loss = sg.train(train_steps, batch=batch, lr_start=2e-3, lr_end=1e-5, seed=9)
+    torch.cuda.synchronize()
+    info = {"glen_n": n, "samples": int(total), "sources": len(sources), "collect_s": round(t_collect, 2),
+            "train_s": round(time.perf_counter() - t1, 2), "loss_first100": float(loss[:100].mean()),
+            "loss_last100": float(loss[-100:].mean())}
+    log("  surrogate n=%g: %s" % (n, info))
+    return sg, info
+
+
+def flux_check(members, rng, all_columns):
+    """The surrogate's flux error on the current state of every member: the
+    member's surrogate update against an FP64 update of the same state, on a
+    seeded 1 % of its staggered columns (and on all of them if all_columns)."""
+    worst_s, worst_a, p9999 = 0.0, 0.0, 0.0
+    for m in members:
+        if m.get("sg") is None:
+            continue
+        sia, f = m["sia"], m["f"]
+        sia.update(f["H"], f["b"], f["h"], f["mask"], f["E"], fast=True)
+        q_s = sia.diffusive_flux()
+        sia.use_surrogate(None)
+        sia.update(f["H"], f["b"], f["h"], f["mask"], f["E"], fast=True)
+        q_p = sia.diffusive_flux()
+        sia.use_surrogate(m["sg"])
+        pick = rng.choice(q_p.size, max(1, q_p.size // 100), replace=False)
+        a, b = q_s[pick], q_p[pick]
+        nz = b != 0
+        if nz.any():
+            worst_s = max(worst_s, float((np.abs(a[nz] - b[nz]) / np.abs(b[nz])).max()))
+        if all_columns:
+            st = error_stats(q_s, q_p)
+            worst_a = max(worst_a, st["max"])
+            p9999 = max(p9999, st["p99_99"])
+    return worst_s, (worst_a if all_columns else None), (p9999 if all_columns else None)
+
+
+def run_combined_leg(S, torch, sgs, steps, check_every, rng=None, min_mz=0):
+    """steps ensemble steps of the combined members, as HIP graphs of
+    check_every steps; with sgs, members of at least min_mz levels are bound
+    to the surrogate of their Glen exponent and its FP64 check runs after
+    each graph, outside the timed launches. Returns the timing, the checks
+    and the final geometry."""
+    members = combined_members(S, torch)
+    if sgs is not None:
+        for m in members:
+            if m["Mz"] >= min_mz:
+                m["sg"] = sgs[m["n"]]
+                m["sia"].use_surrogate(m["sg"])
+    alg = 0.0
+    for m in members:
+        _, thks = m["sia"].theta_thksmooth()
+        g = m["grid"]
+        alg += algorithmic_bytes(thks.reshape(g.Mx, g.My), m["z"], m["Mz"], full=False)["SIA flux"]
+    ens = S.Ensemble([m["sia"] for m in members], [m["geom"] for m in members], [m["f"]["E"] for m in members])
+    chosen, _ = ens.autotune(steps=3)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ms_total, checks = 0.0, []
+    nchunks = steps // check_every
+    for c in range(nchunks):
+        torch.cuda.synchronize()
+        a.record()
+        ens.step(check_every, graph=True)
+        b.record()
+        torch.cuda.synchronize()
+        ms_total += a.elapsed_time(b)
+        if sgs is not None:
+            ws, wa, p = flux_check(members, rng, all_columns=((c + 1) % 10 == 0 or c + 1 == nchunks))
+            checks.append({"step": (c + 1) * check_every, "max_sampled": ws, "max_all": wa, "p99_99_all": p})
+        if (c + 1) % 5 == 0:
+            log("  %s: %d steps, %.2f ms/step%s" % ("surrogate" if sgs else "all-plain", (c + 1) * check_every,
+                                                   ms_total / ((c + 1) * check_every),
+                                                   "" if not checks else ", max sampled flux err %.2e" % max(
+                                                       x["max_sampled"] for x in checks)))
+    ens.check_errors()
+    final = {"H": [m["f"]["H"].cpu().numpy() for m in members], "t": [ens.time(i)[0] for i in range(len(members))]}
+    counts = [m["sia"].surrogate_counts() for m in members if m.get("sg") is not None] if sgs else None
+    bound = sum(1 for m in members if m.get("sg") is not None)
+    cols = sum(m["cols"] for m in members)
+    ens.close()
+    for m in members:
+        m["sia"].close()
+    del members
+    torch.cuda.synchronize()
+    return {"ms_per_step": ms_total / steps, "cols": cols, "alg_bytes": alg, "tile": chosen, "checks": checks,
+            "final": final, "counts": counts, "bound": bound}
+
+
+def leg_error(leg):
+    cnt = np.array(leg["counts"]).sum(axis=0)
+    return {"checks": len(leg["checks"]), "every_steps": COMBINED_CHECK, "sampled_fraction": 0.01,
+            "max_rel_flux_err_sampled": max(c["max_sampled"] for c in leg["checks"]),
+            "max_rel_flux_err_all_columns": max(c["max_all"] for c in leg["checks"] if c["max_all"] is not None),
+            "p99_99_rel_flux_err_all_columns": max(c["p99_99_all"] for c in leg["checks"]
+                                                   if c["p99_99_all"] is not None),
+            "against": "an FP64 update of the same state of each bound member, every %d steps (all columns every "
+                       "%d steps)" % (COMBINED_CHECK, 10 * COMBINED_CHECK),
+            "bound_columns_last_check": {"zero": int(cnt[0]), "surrogate": int(cnt[1]), "fp64_fallback": int(cnt[2])}}
+
+
+def final_vs(leg, ref):
+    dH = [np.abs(a - b) for a, b in zip(leg["final"]["H"], ref["final"]["H"])]
+    vol = [abs(a.sum() - b.sum()) / b.sum() for a, b in zip(leg["final"]["H"], ref["final"]["H"])]
+    yr = 3.15569259747e7
+    return {"max_abs_dH_m": float(max(d.max() for d in dH)), "sheet_max_abs_dH_m": float(dH[0].max()),
+            "max_rel_volume_diff": float(max(vol)),
+            "sheet_model_years": {"this_run": leg["final"]["t"][0] / yr, "all_plain": ref["final"]["t"][0] / yr}}
+
+
+def run_combined(S, torch, args):
+    """The combined run: the learned surrogate inside the 64-member batched
+    ensemble (the 1501x2801x401 sheet + 63 ensemble members), the surrogate's
+    flux error judged in FP64 every 100 steps, beside an all-plain FP64 run
+    of the same seeds. Two bindings are run: the depth policy (members of at
+    least SURROGATE_MIN_MZ levels; the reported value) and every member."""
+    steps = args.combined_steps
+    if steps < COMBINED_CHECK or steps % COMBINED_CHECK:
+        raise SystemExit("--combined-steps must be a positive multiple of %d" % COMBINED_CHECK)
+    log("combined run: training one surrogate per Glen exponent")
+    sgs, training = {}, []
+    for n in (3.0, 4.0):
+        sgs[n], info = train_combined_surrogate(S, torch, n, args.train_steps, args.train_batch)
+        training.append(info)
+    log("combined run: surrogate on members with Mz >= %d, %d steps" % (SURROGATE_MIN_MZ, steps))
+    pol = run_combined_leg(S, torch, sgs, steps, COMBINED_CHECK, np.random.default_rng(20261203),
+                           min_mz=SURROGATE_MIN_MZ)
+    log("combined run: surrogate on every member, %d steps" % steps)
+    allb = run_combined_leg(S, torch, sgs, steps, COMBINED_CHECK, np.random.default_rng(20261203))
+    for sg in sgs.values():
+        sg.close()
+    log("combined run: all-plain FP64 ensemble, %d steps" % steps)
+    pln = run_combined_leg(S, torch, None, steps, COMBINED_CHECK)
+    cols = pol["cols"]
+    gbs = pol["alg_bytes"] / (pol["ms_per_step"] / 1000.0) / 1e9
+    out = {
+        "metric": "SIA column-updates/s, combined run: surrogate inside the 64-member ensemble",
+        "value": round(cols / (pol["ms_per_step"] / 1000.0), 1), "unit": "column-updates/s", "n_gpus": 1,
+        "steps": steps, "ms_per_step": round(pol["ms_per_step"], 4), "higher_is_better": True,
+        "dtype": "f64+bf16", "data": "synthetic (seeded sheets generated in HBM)",
+        "config": {"workload": "1501x2801x401 greenland_like sheet (GPBLD e=3 n=3) + the first 63 members of the "
+                               "64-member ensemble workload (Mx, My in {201,401,601,801}, Mz 101..201, e U[1,5], "
+                               "n in {3,4}); one step = fast SIA update + dt + mass continuity of every member; "
+                               "%d-step HIP graphs" % COMBINED_CHECK,
+                   "members": 64, "columns": cols, "tile_cols_per_wave": pol["tile"],
+                   "binding": "surrogate on members with Mz >= %d (%d bound), FP64 on the others"
+                              % (SURROGATE_MIN_MZ, pol["bound"])},
+        "speedup_vs_all_plain": round(pln["ms_per_step"] / pol["ms_per_step"], 3),
+        "hbm": {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                "algorithmic_bytes_per_step": round(pol["alg_bytes"]),
+                "bytes": "E of every column up to its largest face ks (8 B a level) + 88 B of 2-D data per column"},
+        "surrogate_error": leg_error(pol),
+        "final_state_vs_all_plain": final_vs(pol, pln),
+        "all_plain_fp64": {"value": round(cols / (pln["ms_per_step"] / 1000.0), 1),
+                           "ms_per_step": round(pln["ms_per_step"], 4), "tile_cols_per_wave": pln["tile"]},
+        "every_member_bound": {"value": round(cols / (allb["ms_per_step"] / 1000.0), 1),
+                               "ms_per_step": round(allb["ms_per_step"], 4), "bound": allb["bound"],
+                               "speedup_vs_all_plain": round(pln["ms_per_step"] / allb["ms_per_step"], 3),
+                               "surrogate_error": leg_error(allb),
+                               "final_state_vs_all_plain": final_vs(allb, pln)},
+        "training": training,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def error_stats(a, b):
     """Relative error of a against b over the entries where b != 0."""
     nz = b != 0
@@ -287,6 +520,10 @@ def main():
     ap.add_argument("--train-batch", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-member ensemble measurement")
+    ap.add_argument("--combined", action="store_true",
+                    help="run the combined workload instead (surrogate inside the 64-member ensemble, 5000 steps)")
+    ap.add_argument("--combined-steps", type=int, default=COMBINED_STEPS,
+                    help="steps of each combined-run leg (a multiple of %d)" % COMBINED_CHECK)
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="GIB",
                     help="after the timed steps, run one plain device copy of this many GiB "
                          "(the reference kernel for scaling the rocprofv3 byte counters)")
@@ -303,6 +540,9 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if args.combined:
+        run_combined(S, torch, args)
+        return
     z = T.vertical_levels(LZ, MZ)
     grid = T.make_grid(MX, MY, DX, DX, z)
     params = W.gl_params()
