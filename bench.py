@@ -76,16 +76,17 @@ def staggered_ks(thks, z):
     return np.stack(out, axis=-1)  # [Mx, My, 2]
 
 
-def algorithmic_bytes(thks, z, Mz, full=True):
+def algorithmic_bytes(thks, z, Mz, full=True, H=None):
     """Bytes each stage must move at least once, for one update of the sheet.
 
     SIA flux (column kernel): E of each map column up to the largest ks of the
     four staggered columns that read it (8 B a level); I of each staggered
     column up to its ks (8 B a level, full updates); 2-D: thksmooth, theta,
     h_x, h_y read, flux, D_stag, ks written (88 B a map column).
-    SIA 3D hor. vel. (fused u, v, w kernel): u, v, w written at every level
-    (24 Mz B a map column); I of every staggered column up to its ks; ks,
-    h_x, h_y and mask read (48 B a map column)."""
+    SIA 3D hor. vel. (one pass for u, v, w and the strain heating): u, v, w
+    and Sigma written at every level (32 Mz B a map column); I of every
+    staggered column up to its ks; E of each map column up to kBelowHeight(H)
+    (8 B a level); ks, h_x, h_y, mask and H read (56 B a map column)."""
     ks = staggered_ks(thks, z)  # [Mx, My, 2]
     lev_I = np.where(ks < 0, 1, ks + 1)  # thk == 0 stores I[0] only
     # E levels of map column (i,j): faces (i,j,0), (i,j,1), (i-1,j,0), (i,j-1,1)
@@ -97,7 +98,8 @@ def algorithmic_bytes(thks, z, Mz, full=True):
     flux = e_bytes + 88.0 * n + (i_bytes if full else 0.0)
     out = {"SIA flux": flux}
     if full:
-        out["SIA 3D hor. vel."] = 24.0 * Mz * n + i_bytes + 48.0 * n
+        ksH = np.clip(np.searchsorted(z, H, side="right") - 1, 0, len(z) - 2)
+        out["SIA 3D hor. vel."] = 32.0 * Mz * n + i_bytes + 8.0 * float((ksH + 1).sum()) + 56.0 * n
     else:
         # surrogate: the feature pass reads E of the listed columns and writes
         # 272 B per staggered column (34 inputs, 32 cell baselines, C S_t);
@@ -627,7 +629,7 @@ def main():
     # smoothed thickness of the initial state
     plain.update(f["H"], f["b"], f["h"], f["mask"], f["E"], fast=True)
     _, thks = plain.theta_thksmooth()
-    alg = algorithmic_bytes(thks.reshape(MX, MY), z, MZ, full=not fast_mode)
+    alg = algorithmic_bytes(thks.reshape(MX, MY), z, MZ, full=not fast_mode, H=f["H"].cpu().numpy().reshape(MX, MY))
     prefer = ["surrogate features", "surrogate MLP"] if args.path == "surrogate" else list(alg)
     roofline = roofline_of(stages, alg, prefer)
     if args.path == "surrogate":
