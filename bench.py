@@ -160,6 +160,9 @@ def cpu_baseline(f, grid_z, params, full=True, n_windows=96, win=128, seed=7):
 
 
 TRAIN_SEEDS = [101, 202, 303, 404, 505, 606, 707, 808, 909, 1010, 1111, 1212]
+# share of a training sheet's surrogate columns whose fallback score exceeds
+# the calibrated threshold (they are computed in FP64 instead)
+SG_FALLBACK_FRACTION = 0.005
 
 
 def clone_geometry(S, f):
@@ -332,9 +335,18 @@ def train_combined_surrogate(S, torch, n, train_steps, batch):
     t1 = time.perf_counter()
     loss = sg.train(train_steps, batch=batch, lr_start=2e-3, lr_end=1e-5, seed=9)
     torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+    # score fallback threshold from one of the training sources
+    grid, params, kw = sources[0]
+    ft = S.generate_greenland_like(grid, params, **kw)
+    cal = S.SIAFD(grid, params)
+    tau = sg.calibrate_fallback(cal, ft["H"], ft["b"], ft["h"], ft["mask"], ft["E"], SG_FALLBACK_FRACTION)
+    cal.close()
+    del ft
     info = {"glen_n": n, "samples": int(total), "sources": len(sources), "collect_s": round(t_collect, 2),
-            "train_s": round(time.perf_counter() - t1, 2), "loss_first100": float(loss[:100].mean()),
-            "loss_last100": float(loss[-100:].mean())}
+            "train_s": round(t_train, 2), "loss_first100": float(loss[:100].mean()),
+            "loss_last100": float(loss[-100:].mean()), "fallback_score_tau": tau,
+            "fallback_fraction_calibrated": SG_FALLBACK_FRACTION}
     log("  surrogate n=%g: %s" % (n, info))
     return sg, info
 
@@ -568,10 +580,13 @@ def main():
         collector = S.SIAFD(grid, params, stream=stream.cuda_stream)
         collector.set_sync_check(False)
         n = 0
-        for seed in TRAIN_SEEDS:
+        for k, seed in enumerate(TRAIN_SEEDS):
             ft = S.generate_greenland_like(grid, params, seed=seed)
-            n = sg.collect(collector, ft["H"], ft["b"], ft["h"], ft["mask"], ft["E"], stride=1)
+            # each sheet has its own bed: a new bed state makes the handle smooth it again
+            n = sg.collect(collector, ft["H"], ft["b"], ft["h"], ft["mask"], ft["E"], stride=1,
+                           bed_state_counter=k + 1)
             del ft
+        assert S.lib().sia_check_errors(collector.h) == 0, S.lib().sia_last_error(collector.h)
         collector.close()
         torch.cuda.synchronize()
         t_collect = time.perf_counter() - t0
@@ -580,8 +595,14 @@ def main():
         loss = sg.train(args.train_steps, batch=args.train_batch, lr_start=2e-3, lr_end=1e-5, seed=9)
         torch.cuda.synchronize()
         t_train = time.perf_counter() - t1
-        log("  %d samples (%.1f s), %d steps of %d (%.1f s), loss %.3e -> %.3e" % (
-            n, t_collect, args.train_steps, args.train_batch, t_train, loss[:100].mean(), loss[-100:].mean()))
+        # score fallback threshold from the last training sheet (never the benchmark sheet)
+        ft = S.generate_greenland_like(grid, params, seed=TRAIN_SEEDS[-1])
+        cal = S.SIAFD(grid, params, stream=stream.cuda_stream)
+        tau = sg.calibrate_fallback(cal, ft["H"], ft["b"], ft["h"], ft["mask"], ft["E"], SG_FALLBACK_FRACTION)
+        cal.close()
+        del ft
+        log("  %d samples (%.1f s), %d steps of %d (%.1f s), loss %.3e -> %.3e, fallback score tau %.3e" % (
+            n, t_collect, args.train_steps, args.train_batch, t_train, loss[:100].mean(), loss[-100:].mean(), tau))
         sur = S.SIAFD(grid, params, stream=stream.cuda_stream)
         sur.set_sync_check(False)
         sur.use_surrogate(sg)
@@ -608,7 +629,9 @@ def main():
             "surrogate_error": err,
             "training": {"samples": int(n), "sheets": len(TRAIN_SEEDS), "steps": args.train_steps,
                          "batch": args.train_batch, "collect_s": round(t_collect, 2), "train_s": round(t_train, 2),
-                         "loss_first100": float(loss[:100].mean()), "loss_last100": float(loss[-100:].mean())},
+                         "loss_first100": float(loss[:100].mean()), "loss_last100": float(loss[-100:].mean()),
+                         "fallback_score_tau": tau, "fallback_fraction_calibrated": SG_FALLBACK_FRACTION,
+                         "calibration_sheet_seed": TRAIN_SEEDS[-1]},
         }
         fast_mode = True
     else:

@@ -81,6 +81,21 @@ int sia_use_surrogate(sia_handle *h, sia_surrogate *sg);
    [1] surrogate, [2] FP64 fallback. */
 int sia_surrogate_counts(sia_handle *h, int64_t counts[3]);
 
+/* Score fallback. Besides columns outside the training range, a column whose
+   score sum_m t_m |r_m| / D exceeds tau is computed by the FP64 kernel
+   (t_m = C Qb_m exp(r_m): cell m's predicted share of D; r_m: the network's
+   log-correction). The score is the predicted-share-weighted size of the
+   correction; the columns it singles out carry the largest surrogate errors.
+   tau is +inf (no score fallback) until set, and is saved with the model.
+   calibrate_fallback runs one surrogate fast update of `in` on h (fields the
+   model was trained on, not those it will be judged on) with every in-range
+   column kept, and sets tau to the (1 - fraction) quantile of their scores:
+   about `fraction` of such columns then fall back. tau (host) or NULL. */
+int sia_surrogate_calibrate_fallback(sia_surrogate *sg, sia_handle *h, const sia_inputs *in, double fraction,
+                                     double *tau);
+/* tau >= 0, or INFINITY for no score fallback. */
+int sia_surrogate_set_fallback_score(sia_surrogate *sg, double tau);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,7 +12,8 @@ What it restates:
   converter (enthalpyConverter.cc:92-234), vectorised over levels;
 * the surrogate's cell features, physics baseline and targets, and the MLP
   (34 -> 4 x 128 GELU(tanh form) -> 32) forward and backward, optionally with
-  the BF16 rounding of weights and activations that the GPU kernels apply.
+  the BF16 rounding of weights and activations that the GPU kernels apply;
+* the score of the surrogate's FP64 fallback (fallback_score).
 """
 import numpy as np
 
@@ -188,6 +189,18 @@ def staggered_inputs(grid, fields, thks, theta, hx, hy):
         out[key] = np.stack([parts[0][key], parts[1][key]], axis=-1).reshape(-1)
     out["E"] = np.stack([parts[0]["E"], parts[1]["E"]], axis=2).reshape(-1, Mz)
     return out
+
+
+def fallback_score(CQb, CSt, out, rscale):
+    """The library's score fallback: sum_m t_m |r_m| / D with r_m = out_m rscale
+    (the network's log-correction), t_m = C Qb_m exp(r_m) (cell m's predicted
+    part of D) and D = C S_t + sum_m t_m. Cells without cold levels have
+    CQb_m = 0 and add nothing. (No counterpart in the reference: it decides
+    which columns the FP64 kernel computes instead of the surrogate.)"""
+    r = np.asarray(out, dtype=np.float64) * rscale
+    t = np.asarray(CQb, dtype=np.float64) * np.exp(r)
+    D = np.asarray(CSt, dtype=np.float64) + t.sum(axis=1)
+    return (t * np.abs(r)).sum(axis=1) / D
 
 
 # ---------------------------------------------------------------- MLP

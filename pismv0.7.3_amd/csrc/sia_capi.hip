@@ -49,8 +49,11 @@ struct sia_handle {
   sia_surrogate *sg = nullptr;
   unsigned char *sg_status = nullptr;
   int *sg_block_count = nullptr, *sg_list = nullptr, *sg_nlist = nullptr, *sg_counts = nullptr;
+  int *sg_fb_list = nullptr, *sg_nfb = nullptr;  // FP64 fallback columns and their count
   float *sg_F = nullptr, *sg_CQb = nullptr;
   double *sg_CSt = nullptr;
+  float *sg_score = nullptr;     // per-slot fallback scores, written while calibrating
+  bool sg_calibrating = false;
   // profiling events (Profiling::begin/end of the reference)
   struct Rec {
     int ev;
@@ -71,7 +74,8 @@ struct sia_surrogate {
   SgNorm norm{};
   bool have_norm = false;
   double glen_n = 0.0;  // Glen exponent of the training samples (0: none yet)
-  int version = 0;      // bumped when the normalization changes (ensembles copy it)
+  int version = 0;      // bumped when the normalization or tau changes (ensembles copy them)
+  float tau = INFINITY; // fallback score threshold (calibrated; +inf: none)
   SgSample *samples = nullptr;
   long long *d_n = nullptr;
   int64_t cap = 0, n = 0;
@@ -390,7 +394,7 @@ int ensure_sg_buffers(sia_handle *h) {
   if ((rc = dalloc(h, &h->sg_status, ns)) || (rc = dalloc(h, &h->sg_block_count, 3 * (size_t)sg_classify_blocks(h->dp))) ||
       (rc = dalloc(h, &h->sg_list, ns)) || (rc = dalloc(h, &h->sg_nlist, 1)) || (rc = dalloc(h, &h->sg_counts, 3)) ||
       (rc = dalloc(h, &h->sg_F, ns * SG_FSTRIDE)) || (rc = dalloc(h, &h->sg_CQb, ns * SG_NCELL)) ||
-      (rc = dalloc(h, &h->sg_CSt, ns))) {
+      (rc = dalloc(h, &h->sg_CSt, ns)) || (rc = dalloc(h, &h->sg_fb_list, ns)) || (rc = dalloc(h, &h->sg_nfb, 1))) {
     return rc;
   }
   return SIA_OK;
@@ -401,7 +405,8 @@ SgColumnArgs sg_column_args(sia_handle *h, const sia_inputs *in) {
 }
 
 SgListArgs sg_list_args(sia_handle *h) {
-  return SgListArgs{h->sg_status, h->sg_block_count, h->sg_list, h->sg_nlist, h->sg_counts, h->flux, h->Dstag};
+  return SgListArgs{h->sg_status, h->sg_block_count, h->sg_list, h->sg_nlist, h->sg_counts, h->flux, h->Dstag,
+                    h->sg_fb_list, h->sg_nfb};
 }
 
 // flux, D_stag and D_max of a fast update through the surrogate: classify and
@@ -440,14 +445,19 @@ int surrogate_flux(sia_handle *h, const sia_inputs *in, ColumnArgs A) {
   M.status = h->sg_status;
   M.flux = h->flux;
   M.Dstag = h->Dstag;
-  M.dmax_part = h->dmax_part + column_blocks(P);
+  M.dmax_part = h->dmax_part + column_list_blocks(P);
   M.counts = h->sg_counts;
+  M.tau = h->sg_calibrating ? INFINITY : h->sg->tau;
+  M.score = h->sg_calibrating ? h->sg_score : nullptr;
+  M.fb_list = h->sg_fb_list;
+  M.n_fb = h->sg_nfb;
   {
     Stage stage(h, SIA_EV_SG_MLP);
     launch_sg_mlp(P, M, st);
   }
-  A.only = h->sg_status;
-  launch_column(P, A, false, st, sg_mlp_blocks());
+  // the FP64 kernel on the listed fallback columns only
+  A.only = nullptr;
+  launch_column_list(P, A, h->sg_fb_list, h->sg_nfb, st, sg_mlp_blocks());
   HIP_TRY(h, hipGetLastError());
   return SIA_OK;
 }
@@ -528,7 +538,7 @@ struct SgBlobHeader {
   char magic[8];
   int32_t nparam, nin;
   double glen_n;
-  int32_t have_norm, pad;
+  int32_t have_norm, has_tau;  // has_tau: the fallback threshold follows the parameters
 };
 
 }  // namespace
@@ -635,7 +645,8 @@ int sia_destroy(sia_handle *h) {
                   h->flux, h->Dstag, h->I0, h->I1, h->u, h->v, h->w, h->Sigma, h->ks, h->dmax_bits, h->cfl_bits, h->err,
                   h->dt_dev, h->acc_dev, h->partials, h->H_tmp, h->dmax_part, h->sl_hx, h->sl_hy,
                   h->sl_x, h->sl_y, h->sg_status,
-                  h->sg_block_count, h->sg_list, h->sg_nlist, h->sg_counts, h->sg_F, h->sg_CQb, h->sg_CSt};
+                  h->sg_block_count, h->sg_list, h->sg_nlist, h->sg_counts, h->sg_F, h->sg_CQb, h->sg_CSt,
+                  h->sg_score, h->sg_fb_list, h->sg_nfb};
   for (void *b : bufs) dfree(b);
   for (auto &r : h->recs) {
     (void)hipEventDestroy(r.a);
@@ -1124,7 +1135,7 @@ int sia_surrogate_grad(sia_surrogate *sg, int batch, const float *x, const float
 
 int sia_surrogate_save(const sia_surrogate *sg, void *blob, size_t *size) {
   if (!sg || !size) return SIA_ERR_ARGUMENT;
-  const size_t need = sizeof(SgBlobHeader) + sizeof(SgNorm) + sizeof(float) * SG_NPARAM;
+  const size_t need = sizeof(SgBlobHeader) + sizeof(SgNorm) + sizeof(float) * (SG_NPARAM + 1);
   if (!blob) {
     *size = need;
     return SIA_OK;
@@ -1136,12 +1147,14 @@ int sia_surrogate_save(const sia_surrogate *sg, void *blob, size_t *size) {
   hdr.nin = SG_NIN;
   hdr.glen_n = sg->glen_n;
   hdr.have_norm = sg->have_norm ? 1 : 0;
+  hdr.has_tau = 1;
   char *b = (char *)blob;
   std::memcpy(b, &hdr, sizeof(hdr));
   std::memcpy(b + sizeof(hdr), &sg->norm, sizeof(SgNorm));
   if (hipMemcpy(b + sizeof(hdr) + sizeof(SgNorm), sg->master, sizeof(float) * SG_NPARAM, hipMemcpyDeviceToHost) !=
       hipSuccess)
     return SIA_ERR_HIP;
+  std::memcpy(b + sizeof(hdr) + sizeof(SgNorm) + sizeof(float) * SG_NPARAM, &sg->tau, sizeof(float));
   *size = need;
   return SIA_OK;
 }
@@ -1158,6 +1171,10 @@ int sia_surrogate_load(sia_surrogate *sg, const void *blob, size_t size) {
   const char *b = (const char *)blob;
   std::memcpy(&sg->norm, b + sizeof(hdr), sizeof(SgNorm));
   sg->have_norm = hdr.have_norm != 0;
+  sg->tau = INFINITY;
+  if (hdr.has_tau && size >= need + sizeof(float)) {
+    std::memcpy(&sg->tau, b + sizeof(hdr) + sizeof(SgNorm) + sizeof(float) * SG_NPARAM, sizeof(float));
+  }
   sg->version++;
   sg->glen_n = hdr.glen_n;
   return sia_surrogate_set_params(sg, (const float *)(b + sizeof(hdr) + sizeof(SgNorm)), SG_NPARAM);
@@ -1188,6 +1205,46 @@ int sia_use_surrogate(sia_handle *h, sia_surrogate *sg) {
   int rc;
   if ((rc = ensure_sg_buffers(h))) return rc;
   h->sg = sg;
+  return SIA_OK;
+}
+
+int sia_surrogate_set_fallback_score(sia_surrogate *sg, double tau) {
+  if (!sg || std::isnan(tau) || tau < 0.0) return SIA_ERR_ARGUMENT;
+  sg->tau = (float)tau;
+  sg->version++;
+  return SIA_OK;
+}
+
+// a surrogate update of `in` on h with every in-range column kept and its
+// fallback score recorded; tau = the (1 - fraction) quantile of those scores
+int sia_surrogate_calibrate_fallback(sia_surrogate *sg, sia_handle *h, const sia_inputs *in, double fraction,
+                                     double *tau) {
+  if (!h) return SIA_ERR_ARGUMENT;
+  if (!sg || !in || !(fraction > 0.0 && fraction < 1.0)) {
+    return set_err(h, SIA_ERR_ARGUMENT, "sia_surrogate_calibrate_fallback: bad argument");
+  }
+  sia_surrogate *saved = h->sg;
+  int rc = sia_use_surrogate(h, sg);
+  if (rc == SIA_OK) rc = dalloc(h, &h->sg_score, 2 * (size_t)h->grid.Mx * h->grid.My);
+  if (rc == SIA_OK) {
+    h->sg_calibrating = true;
+    rc = update_impl(h, in, 1, true);
+    h->sg_calibrating = false;
+  }
+  h->sg = saved;
+  if (rc) return rc;
+  int nlist = 0;
+  HIP_TRY(h, hipMemcpyAsync(&nlist, h->sg_nlist, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  std::vector<float> sc((size_t)nlist);
+  if (nlist > 0) HIP_TRY(h, hipMemcpy(sc.data(), h->sg_score, sizeof(float) * (size_t)nlist, hipMemcpyDeviceToHost));
+  sc.erase(std::remove_if(sc.begin(), sc.end(), [](float v) { return !(v >= 0.0f); }), sc.end());
+  if (sc.empty()) return set_err(h, SIA_ERR_STATE, "calibration: the surrogate handles no column of these inputs");
+  const size_t k = std::min(sc.size() - 1, (size_t)std::floor((1.0 - fraction) * (double)sc.size()));
+  std::nth_element(sc.begin(), sc.begin() + (long)k, sc.end());
+  sg->tau = sc[k];
+  sg->version++;
+  if (tau) *tau = sg->tau;
   return SIA_OK;
 }
 
@@ -1537,6 +1594,10 @@ int sia_ensemble_create(sia_handle *const *members, const sia_geometry *geometry
       S.M.flux = h->flux;
       S.M.Dstag = h->Dstag;
       S.M.counts = h->sg_counts;
+      S.M.tau = g.sg->tau;
+      S.M.score = nullptr;
+      S.M.fb_list = h->sg_fb_list;
+      S.M.n_fb = h->sg_nfb;
       g.table.push_back(S);
     }
     if ((err = hipMalloc((void **)&g.d_table, sizeof(EnsSgMember) * g.table.size())) != hipSuccess) {
@@ -1607,7 +1668,10 @@ int sia_ensemble_step(sia_ensemble *e, int n_steps, int use_graph) {
   bool renorm = false;
   for (auto &g : e->groups) {
     if (g.version != g.sg->version) {
-      for (auto &S : g.table) S.M.norm = g.sg->norm;
+      for (auto &S : g.table) {
+        S.M.norm = g.sg->norm;
+        S.M.tau = g.sg->tau;
+      }
       g.version = g.sg->version;
       renorm = true;
     }

@@ -397,10 +397,12 @@ __global__ void k_theta_thk(DevParams P, Fields2D F, const double *__restrict__ 
 // and ks leave as one coalesced store per wave. D_max is reduced per block
 // into dmax_part[block]; a one-block kernel takes the max of the partials.
 // block bid of the column grid with COLS_PER_WAVE = CPW; zs: Mz doubles of
-// LDS, wmax: WAVES_PER_BLOCK doubles of LDS
+// LDS, wmax: WAVES_PER_BLOCK doubles of LDS. With a list, the block's columns
+// are list entries bid * COLS_PER_BLOCK .. (n_list of them in all) instead of
+// staggered columns. Returns the block's D_max in thread 0.
 template <int LAW, bool FULL, int CPW>
-__device__ __forceinline__ void column_block(const DevParams &P, const ColumnArgs &A, int bid, double *zs,
-                                             double *wmax) {
+__device__ __forceinline__ double column_block(const DevParams &P, const ColumnArgs &A, int bid, double *zs,
+                                               double *wmax, const int *list = nullptr, int n_list = 0) {
   constexpr int COLS_PER_WAVE = CPW;
   constexpr int COLS_PER_BLOCK = WAVES_PER_BLOCK * COLS_PER_WAVE;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -410,15 +412,13 @@ __device__ __forceinline__ void column_block(const DevParams &P, const ColumnArg
     // a block none of whose columns is selected leaves at once
     const int sb = bid * COLS_PER_BLOCK + (int)threadIdx.x;
     const bool mine = (int)threadIdx.x < COLS_PER_BLOCK && sb < nstag && A.only[sb] == ONLY_MARK;
-    if (!__syncthreads_or(mine)) {
-      if (threadIdx.x == 0) A.dmax_part[bid] = 0.0;
-      return;
-    }
+    if (!__syncthreads_or(mine)) return 0.0;
   }
   for (int k = threadIdx.x; k < Mz; k += blockDim.x) zs[k] = A.z[k];
   __syncthreads();
   const int s0 = (bid * WAVES_PER_BLOCK + wv) * COLS_PER_WAVE;
-  const int s = s0 + lane;
+  // list mode: lane c < COLS_PER_WAVE takes list entry s0 + c (nstag: none)
+  const int s = list ? ((lane < COLS_PER_WAVE && s0 + lane < n_list) ? list[s0 + lane] : nstag) : s0 + lane;
 
   // lane c < COLS_PER_WAVE: column s0 + c. c_ks: -4 not selected (A.only),
   // -3 no column, -2 zero thickness (delta = 0, SIAFD.cc:635-641), -1
@@ -481,7 +481,7 @@ __device__ __forceinline__ void column_block(const DevParams &P, const ColumnArg
     const double Ea0 = nEa, Eb0 = nEb, Aa0 = nAa, Ab0 = nAb;
     if (c + 1 < COLS_PER_WAVE) prefetch(c + 1);
     const long qa = lane_value(c_qa, c), qb = lane_value(c_qb, c);
-    const int s_c = s0 + c;
+    const int s_c = list ? lane_value(s, c) : s0 + c;
     double *Icol = FULL ? ((s_c & 1) ? A.I1 : A.I0) + (long)(s_c >> 1) * Mz : nullptr;
     if (ks < 0) {
       // I = 0 at every level; above ks readers use I[ks] with ks stored as 0
@@ -548,11 +548,12 @@ __device__ __forceinline__ void column_block(const DevParams &P, const ColumnArg
   dmax = wave_max(dmax);  // D >= 0: +0.0 is the identity (std::max, SIAFD.cc:699)
   if (lane == 0) wmax[wv] = dmax;
   __syncthreads();
+  double m = 0.0;
   if (threadIdx.x == 0) {
-    double m = wmax[0];
+    m = wmax[0];
     for (int w = 1; w < WAVES_PER_BLOCK; w++) m = (m < wmax[w]) ? wmax[w] : m;
-    A.dmax_part[bid] = m;
   }
+  return m;
 }
 
 constexpr int COLS_PER_WAVE = 16;
@@ -562,7 +563,25 @@ template <int LAW, bool FULL>
 __global__ __launch_bounds__(256) void k_column(DevParams P, ColumnArgs A) {
   extern __shared__ double zs[];  // Mz levels, staged once per block
   __shared__ double wmax[WAVES_PER_BLOCK];
-  column_block<LAW, FULL, COLS_PER_WAVE>(P, A, blockIdx.x, zs, wmax);
+  const double m = column_block<LAW, FULL, COLS_PER_WAVE>(P, A, blockIdx.x, zs, wmax);
+  if (threadIdx.x == 0) A.dmax_part[blockIdx.x] = m;
+}
+
+// fast-update column pass over a list of staggered columns (the surrogate's
+// FP64 fallback): gridDim.x blocks walk the list's chunks of COLS_PER_BLOCK;
+// each block's D_max goes to dmax_part[blockIdx.x]
+template <int LAW>
+__global__ __launch_bounds__(256) void k_column_list(DevParams P, ColumnArgs A, const int *__restrict__ list,
+                                                     const int *__restrict__ n_list) {
+  extern __shared__ double zs[];
+  __shared__ double wmax[WAVES_PER_BLOCK];
+  const int n = *n_list;
+  double m = 0.0;
+  for (int vb = blockIdx.x; vb * COLS_PER_BLOCK < n; vb += gridDim.x) {
+    const double mb = column_block<LAW, false, COLS_PER_WAVE>(P, A, vb, zs, wmax, list, n);
+    m = (m < mb) ? mb : m;
+  }
+  if (threadIdx.x == 0) A.dmax_part[blockIdx.x] = m;
 }
 
 // max of the per-block partials (one block); D >= 0 so +0.0 is the identity
@@ -1210,6 +1229,28 @@ static void launch_column_law(const DevParams &P, const ColumnArgs &A, bool full
   hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, st, A.dmax_part, nb + extra, A.dmax_bits);
 }
 
+template <int LAW>
+static void launch_column_list_law(const DevParams &P, const ColumnArgs &A, const int *list, const int *n_list,
+                                   hipStream_t st, int extra) {
+  const int nb = column_list_blocks(P);
+  hipLaunchKernelGGL((k_column_list<LAW>), dim3(nb), dim3(256), sizeof(double) * (size_t)P.Mz, st, P, A, list,
+                     n_list);
+  hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, st, A.dmax_part, nb + extra, A.dmax_bits);
+}
+
+void launch_column_list(const DevParams &P, const ColumnArgs &A, const int *list, const int *n_list,
+                        hipStream_t st, int extra) {
+  switch (P.law) {
+  case LAW_ISOGLEN: launch_column_list_law<LAW_ISOGLEN>(P, A, list, n_list, st, extra); break;
+  case LAW_PB: launch_column_list_law<LAW_PB>(P, A, list, n_list, st, extra); break;
+  case LAW_GPBLD: launch_column_list_law<LAW_GPBLD>(P, A, list, n_list, st, extra); break;
+  case LAW_HOOKE: launch_column_list_law<LAW_HOOKE>(P, A, list, n_list, st, extra); break;
+  case LAW_ARR: launch_column_list_law<LAW_ARR>(P, A, list, n_list, st, extra); break;
+  case LAW_ARRWARM: launch_column_list_law<LAW_ARRWARM>(P, A, list, n_list, st, extra); break;
+  default: launch_column_list_law<LAW_GK>(P, A, list, n_list, st, extra); break;
+  }
+}
+
 void launch_column(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st, int extra) {
   switch (P.law) {
   case LAW_ISOGLEN: launch_column_law<LAW_ISOGLEN>(P, A, full, st, extra); break;
@@ -1225,6 +1266,8 @@ void launch_column(const DevParams &P, const ColumnArgs &A, bool full, hipStream
 static inline unsigned blocks_for(long n, int per_block) { return (unsigned)((n + per_block - 1) / per_block); }
 
 int column_blocks(const DevParams &P) { return (int)blocks_for(2L * P.Mx * P.My, COLS_PER_BLOCK); }
+
+int column_list_blocks(const DevParams &P) { return column_blocks(P) < 1024 ? column_blocks(P) : 1024; }
 
 void launch_reset(unsigned long long *dmax_bits, int *err, unsigned long long *cfl_bits, double max_dt,
                   hipStream_t st) {
@@ -1374,7 +1417,8 @@ __global__ __launch_bounds__(256) void k_ens_column(const EnsMember *__restrict_
   __shared__ double wmax[WAVES_PER_BLOCK];
   const int m = ens_member(start, nm, blockIdx.x);
   const ColumnArgs A = global_view(T[m].A);
-  column_block<LAW, FULL, CPW>(T[m].P, A, blockIdx.x - start[m], zs, wmax);
+  const double mb = column_block<LAW, FULL, CPW>(T[m].P, A, blockIdx.x - start[m], zs, wmax);
+  if (threadIdx.x == 0) A.dmax_part[blockIdx.x - start[m]] = mb;
 }
 
 // per member (one block each): D_max from the column partials, then dt and t += dt

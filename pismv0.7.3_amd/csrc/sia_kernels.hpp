@@ -76,6 +76,12 @@ int column_blocks(const DevParams &P);
 // extra_partials: further per-block D_max partials that other kernels wrote
 // after A.dmax_part[column_blocks(P) - 1]; all are reduced into A.dmax_bits
 void launch_column(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st, int extra_partials = 0);
+// fast-update FP64 pass over the n_list (device count) staggered columns of
+// list: column_list_blocks(P) <= column_blocks(P) per-block D_max partials,
+// then extra_partials more, reduced into A.dmax_bits
+int column_list_blocks(const DevParams &P);
+void launch_column_list(const DevParams &P, const ColumnArgs &A, const int *list, const int *n_list,
+                        hipStream_t st, int extra_partials = 0);
 void launch_velocity_uvw(const DevParams &P, const VelArgs &A, hipStream_t st);
 void launch_sia_sliding(const DevParams &P, const SlideArgs &A, hipStream_t st);
 void launch_surface_values(const DevParams &P, const double *z, const double *f, const double *H, double *out,

@@ -166,9 +166,11 @@ __global__ __launch_bounds__(SG_CLS_BLOCK) void k_sg_classify(DevParams P, SgCol
   sg_classify_block(P, C, L, blockIdx.x);
 }
 
-// exclusive scan of the per-block surrogate counts (one block of 1024); the
-// offsets overwrite block_count[3 b + 1]; totals go to counts[0..2], n_list
-__device__ __forceinline__ void sg_scan_block(int nb, const SgListArgs &L, long long *part, long long *tot) {
+// exclusive scans of the per-block surrogate and fallback counts (one block of
+// 1024); the offsets overwrite block_count[3 b + 1] and [3 b + 2]; totals go
+// to counts[0..2], n_list and n_fb. part, partf: 1024 entries of LDS each.
+__device__ __forceinline__ void sg_scan_block(int nb, const SgListArgs &L, long long *part, long long *partf,
+                                              long long *tot) {
   const int t = threadIdx.x;
   const int per = (nb + 1023) / 1024;
   const int b0 = t * per, b1 = min(nb, b0 + per);
@@ -179,57 +181,70 @@ __device__ __forceinline__ void sg_scan_block(int nb, const SgListArgs &L, long 
     f += L.block_count[3 * b + 2];
   }
   part[t] = s;
+  partf[t] = f;
   if (t < 3) tot[t] = 0;
   __syncthreads();
   atomicAdd((unsigned long long *)&tot[0], (unsigned long long)z);
-  atomicAdd((unsigned long long *)&tot[2], (unsigned long long)f);
-  // Hillis-Steele over the 1024 partial sums
+  // Hillis-Steele over the 1024 partial sums of both counts
   for (int off = 1; off < 1024; off <<= 1) {
     const long long v = (t >= off) ? part[t - off] : 0;
+    const long long vf = (t >= off) ? partf[t - off] : 0;
     __syncthreads();
     part[t] += v;
+    partf[t] += vf;
     __syncthreads();
   }
-  long long run = part[t] - s;  // exclusive start of this thread's chunk
+  long long run = part[t] - s, runf = partf[t] - f;  // exclusive starts of this thread's chunk
   for (int b = b0; b < b1; b++) {
-    const int c = L.block_count[3 * b + 1];
+    const int c = L.block_count[3 * b + 1], cf = L.block_count[3 * b + 2];
     L.block_count[3 * b + 1] = (int)run;
+    L.block_count[3 * b + 2] = (int)runf;
     run += c;
+    runf += cf;
   }
   if (t == 1023) {
     *L.n_list = (int)part[1023];
     L.counts[1] = (int)part[1023];
+    *L.n_fb = (int)partf[1023];
+    L.counts[2] = (int)partf[1023];
   }
   __syncthreads();
-  if (t == 0) {
-    L.counts[0] = (int)tot[0];
-    L.counts[2] = (int)tot[2];
-  }
+  if (t == 0) L.counts[0] = (int)tot[0];
 }
 
 __global__ __launch_bounds__(1024) void k_sg_scan(int nb, SgListArgs L) {
-  __shared__ long long part[1024];
+  __shared__ long long part[1024], partf[1024];
   __shared__ long long tot[3];
-  sg_scan_block(nb, L, part, tot);
+  sg_scan_block(nb, L, part, partf, tot);
 }
 
-// ordered compaction of the SG_SURR columns
+// ordered compaction of the SG_SURR columns into list and of the classified
+// SG_FALLBACK columns into fb_list; wsum: 2 * SG_CLS_BLOCK / 64 ints of LDS
 __device__ __forceinline__ void sg_scatter_block(const DevParams &P, const SgListArgs &L, int bid, int *wsum) {
   const int s = bid * SG_CLS_BLOCK + threadIdx.x;
   const int nstag = 2 * P.Mx * P.My;
-  const bool flag = (s < nstag) && L.status[s] == SG_SURR;
+  const unsigned char st = (s < nstag) ? L.status[s] : (unsigned char)SG_ZERO;
+  const bool flag = st == SG_SURR, fflag = st == SG_FALLBACK;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const unsigned long long m = __ballot(flag);
-  const int rank = __popcll(m & ((1ull << lane) - 1ull));
-  if (lane == 0) wsum[wv] = __popcll(m);
+  constexpr int NW = SG_CLS_BLOCK / 64;
+  const unsigned long long m = __ballot(flag), mf = __ballot(fflag);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  if (lane == 0) {
+    wsum[wv] = __popcll(m);
+    wsum[NW + wv] = __popcll(mf);
+  }
   __syncthreads();
-  int base = L.block_count[3 * bid + 1];
-  for (int w = 0; w < wv; w++) base += wsum[w];
-  if (flag) L.list[base + rank] = s;
+  int base = L.block_count[3 * bid + 1], basef = L.block_count[3 * bid + 2];
+  for (int w = 0; w < wv; w++) {
+    base += wsum[w];
+    basef += wsum[NW + w];
+  }
+  if (flag) L.list[base + __popcll(m & below)] = s;
+  if (fflag) L.fb_list[basef + __popcll(mf & below)] = s;
 }
 
 __global__ __launch_bounds__(SG_CLS_BLOCK) void k_sg_scatter(DevParams P, SgListArgs L) {
-  __shared__ int wsum[SG_CLS_BLOCK / 64];
+  __shared__ int wsum[2 * SG_CLS_BLOCK / 64];
   sg_scatter_block(P, L, blockIdx.x, wsum);
 }
 
@@ -785,27 +800,40 @@ __device__ __forceinline__ void sg_mlp_block(const DevParams &P, const SgMlpArgs
       o0 = mfma(load_frag(wfrag + frag_index(4, 0, cc) * 512, lane), h[cc], o0);
       o1 = mfma(load_frag(wfrag + frag_index(4, 1, cc) * 512, lane), h[cc], o1);
     }
-    // D = C S_t + sum_m C Qb_m exp(r_m); this lane holds cells 4g..4g+3, 16+4g..16+4g+3
+    // D = C S_t + sum_m t_m with t_m = C Qb_m exp(r_m); this lane holds cells
+    // 4g..4g+3, 16+4g..16+4g+3. The fallback score sum_m t_m |r_m| / D is the
+    // predicted-share-weighted size of the log-correction.
     const float *bo = bias + 4 * SG_WIDTH;
-    float part = 0.0f;
+    float part = 0.0f, spart = 0.0f;
     if (live) {
       const float4 q0 = *reinterpret_cast<const float4 *>(A.CQb + (long)slot * SG_NCELL + 4 * g);
       const float4 q1 = *reinterpret_cast<const float4 *>(A.CQb + (long)slot * SG_NCELL + 16 + 4 * g);
       const float rs = N.rscale;
-      part = q0.x * __expf((o0[0] + bo[4 * g + 0]) * rs) + q0.y * __expf((o0[1] + bo[4 * g + 1]) * rs) +
-             q0.z * __expf((o0[2] + bo[4 * g + 2]) * rs) + q0.w * __expf((o0[3] + bo[4 * g + 3]) * rs) +
-             q1.x * __expf((o1[0] + bo[16 + 4 * g + 0]) * rs) + q1.y * __expf((o1[1] + bo[16 + 4 * g + 1]) * rs) +
-             q1.z * __expf((o1[2] + bo[16 + 4 * g + 2]) * rs) + q1.w * __expf((o1[3] + bo[16 + 4 * g + 3]) * rs);
+      const float q[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+      const float o[8] = {o0[0], o0[1], o0[2], o0[3], o1[0], o1[1], o1[2], o1[3]};
+#pragma unroll
+      for (int c = 0; c < 8; c++) {
+        const float r = (o[c] + bo[(c < 4 ? 0 : 16) + 4 * g + (c & 3)]) * rs;
+        const float t = q[c] * __expf(r);
+        part = (c == 0) ? t : part + t;
+        spart = __builtin_fmaf(t, __builtin_fabsf(r), spart);
+      }
     }
     part += __shfl_xor(part, 16, 64);
     part += __shfl_xor(part, 32, 64);
+    spart += __shfl_xor(spart, 16, 64);
+    spart += __shfl_xor(spart, 32, 64);
     if (g == 0 && live) {
       const int s = A.list[slot];
-      if (ood) {
+      const double Dp = A.CSt[slot] + (double)part;
+      const float score = ood ? -1.0f : (Dp > 0.0 ? (float)((double)spart / Dp) : 0.0f);
+      if (A.score) A.score[slot] = score;
+      if (ood || score > A.tau) {
         A.status[s] = SG_FALLBACK;
+        A.fb_list[atomicAdd(A.n_fb, 1)] = s;
         nfall++;
       } else {
-        const double D = A.CSt[slot] + (double)part;
+        const double D = Dp;
         const int q = s >> 1, oo = s & 1;
         const int i = q / P.My, j = q - i * P.My;
         const double slope = (oo == 0) ? A.hx[s] : A.hy[s];
@@ -871,17 +899,17 @@ __global__ __launch_bounds__(SG_CLS_BLOCK) void k_ens_sg_classify(const EnsSgMem
 // one block per member
 __global__ __launch_bounds__(1024) void k_ens_sg_scan(const EnsSgMember *__restrict__ T,
                                                       const int *__restrict__ start, int nm) {
-  __shared__ long long part[1024];
+  __shared__ long long part[1024], partf[1024];
   __shared__ long long tot[3];
   const int m = blockIdx.x;
   if (m >= nm) return;
   const SgListArgs L = global_view(T[m].L);
-  sg_scan_block(start[m + 1] - start[m], L, part, tot);
+  sg_scan_block(start[m + 1] - start[m], L, part, partf, tot);
 }
 
 __global__ __launch_bounds__(SG_CLS_BLOCK) void k_ens_sg_scatter(const EnsSgMember *__restrict__ T,
                                                                   const int *__restrict__ start, int nm) {
-  __shared__ int wsum[SG_CLS_BLOCK / 64];
+  __shared__ int wsum[2 * SG_CLS_BLOCK / 64];
   const int m = ens_member(start, nm, blockIdx.x);
   const SgListArgs L = global_view(T[m].L);
   sg_scatter_block(T[m].P, L, blockIdx.x - start[m], wsum);

@@ -79,6 +79,8 @@ struct SgListArgs {
   int *n_list;            // device scalar
   int *counts;            // device [3]: zero, surrogate, fallback (fallback grows in the MLP pass)
   double *flux, *Dstag;   // written for SG_ZERO columns
+  int *fb_list;           // [2 Mx My] SG_FALLBACK columns: the classified ones in order, then the MLP's
+  int *n_fb;              // device count
 };
 
 int sg_classify_blocks(const DevParams &P);
@@ -114,6 +116,9 @@ struct SgMlpArgs {
   double *flux, *Dstag;
   double *dmax_part;           // per block
   int *counts;
+  float tau;                   // fallback score threshold (+inf: none)
+  float *score;                // per slot fallback score (-1 out of range), or nullptr
+  int *fb_list, *n_fb;         // fallback columns are appended here
 };
 int sg_mlp_blocks();
 void launch_sg_mlp(const DevParams &P, const SgMlpArgs &A, hipStream_t st);
@@ -139,6 +144,8 @@ __device__ __forceinline__ SgListArgs global_view(SgListArgs a) {
   a.counts = glb(a.counts);
   a.flux = glb(a.flux);
   a.Dstag = glb(a.Dstag);
+  a.fb_list = glb(a.fb_list);
+  a.n_fb = glb(a.n_fb);
   return a;
 }
 
@@ -170,6 +177,9 @@ __device__ __forceinline__ SgMlpArgs global_view(const SgMlpArgs &a) {
   g.Dstag = glb(a.Dstag);
   g.dmax_part = glb(a.dmax_part);
   g.counts = glb(a.counts);
+  g.score = glb(a.score);
+  g.fb_list = glb(a.fb_list);
+  g.n_fb = glb(a.n_fb);
   return g;
 }
 

@@ -83,6 +83,9 @@ SIGNATURES = {
     "sia_surrogate_samples": (C.c_int, [_vp, _PP, C.POINTER(C.c_int64)]),
     "sia_use_surrogate": (C.c_int, [_vp, _vp]),
     "sia_surrogate_counts": (C.c_int, [_vp, C.POINTER(C.c_int64)]),
+    "sia_surrogate_calibrate_fallback": (C.c_int, [_vp, _vp, C.POINTER(T.SiaInputs), C.c_double,
+                                                   C.POINTER(C.c_double)]),
+    "sia_surrogate_set_fallback_score": (C.c_int, [_vp, C.c_double]),
     # batched ensembles (include/sia_ensemble.h)
     "sia_ensemble_create": (C.c_int, [C.POINTER(_vp), C.POINTER(T.SiaGeometry), C.POINTER(_vp),
                                       C.POINTER(C.c_double), C.c_int, _PP]),
@@ -380,13 +383,32 @@ class Surrogate:
         except Exception:
             pass
 
-    def collect(self, sia, H, b, h, mask, E, stride=1):
-        inp = T.SiaInputs(ptr(H), ptr(b), ptr(h), ptr(mask), ptr(E), None, None, None, None, 0)
+    def collect(self, sia, H, b, h, mask, E, stride=1, bed_state_counter=0):
+        """Appends the samples of the FP64 fast update of these fields on handle
+        sia. As in update(), the bed is smoothed again only when
+        bed_state_counter exceeds the last one the handle saw: a handle reused
+        for another bed needs a larger counter."""
+        inp = T.SiaInputs(ptr(H), ptr(b), ptr(h), ptr(mask), ptr(E), None, None, None, None, bed_state_counter)
         n = C.c_int64()
         rc = lib().sia_surrogate_collect(self.h, sia.h, C.byref(inp), stride, C.byref(n))
         if rc != 0:
             raise SIAError(rc, lib().sia_last_error(sia.h).decode())
         return n.value
+
+    def calibrate_fallback(self, sia, H, b, h, mask, E, fraction, bed_state_counter=0):
+        """Sets the score fallback threshold tau from one surrogate update of
+        these fields (training fields, not the ones to be judged): about
+        `fraction` of the in-range columns then fall back to FP64. Returns tau."""
+        inp = T.SiaInputs(ptr(H), ptr(b), ptr(h), ptr(mask), ptr(E), None, None, None, None, bed_state_counter)
+        tau = C.c_double()
+        rc = lib().sia_surrogate_calibrate_fallback(self.h, sia.h, C.byref(inp), float(fraction), C.byref(tau))
+        if rc != 0:
+            raise SIAError(rc, lib().sia_last_error(sia.h).decode())
+        return tau.value
+
+    def set_fallback_score(self, tau):
+        """Score threshold of the FP64 fallback (float("inf"): none)."""
+        self._check(lib().sia_surrogate_set_fallback_score(self.h, float(tau)), "set_fallback_score")
 
     def fit_normalization(self):
         self._check(lib().sia_surrogate_fit_normalization(self.h), "fit_normalization")

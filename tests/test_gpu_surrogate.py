@@ -15,7 +15,12 @@ Tolerances (stated per check):
   relative error of D below 1e-3, and the network must beat the physics
   baseline alone (the ablation);
 * out-of-distribution columns fall back to the FP64 kernel: their D is
-  bitwise the FP64 one.
+  bitwise the FP64 one;
+* score fallback: calibrated to 5 % on a training window, 4-7 % of the
+  in-range columns there fall back; on the held-out window a column whose
+  oracle score is 5 % or more above tau is bitwise FP64, one 5 % or more below
+  keeps the surrogate's D (1e-4 to the oracle forward); tau survives the
+  model blob.
 """
 import struct
 
@@ -238,3 +243,46 @@ def test_surrogate_config_errors(S, trained):
     sia.close()
     sia2.close()
     fresh.close()
+
+
+def test_score_fallback(S, trained):
+    say("[gpu] surrogate score fallback: calibration and the columns it sends to FP64")
+    sg, _ = trained
+    try:
+        wt, devt = window_fields(S, WINDOWS_TRAIN[0])
+        cal = S.SIAFD(wt.grid, wt.params)
+        tau = sg.calibrate_fallback(cal, devt["H"], devt["b"], devt["h"], devt["mask"], devt["E"], 0.05)
+        cal.close()
+        assert 0.0 < tau < float("inf")
+        zero, nsur, nfall = surrogate_fast(S, sg, wt, devt)["counts"]
+        say("  tau %.3e; calibration window: %d surrogate, %d FP64" % (tau, nsur, nfall))
+        assert 0.04 < nfall / (nsur + nfall) < 0.07
+        w, dev = window_fields(S, WINDOW_TEST)
+        ref = plain_fast(S, w, dev)
+        sur = surrogate_fast(S, sg, w, dev)
+        norm, wts = parse_norm(sg.save())
+        st = SO.staggered_inputs(w.grid, w.fields, ref["thks"], ref["theta"], ref["hx"], ref["hy"])
+        sel = np.nonzero(eligible(w, st))[0]
+        z = np.array([w.grid.z[k] for k in range(w.grid.Mz)])
+        c = SO.columns(w.params, z, st["thk"][sel], st["alpha"][sel], st["theta"][sel], st["E"][sel], targets=False)
+        F = c["F"].astype(np.float32)
+        inside = np.all((F >= norm["lo"]) & (F <= norm["hi"]), axis=1)
+        X = np.zeros((len(sel), SO.KIN), dtype=np.float32)
+        X[:, :34] = (F - norm["mean"]) * norm["inv_std"]
+        out, _ = SO.forward(wts[:SO.NPARAM].astype(np.float64), X, rounding=True)
+        score = SO.fallback_score(c["CQb"], c["CSt"], out, norm["rscale"])
+        Ds, Dp = sur["D"][sel], ref["D"][sel]
+        hi = inside & (score > 1.05 * tau)
+        lo = inside & (score < 0.95 * tau)
+        say("  held-out window: %d columns above tau, %d below" % (hi.sum(), lo.sum()))
+        assert hi.sum() > 0 and lo.sum() > 0
+        assert np.array_equal(Ds[hi], Dp[hi])
+        Dor = c["CSt"] + (c["CQb"].astype(np.float64) * np.exp(out * norm["rscale"])).sum(axis=1)
+        assert (np.abs(Ds[lo] - Dor[lo]) / Dor[lo]).max() < 1e-4
+        # the blob carries tau
+        other = S.Surrogate(seed=2)
+        other.load(sg.save())
+        assert np.array_equal(surrogate_fast(S, other, w, dev)["D"], sur["D"])
+        other.close()
+    finally:
+        sg.set_fallback_score(float("inf"))

@@ -5,7 +5,10 @@
   (this pins the level weights W_k, including the top partial cell);
 * the vectorised GPBLD softness and enthalpy converter equal the C oracle's;
 * the MLP backward equals central finite differences of the forward (float64);
-* BF16 rounding is round-to-nearest-even.
+* BF16 rounding is round-to-nearest-even;
+* the fallback score's closed forms: zero without a correction, the one-cell
+  value, invariance under a common scale of C Qb and C S_t, and the weighting
+  by predicted shares (a cell with no cold levels adds nothing).
 """
 import numpy as np
 import pytest
@@ -100,3 +103,29 @@ def test_bf16_rounding():
     assert r[3] == -2.5
     assert r[4] == 1.0
     assert r[5] == pytest.approx(3.0e-39, rel=1e-2)
+
+
+def test_fallback_score_closed_forms():
+    rng = np.random.default_rng(5)
+    CQb = rng.uniform(0.0, 2.0, (6, 32))
+    CSt = rng.uniform(0.0, 1.0, 6)
+    # no correction: score 0
+    assert np.all(SO.fallback_score(CQb, CSt, np.zeros((6, 32)), 1.0) == 0.0)
+    # one cell with a correction x: q e^x |x| / (s + q e^x + the other cells)
+    out = np.zeros((6, 32))
+    out[:, 7] = 0.3
+    x = 0.3 * 2.5
+    expect = CQb[:, 7] * np.exp(x) * x / (CSt + CQb.sum(axis=1) - CQb[:, 7] + CQb[:, 7] * np.exp(x))
+    np.testing.assert_allclose(SO.fallback_score(CQb, CSt, out, 2.5), expect, rtol=1e-14)
+    # homogeneous of degree 0 in (C Qb, C S_t); |r| makes the sign of r irrelevant to the weight
+    out = rng.normal(0.0, 0.2, (6, 32))
+    s1 = SO.fallback_score(CQb, CSt, out, 1.5)
+    np.testing.assert_allclose(SO.fallback_score(7.0 * CQb, 7.0 * CSt, out, 1.5), s1, rtol=1e-13)
+    assert np.all(s1 > 0)
+    # a cell without cold levels (C Qb = 0) contributes nothing, whatever its output
+    CQb0 = CQb.copy()
+    CQb0[:, 3] = 0.0
+    out2 = out.copy()
+    out2[:, 3] = 50.0
+    np.testing.assert_allclose(SO.fallback_score(CQb0, CSt, out2, 1.5), SO.fallback_score(CQb0, CSt, out, 1.5),
+                               rtol=1e-14)
