@@ -1280,14 +1280,17 @@ struct sia_ensemble {
     std::vector<int> idx;               // ensemble member indices
     std::vector<EnsSgMember> table;
     EnsSgMember *d_table = nullptr;
-    std::vector<int> start[3];          // block prefixes: classify, features, MLP
-    int *d_start[3] = {nullptr, nullptr, nullptr};
+    std::vector<EnsListMember> ltable;  // the FP64 pass over each member's fallback list
+    EnsListMember *d_ltable = nullptr;
+    std::vector<int> start[4];          // block prefixes: classify, features, MLP, fallback list
+    int *d_start[4] = {nullptr, nullptr, nullptr, nullptr};
     int feat_waves = 4, nm1 = 2, max_mz = 0;
     int version = 0;                    // sg->version the table's normalization copies
   };
   std::vector<SgGroup> groups;
   std::vector<sia_surrogate *> bound;   // each member's surrogate at creation (nullptr: FP64)
   std::vector<int> mlp_part;            // MLP D_max partials of each member
+  std::vector<int> lst_part;            // fallback-list D_max partials of each member
   hipStream_t own = nullptr, user = nullptr;
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
   hipGraph_t graph = nullptr;
@@ -1316,17 +1319,22 @@ int ens_err(sia_ensemble *e, int code, const std::string &msg) {
 // member table for the current tile width, uploaded on the ensemble's stream
 int ens_upload(sia_ensemble *e) {
   const int c = cpw_slot(e->cpw);
+  // member partials: column stage (none for surrogate-bound members), then MLP, then fallback list
   for (int i = 0; i < e->n; i++) {
     const int ncol = e->col_start[c][i + 1] - e->col_start[c][i];
     e->table[i].A.dmax_part = e->d_part + e->part_off[i];
-    e->table[i].n_part = ncol + e->mlp_part[i];
+    e->table[i].n_part = ncol + e->mlp_part[i] + e->lst_part[i];
   }
   for (auto &g : e->groups) {
     for (size_t k = 0; k < g.idx.size(); k++) {
       const int i = g.idx[k];
-      g.table[k].M.dmax_part = e->table[i].A.dmax_part + (e->col_start[c][i + 1] - e->col_start[c][i]);
+      const int ncol = e->col_start[c][i + 1] - e->col_start[c][i];
+      g.table[k].M.dmax_part = e->table[i].A.dmax_part + ncol;
+      g.ltable[k].A.dmax_part = e->table[i].A.dmax_part + ncol + e->mlp_part[i];
     }
     ENS_TRY(e, hipMemcpyAsync(g.d_table, g.table.data(), sizeof(EnsSgMember) * g.table.size(),
+                              hipMemcpyHostToDevice, e->own));
+    ENS_TRY(e, hipMemcpyAsync(g.d_ltable, g.ltable.data(), sizeof(EnsListMember) * g.ltable.size(),
                               hipMemcpyHostToDevice, e->own));
   }
   ENS_TRY(e, hipMemcpyAsync(e->d_table, e->table.data(), sizeof(EnsMember) * e->n, hipMemcpyHostToDevice, e->own));
@@ -1368,6 +1376,7 @@ void ens_issue_step(const sia_ensemble *e, const EnsLaunch &L, hipStream_t st) {
     G.nm1 = g.nm1;
     G.max_mz = g.max_mz;
     launch_ens_sg(G, st);
+    launch_ens_column_list(g.d_ltable, g.d_start[3], (int)g.idx.size(), g.start[3].back(), g.max_mz, e->law, st);
   }
   launch_ens_back(L, st);
 }
@@ -1413,6 +1422,7 @@ int sia_ensemble_create(sia_handle *const *members, const sia_geometry *geometry
   e->part_off.assign(n, 0);
   e->bound.assign(n, nullptr);
   e->mlp_part.assign(n, 0);
+  e->lst_part.assign(n, 0);
   size_t part_total = 0;
   long long pt_total = 0, col_total[3] = {0, 0, 0};
   for (int i = 0; i < n; i++) {
@@ -1454,8 +1464,9 @@ int sia_ensemble_create(sia_handle *const *members, const sia_geometry *geometry
     e->pt_start[i] = (int)pt_total;
     pt_total += ens_point_blocks(h->dp);
     for (int c = 0; c < 3; c++) {
+      // a surrogate-bound member's FP64 columns run in its group's fallback-list pass
       e->col_start[c][i] = (int)col_total[c];
-      col_total[c] += ens_column_blocks(h->dp, ENS_CPW[c]);
+      col_total[c] += h->sg ? 0 : ens_column_blocks(h->dp, ENS_CPW[c]);
     }
   }
   // surrogate groups: per-member block shares of the feature pass and the MLP
@@ -1465,18 +1476,21 @@ int sia_ensemble_create(sia_handle *const *members, const sia_geometry *geometry
     for (int i : g.idx) tot += 2LL * members[i]->grid.Mx * members[i]->grid.My;
     g.feat_waves = sg_features_waves(g.max_mz);
     const int feat_total = 8192 / g.feat_waves;
-    for (int k = 0; k < 3; k++) g.start[k].assign(1, 0);
+    for (int k = 0; k < 4; k++) g.start[k].assign(1, 0);
     for (int i : g.idx) {
       const double share = (double)(2LL * members[i]->grid.Mx * members[i]->grid.My) / (double)tot;
-      const int nb[3] = {sg_classify_blocks(members[i]->dp), std::max(1, (int)std::lround(feat_total * share)),
-                         std::max(1, (int)std::lround(sg_mlp_blocks() * share))};
-      for (int k = 0; k < 3; k++) g.start[k].push_back(g.start[k].back() + nb[k]);
+      const int nb[4] = {sg_classify_blocks(members[i]->dp), std::max(1, (int)std::lround(feat_total * share)),
+                         std::max(1, (int)std::lround(sg_mlp_blocks() * share)),
+                         std::max(1, (int)std::lround(1024 * share))};
+      for (int k = 0; k < 4; k++) g.start[k].push_back(g.start[k].back() + nb[k]);
       e->mlp_part[i] = nb[2];
+      e->lst_part[i] = nb[3];
     }
   }
   for (int i = 0; i < n; i++) {
     e->part_off[i] = part_total;
-    part_total += (size_t)ens_column_blocks(members[i]->dp, 8) + (size_t)e->mlp_part[i];
+    part_total += (size_t)(members[i]->sg ? 0 : ens_column_blocks(members[i]->dp, 8)) + (size_t)e->mlp_part[i] +
+                  (size_t)e->lst_part[i];
   }
   if (pt_total >= 2147483647LL || col_total[0] >= 2147483647LL) {
     delete e;
@@ -1559,7 +1573,6 @@ int sia_ensemble_create(sia_handle *const *members, const sia_geometry *geometry
     M.dt = e->d_dt + i;
     M.err_acc = e->d_err_acc + i;
     M.t_end = t_end ? t_end[i] : 0.0;
-    if (h->sg) M.A.only = h->sg_status;  // the column stage takes the fallback columns only
   }
   for (auto &g : e->groups) {
     for (int i : g.idx) {
@@ -1599,11 +1612,19 @@ int sia_ensemble_create(sia_handle *const *members, const sia_geometry *geometry
       S.M.fb_list = h->sg_fb_list;
       S.M.n_fb = h->sg_nfb;
       g.table.push_back(S);
+      EnsListMember Lm{};
+      Lm.P = e->table[i].P;
+      Lm.A = e->table[i].A;
+      Lm.A.only = nullptr;
+      Lm.list = h->sg_fb_list;
+      Lm.n_list = h->sg_nfb;
+      g.ltable.push_back(Lm);
     }
-    if ((err = hipMalloc((void **)&g.d_table, sizeof(EnsSgMember) * g.table.size())) != hipSuccess) {
+    if ((err = hipMalloc((void **)&g.d_table, sizeof(EnsSgMember) * g.table.size())) != hipSuccess ||
+        (err = hipMalloc((void **)&g.d_ltable, sizeof(EnsListMember) * g.ltable.size())) != hipSuccess) {
       return fail(err, "hipMalloc");
     }
-    for (int k = 0; k < 3; k++) {
+    for (int k = 0; k < 4; k++) {
       const size_t b = sizeof(int) * g.start[k].size();
       if ((err = hipMalloc((void **)&g.d_start[k], b)) != hipSuccess ||
           (err = hipMemcpy(g.d_start[k], g.start[k].data(), b, hipMemcpyHostToDevice)) != hipSuccess) {
@@ -1629,7 +1650,8 @@ int sia_ensemble_destroy(sia_ensemble *e) {
   for (void *b : bufs) dfree(b);
   for (auto &g : e->groups) {
     dfree(g.d_table);
-    for (int k = 0; k < 3; k++) dfree(g.d_start[k]);
+    dfree(g.d_ltable);
+    for (int k = 0; k < 4; k++) dfree(g.d_start[k]);
   }
   if (e->ev_in) (void)hipEventDestroy(e->ev_in);
   if (e->ev_out) (void)hipEventDestroy(e->ev_out);

@@ -1459,6 +1459,38 @@ __global__ __launch_bounds__(256) void k_ens_mask(const EnsMember *__restrict__ 
   mask_point(E.P, glb(E.M.H_new), glb(E.F.b), glb(E.H), glb(E.mask), glb(E.h), glb(E.A.err), q);
 }
 
+template <int LAW>
+__global__ __launch_bounds__(256) void k_ens_column_list(const EnsListMember *__restrict__ T,
+                                                         const int *__restrict__ start, int nm) {
+  extern __shared__ double zs[];
+  __shared__ double wmax[WAVES_PER_BLOCK];
+  const int m = ens_member(start, nm, blockIdx.x);
+  const ColumnArgs A = global_view(T[m].A);
+  const int *list = glb(T[m].list);
+  const int n = *glb(T[m].n_list);
+  const int bid = blockIdx.x - start[m], nblk = start[m + 1] - start[m];
+  double mx = 0.0;
+  for (int vb = bid; vb * COLS_PER_BLOCK < n; vb += nblk) {
+    const double mb = column_block<LAW, false, COLS_PER_WAVE>(T[m].P, A, vb, zs, wmax, list, n);
+    mx = (mx < mb) ? mb : mx;
+  }
+  if (threadIdx.x == 0) A.dmax_part[bid] = mx;
+}
+
+void launch_ens_column_list(const EnsListMember *T, const int *start, int nm, int nblocks, int max_mz, int law,
+                            hipStream_t st) {
+  const size_t lds = sizeof(double) * (size_t)max_mz;
+  switch (law) {
+  case LAW_ISOGLEN: hipLaunchKernelGGL(k_ens_column_list<LAW_ISOGLEN>, dim3(nblocks), dim3(256), lds, st, T, start, nm); break;
+  case LAW_PB: hipLaunchKernelGGL(k_ens_column_list<LAW_PB>, dim3(nblocks), dim3(256), lds, st, T, start, nm); break;
+  case LAW_GPBLD: hipLaunchKernelGGL(k_ens_column_list<LAW_GPBLD>, dim3(nblocks), dim3(256), lds, st, T, start, nm); break;
+  case LAW_HOOKE: hipLaunchKernelGGL(k_ens_column_list<LAW_HOOKE>, dim3(nblocks), dim3(256), lds, st, T, start, nm); break;
+  case LAW_ARR: hipLaunchKernelGGL(k_ens_column_list<LAW_ARR>, dim3(nblocks), dim3(256), lds, st, T, start, nm); break;
+  case LAW_ARRWARM: hipLaunchKernelGGL(k_ens_column_list<LAW_ARRWARM>, dim3(nblocks), dim3(256), lds, st, T, start, nm); break;
+  default: hipLaunchKernelGGL(k_ens_column_list<LAW_GK>, dim3(nblocks), dim3(256), lds, st, T, start, nm); break;
+  }
+}
+
 int ens_point_blocks(const DevParams &P) { return (int)(((long)P.Mx * P.My + 255) / 256); }
 
 int ens_column_blocks(const DevParams &P, int cpw) {
@@ -1491,7 +1523,9 @@ void launch_ens_front(const EnsLaunch &L, hipStream_t st) {
 }
 
 void launch_ens_back(const EnsLaunch &L, hipStream_t st) {
-  switch (L.cpw) {
+  // (no column blocks when every member runs through a surrogate)
+  switch (L.col_blocks > 0 ? L.cpw : 0) {
+  case 0: break;
   case 8: ens_column_cpw<8>(L.law, L.table, L.col_start, L.nm, L.col_blocks, L.max_mz, st); break;
   case 32: ens_column_cpw<32>(L.law, L.table, L.col_start, L.nm, L.col_blocks, L.max_mz, st); break;
   default: ens_column_cpw<16>(L.law, L.table, L.col_start, L.nm, L.col_blocks, L.max_mz, st); break;

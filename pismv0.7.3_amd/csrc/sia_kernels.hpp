@@ -181,6 +181,18 @@ struct EnsLaunch {
   int nm, pt_blocks, col_blocks, cpw, law, max_mz;
 };
 
+// list-driven FP64 fast column pass of many members (the surrogate's
+// fallback lists): block b of the grid belongs to member m with start[m] <= b <
+// start[m + 1] and walks that member's list with its share of blocks; its D_max
+// goes to A.dmax_part[b - start[m]]
+struct EnsListMember {
+  DevParams P;
+  ColumnArgs A;                     // A.only unused
+  const int *list, *n_list;         // staggered columns, device count
+};
+void launch_ens_column_list(const EnsListMember *T, const int *start, int nm, int nblocks, int max_mz, int law,
+                            hipStream_t st);
+
 int ens_point_blocks(const DevParams &P);
 int ens_column_blocks(const DevParams &P, int cpw);
 void launch_ens_step(const EnsLaunch &L, hipStream_t st);

@@ -16,8 +16,10 @@ from test_gpu_ensemble import MEMBERS, Member, assert_same, member_workload
 
 pytestmark = pytest.mark.gpu
 
-# member index -> surrogate (by Glen exponent); member 3 stays FP64
+# member index -> surrogate (by Glen exponent): member 3 stays FP64 in BOUND;
+# ALL_BOUND leaves the column stage without blocks
 BOUND = {0: 3.0, 1: 4.0, 2: 3.0}
+ALL_BOUND = {0: 3.0, 1: 4.0, 2: 3.0, 3: 4.0}
 
 
 @pytest.fixture(scope="module")
@@ -49,9 +51,9 @@ def surrogates(S):
     return out
 
 
-def make_bound(S, surrogates):
+def make_bound(S, surrogates, bound=BOUND):
     ms = [Member(S, member_workload(*m)) for m in MEMBERS]
-    for i, n in BOUND.items():
+    for i, n in bound.items():
         ms[i].sia.use_surrogate(surrogates[n])
     return ms
 
@@ -63,23 +65,27 @@ def state(m, bound):
     return st
 
 
-@pytest.fixture(scope="module")
-def reference(S, surrogates):
-    say("[gpu] ensemble surrogate: members stepped alone")
-    ms = make_bound(S, surrogates)
+def run_alone(S, surrogates, bound):
+    say("[gpu] ensemble surrogate: members stepped alone, bound %s" % sorted(bound))
+    ms = make_bound(S, surrogates, bound)
     after3, after6, t6 = [], [], []
     for i, m in enumerate(ms):
         m.step_alone(3)
-        after3.append(state(m, i in BOUND))
+        after3.append(state(m, i in bound))
         m.step_alone(3)
-        after6.append(state(m, i in BOUND))
+        after6.append(state(m, i in bound))
         t6.append(m.t.item())
-    for i in BOUND:
+    for i in bound:
         c = after6[i]["counts"]
         assert c[1] > 0, "member %d: no surrogate columns" % i
     for m in ms:
         m.sia.close()
     return after3, after6, t6
+
+
+@pytest.fixture(scope="module")
+def reference(S, surrogates):
+    return run_alone(S, surrogates, BOUND)
 
 
 @pytest.mark.parametrize("graph", [False, True])
@@ -111,6 +117,25 @@ def test_binding_change_is_refused(S, surrogates):
     ms[0].sia.use_surrogate(surrogates[3.0])
     ens.step(1)
     ens.check_errors()
+    ens.close()
+    for m in ms:
+        m.sia.close()
+
+
+def test_every_member_bound(S, surrogates):
+    """All members bound (the column stage has no blocks), from a graph."""
+    after3, after6, t6 = run_alone(S, surrogates, ALL_BOUND)
+    say("[gpu] ensemble with every member bound")
+    ms = make_bound(S, surrogates, ALL_BOUND)
+    ens = S.Ensemble([m.sia for m in ms], [m.geom for m in ms], [m.dev["E"] for m in ms])
+    ens.step(3, graph=True)
+    ens.check_errors()
+    for i, m in enumerate(ms):
+        assert_same(state(m, True), after3[i], "member %d after 3" % i)
+    ens.step(3, graph=True)
+    for i, m in enumerate(ms):
+        assert_same(state(m, True), after6[i], "member %d after 6" % i)
+        assert ens.time(i)[0] == t6[i]
     ens.close()
     for m in ms:
         m.sia.close()
