@@ -274,8 +274,9 @@ __device__ __forceinline__ double pow_n(const DevParams &P, double p) { return i
 // W_k = d_k (dz_k [k >= 1] + dz_{k+1} [k < ks]) / 2 + d_ks^2 / 2 [k == ks]
 // (SIAFD.cc:677-684 regrouped per delta_k), G_k = W_k p_k^n.
 constexpr int SG_FEAT_COLS = 16;
-// 128-level batches of a pair's enthalpy loads issued together
-constexpr int SG_STAGE_BATCHES = 3;
+// 128-level batches of a pair's enthalpy loads issued together (two: with the
+// loads in flight the inference pass fits 128 VGPRs, 4 waves per SIMD)
+constexpr int SG_STAGE_BATCHES = 2;
 // floats of a wavefront's output rows: F [16][36], CQb [16][32], CSt [16] (FP64)
 constexpr int SG_FEAT_OUT = SG_FEAT_COLS * (SG_FSTRIDE + SG_NCELL) + 2 * SG_FEAT_COLS;
 
@@ -588,8 +589,12 @@ __device__ __forceinline__ void sg_features_block(const DevParams &P, const SgCo
   }
 }
 
+// inference: at most 128 VGPRs (4 waves per SIMD; measured 2.69 -> 2.59 ms on
+// the 1501x2801x401 sheet against 3 waves with 3 load batches, a few spilled
+// registers outside the level loop); training samples: no cap
 template <bool TARGETS, int NP>
-__global__ __launch_bounds__(256) void k_sg_features(DevParams P, SgColumnArgs C, SgFeatureArgs F) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TARGETS ? 1 : 4)))
+void k_sg_features(DevParams P, SgColumnArgs C, SgFeatureArgs F) {
   extern __shared__ double2 zsk[];
   __shared__ int bucket[SG_NBUCKET];
   sg_features_block<TARGETS, NP>(P, C, F, blockIdx.x, gridDim.x, zsk, bucket);
@@ -916,7 +921,7 @@ __global__ __launch_bounds__(SG_CLS_BLOCK) void k_ens_sg_scatter(const EnsSgMemb
 }
 
 template <int NP>
-__global__ __launch_bounds__(256) void k_ens_sg_features(const EnsSgMember *__restrict__ T,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_ens_sg_features(const EnsSgMember *__restrict__ T,
                                                          const int *__restrict__ start, int nm) {
   extern __shared__ double2 zsk[];
   __shared__ int bucket[SG_NBUCKET];
