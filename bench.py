@@ -35,7 +35,7 @@ with per-member launches.
 
 --combined runs the combined workload instead and prints its own line: the
 surrogate inside the 64-member ensemble (the sheet + 63 ensemble members),
-5000 steps in HIP graphs of 100, the surrogate's flux error judged in FP64
+5000 steps in HIP graphs of 100, the surrogate's flux error checked in FP64
 every 100 steps, beside an all-plain FP64 run of the same seeds.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--path surrogate|plain]
@@ -453,7 +453,7 @@ def final_vs(leg, ref):
 def run_combined(S, torch, args):
     """The combined run: the learned surrogate inside the 64-member batched
     ensemble (the 1501x2801x401 sheet + 63 ensemble members), the surrogate's
-    flux error judged in FP64 every 100 steps, beside an all-plain FP64 run
+    flux error checked in FP64 every 100 steps, beside an all-plain FP64 run
     of the same seeds. Two bindings are run: the depth policy (members of at
     least SURROGATE_MIN_MZ levels; the reported value) and every member."""
     steps = args.combined_steps

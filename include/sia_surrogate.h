@@ -88,7 +88,7 @@ int sia_surrogate_counts(sia_handle *h, int64_t counts[3]);
    correction; the columns it singles out carry the largest surrogate errors.
    tau is +inf (no score fallback) until set, and is saved with the model.
    calibrate_fallback runs one surrogate fast update of `in` on h (fields the
-   model was trained on, not those it will be judged on) with every in-range
+   model was trained on, not those it will be evaluated on) with every in-range
    column kept, and sets tau to the (1 - fraction) quantile of their scores:
    about `fraction` of such columns then fall back. tau (host) or NULL. */
 int sia_surrogate_calibrate_fallback(sia_surrogate *sg, sia_handle *h, const sia_inputs *in, double fraction,

@@ -397,7 +397,7 @@ class Surrogate:
 
     def calibrate_fallback(self, sia, H, b, h, mask, E, fraction, bed_state_counter=0):
         """Sets the score fallback threshold tau from one surrogate update of
-        these fields (training fields, not the ones to be judged): about
+        these fields (training fields, not the ones it will be evaluated on): about
         `fraction` of the in-range columns then fall back to FP64. Returns tau."""
         inp = T.SiaInputs(ptr(H), ptr(b), ptr(h), ptr(mask), ptr(E), None, None, None, None, bed_state_counter)
         tau = C.c_double()
