@@ -534,8 +534,6 @@ def main():
     ap.add_argument("--train-batch", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-member ensemble measurement")
-    ap.add_argument("--gpu-warmup-s", type=float, default=5.0,
-                    help="seconds of untimed full updates before the first timed region")
     ap.add_argument("--combined", action="store_true",
                     help="run the combined workload instead (surrogate inside the 64-member ensemble, 5000 steps)")
     ap.add_argument("--combined-steps", type=int, default=COMBINED_STEPS,
@@ -568,13 +566,6 @@ def main():
 
     plain = S.SIAFD(grid, params, stream=stream.cuda_stream)
     plain.set_sync_check(False)
-    # the first seconds of load after an idle GPU run slower (the full update's
-    # velocity pass measured 25.1, 24.0, 22.4 ms on three successive handles of
-    # one fresh process): untimed full updates before the first timed region
-    t_w = time.perf_counter()
-    while time.perf_counter() - t_w < args.gpu_warmup_s:
-        plain.update(f["H"], f["b"], f["h"], f["mask"], f["E"], fast=False)
-        torch.cuda.synchronize()
     results = {}
     for mode in (("full", "fast") if args.path == "surrogate" else (args.mode,)):
         ms, stages, _ = time_steps(S, torch, plain, f, args.steps, args.warmup, fast=(mode == "fast"))
