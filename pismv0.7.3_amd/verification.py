@@ -1,10 +1,17 @@
-"""Verification tests C and E of pismv on the HIP path: the known-answer
+"""Verification tests B, C and E of pismv on the HIP path: the known-answer
 end-to-end checks of SIA + mass continuity (C: isothermal SIA with
-time-dependent SMB) and of the SIA-type sliding path (E: test_14.sh).
+time-dependent SMB), of the SIA-type sliding path (E: test_14.sh) and the
+mass-conservation invariant of test_12.sh (B: Halfar's solution, zero SMB).
 
 Restated from the reference (paths relative to its root):
 * exactC, the Bueler et al. (2005) similarity solution with lambda = 5
-  (src/verif/tests/exactTestsABCDE.c:80-107);
+  (src/verif/tests/exactTestsABCDE.c:80-107), and exactB, the lambda = 0
+  member (Halfar 1983; exactTestsABCDE.c:56-77);
+* the test B domain: Lx = 1200 km, Lz = 4000 m (src/pismv.cc:91-97), the rest
+  as test C (initTestABCDEH, src/verif/iceCompModel.cc:274-280, 302-377);
+* test_12.sh: -Mx 31 -My 31 -Mz 31 -ys 1000 -y 5000 -max_dt 25, ice volume
+  every 25 a from year 1000 (test/regression/test_12.sh:14-20), and its check
+  diff(ivol).max() < 10^(floor(log10 max ivol) - 14) (test_12.sh:27-44);
 * the test C setup of pismv / IceCompModel: Lx = 1000 km, Lz = 4000 m, EQUAL
   levels, non-periodic grid (src/pismv.cc:64-136); zero bed, arr flow law with
   T chosen so that A = 1e-16 Pa-3 a-1, enhancement 1, no bed smoothing, dry,
@@ -45,6 +52,56 @@ def exactC(t, r):
         Rm = R0 * math.pow(0.1 * SPERA_EXACT / t0, beta)
         M = np.where(r < Rm, 5 * H0 / t0, 0.0)
     return H, M
+
+
+def exactB(t, r):
+    """exactTestsABCDE.c:56-77 (vectorised over r): H of Halfar's solution,
+    alpha = 1/9, beta = 1/18, t0 = 422.45 a of the closed form's year; M = 0."""
+    n, H0, R0 = 3.0, 3600.0, 750000.0
+    alpha, beta = 1.0 / 9.0, 1.0 / 18.0
+    t0 = 422.45 * SPERA_EXACT
+    r = np.asarray(r, dtype=np.float64)
+    Rmargin = R0 * math.pow(t / t0, beta)
+    H = np.zeros_like(r)
+    inside = r < Rmargin
+    if inside.any():
+        H[inside] = H0 * math.pow(t / t0, -alpha) * np.power(
+            1.0 - np.power(math.pow(t / t0, -beta) * (r[inside] / R0), (n + 1) / n), n / (2 * n + 1))
+    return H
+
+
+def test_b_setup(Mx, My, Mz=31, max_dt_years=25.0):
+    """pismv -test B: test C's setup on the 2400 km domain (pismv.cc:91-97),
+    with -max_dt (the maximum_time_step parameter, in seconds)."""
+    grid, P, r, E0 = test_c_setup(Mx, My, Mz, Lx=1200e3)
+    P.maximum_time_step = max_dt_years * SECPERA
+    return grid, P, r, E0
+
+
+def ice_volume(grid, H):
+    """IceModel::ice_volume: the sum of H dx dy over the cells (m^3)."""
+    return float(np.sum(H)) * grid.dx * grid.dy
+
+
+def run_reporting_loop(step_fn, report_fn, t_start_years, years, every_years):
+    """The run loop from t_start for `years`, every step ending no later than
+    the next reporting time (t_start + k every_years; reporting times cap the
+    step as -ts_times does, IceModel::max_timestep). step_fn(t, time_to_end)
+    takes one step and returns its dt; report_fn(t) runs at the start and at
+    each reporting time. Returns the number of steps."""
+    t = t_start_years * SECPERA
+    t_end = (t_start_years + years) * SECPERA
+    report_fn(t)
+    k, steps = 1, 0
+    while t < t_end:
+        t_mark = min(t_end, (t_start_years + k * every_years) * SECPERA)
+        t += step_fn(t, t_mark - t)
+        steps += 1
+        if t_mark - t < 1e-3:
+            t = t_mark
+            report_fn(t)
+            k += 1
+    return steps
 
 
 def test_c_setup(Mx, My, Mz=31, Lx=1000e3, Lz=4000.0):
@@ -221,6 +278,36 @@ def run_test_c_gpu(Mx, My, Mz=31, years=5000.0):
     t, steps = run_time_loop(step, years)
     Hf = dev["H"].cpu().numpy()
     return geometry_errors(grid, Hf, r, t), steps, Hf
+
+
+def run_test_b_gpu(Mx=31, t_start_years=1000.0, years=5000.0, every_years=25.0):
+    """test_12.sh's run of test B through the HIP library (C ABI): full SIA
+    updates, dt (at most 25 a, capped at the reporting times), mass continuity
+    with zero SMB. Returns (ice volumes at the reporting times, final H, steps)."""
+    import pism_sia as S
+    import workloads as W
+    grid, P, r, E0 = test_b_setup(Mx, Mx)
+    n2 = Mx * Mx
+    H0 = exactB(t_start_years * SECPERA, r)
+    b = np.zeros(n2)
+    mask, h = W.geometry_mask_surface(P, b, H0)
+    dev = dict(H=S.to_device(H0), b=S.to_device(b), h=S.to_device(h), mask=S.to_device(mask),
+               E=S.to_device(np.full(n2 * grid.Mz, E0)))
+    sia = S.SIAFD(grid, P)
+    geom = S.geometry(dev["H"], dev["h"], dev["mask"], dev["b"], smb=S.to_device(np.zeros(n2)))
+    ivol = []
+
+    def step(t, tte):
+        sia.update(dev["H"], dev["b"], dev["h"], dev["mask"], dev["E"], fast=False)
+        dt = sia.max_timestep(tte)
+        sia.mass_continuity_step(geom, dt, accounting=False)
+        return dt
+
+    steps = run_reporting_loop(step, lambda t: ivol.append(ice_volume(grid, dev["H"].cpu().numpy())),
+                               t_start_years, years, every_years)
+    Hf = dev["H"].cpu().numpy()
+    sia.close()
+    return np.array(ivol), Hf, steps
 
 
 def run_test_e_gpu(Mx, years=100.0):

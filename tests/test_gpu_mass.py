@@ -168,3 +168,21 @@ def test_gpu_reproduces_test_14_table(S, Mx, key):
     say("  geometry", got_geo, "golden", gold["geometry"])
     say("  base vels", got_vel, "golden", gold["base_vels"])
     assert got_geo == gold["geometry"] and got_vel == gold["base_vels"]
+
+
+def test_gpu_test_12_mass_conservation(S, oracle):
+    """test_12.sh on the HIP path: test B (Halfar), 31x31x31, years 1000-6000,
+    max_dt 25 a, ice volume every 25 a; diff(ivol).max() below the 14th digit
+    of max(ivol). The final thickness agrees with the oracle's run to 1e-9 of
+    max(H) (FP64 rounding over 200 steps)."""
+    say("[gpu] test_12: test B mass conservation, 200 steps")
+    import verification as V
+    from test_verification_B import run_oracle_test_b
+    ivol, H, steps = V.run_test_b_gpu(31)
+    assert len(ivol) == 201 and steps >= 200
+    threshold = 10.0 ** (math.floor(math.log10(ivol.max())) - 14)
+    say("  diff(ivol) in [%.2e, %.2e] m^3, threshold %.1e" % (np.diff(ivol).min(), np.diff(ivol).max(), threshold))
+    assert np.diff(ivol).max() < threshold
+    ivol_o, H_o, _ = run_oracle_test_b(oracle)
+    assert np.abs(H - H_o).max() <= 1e-9 * H_o.max()
+    assert np.allclose(ivol, ivol_o, rtol=1e-12, atol=0.0)
