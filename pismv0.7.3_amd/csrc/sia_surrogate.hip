@@ -13,8 +13,6 @@
 //       k_sg_loss, k_sg_act_bwd, k_sg_colsum, k_sg_adam
 #include "sia_surrogate.hpp"
 
-#include <type_traits>
-
 namespace pism_sia {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -89,35 +87,6 @@ __device__ __forceinline__ double half_sum(double v, int half) {
   const double s = sg_scan(v);
   const double a = sg_lane(s, 31), b = sg_lane(s, 63);
   return half ? b - a : a;
-}
-
-// the same with five DPP steps: row scans, then row 0 (2) added into row 1 (3)
-__device__ __forceinline__ double half_total(double v, int half) {
-  v += sg_dpp<0x111, 0xf, true>(v);
-  v += sg_dpp<0x112, 0xf, true>(v);
-  v += sg_dpp<0x114, 0xf, true>(v);
-  v += sg_dpp<0x118, 0xf, true>(v);
-  v += sg_dpp<0x142, 0xa, false>(v);
-  return half ? sg_lane(v, 63) : sg_lane(v, 31);
-}
-
-// 1 / x: hardware reciprocal estimate refined by two Newton steps (about 1 ulp)
-__device__ __forceinline__ double rcp_nr(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  double e = __builtin_fma(-x, r, 1.0);
-  r = __builtin_fma(r, e, r);
-  e = __builtin_fma(-x, r, 1.0);
-  return __builtin_fma(r, e, r);
-}
-
-// exp(a) = 2^n 2^f with n = rint(a log2 e) and |f| <= 1/2 evaluated by the
-// FP32 hardware exp2: relative error about 1e-7 for |a| < 700 (the surrogate's
-// physics baseline only; the training targets use the FP64 exp)
-__device__ __forceinline__ double exp_split(double a) {
-  const double t = a * 1.4426950408889634;
-  const double n = __builtin_rint(t);
-  const float f = (float)(t - n);
-  return __builtin_ldexp((double)__builtin_amdgcn_exp2f(f), (int)n);
 }
 
 // ---------------------------------------------------------------- classification
@@ -315,45 +284,287 @@ __device__ __forceinline__ double pown(double p) {
   return N == 2 ? p2 : (N == 3 ? p2 * p : p2 * p2);
 }
 
-// Inference pass over the levels of cell m (levels [klo, khi)) of one column.
-// x_k = T_pa - T_ref = E_k / c_i + T0 - T_ref + beta p_k, staged without the
-// beta p_k term as y_k; cold iff x_k < T_melt - T_ref (E_k < E_cts, since
-// E - E_cts = c_i (x - (T_melt - T_ref))). p_k = p_air + rho g (thk - z_k).
-// Sums are raw (not scaled by thk^2 p_base^n): g0 = sum G, g1 = sum G x,
-// g2 = sum G x^2 over the cold levels, st = exact temperate sum of G A.
+template <int N>
+__device__ __forceinline__ float pownf(float p) {
+  const float p2 = p * p;
+  return N == 2 ? p2 : (N == 3 ? p2 * p : p2 * p2);
+}
+
+__device__ __forceinline__ float lane_value_f(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// sum of v over the 32 lanes of this lane's half wave (FP32, five DPP steps)
+template <int CTRL, int ROW_MASK, bool BOUND_ZERO>
+__device__ __forceinline__ float sg_dppf(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xf, BOUND_ZERO));
+}
+__device__ __forceinline__ float half_total_f(float v, int half) {
+  v += sg_dppf<0x111, 0xf, true>(v);
+  v += sg_dppf<0x112, 0xf, true>(v);
+  v += sg_dppf<0x114, 0xf, true>(v);
+  v += sg_dppf<0x118, 0xf, true>(v);
+  v += sg_dppf<0x142, 0xa, false>(v);
+  return half ? lane_value_f(v, 63) : lane_value_f(v, 31);
+}
+
+// 1 / x: hardware estimate refined by one Newton step
+__device__ __forceinline__ float rcpf_nr(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  return __builtin_fmaf(r, __builtin_fmaf(-x, r, 1.0f), r);
+}
+
+// kBelowHeight (IceGrid.cc:379-391) clamped to Mz - 2, from the bucket of the
+// height: the last level below the bucket's lower edge, then 0-2 steps
+__device__ __forceinline__ int kbelow_bucket(const double *zd, const int *bucket, double bscale, int Mz, double Lz,
+                                             double height) {
+  if (height < 0.0 - 1.0e-6 || height > Lz + 1.0e-6) return -1;
+  int b = (int)(height * bscale);
+  b = b < 0 ? 0 : (b > SG_NBUCKET - 1 ? SG_NBUCKET - 1 : b);
+  int k = bucket[b] - 1;
+  k = k < 0 ? 0 : (k > Mz - 2 ? Mz - 2 : k);
+  while (k > 0 && zd[k] > height) k--;
+  while (k < Mz - 2 && zd[k + 1] <= height) k++;
+  return k;
+}
+
+// Inference pass (FP32 level arithmetic; the column constants, the cell
+// boundaries and the outputs' scales stay FP64). A wavefront owns 16
+// consecutive list entries; lane c < 16 prepares entry c. The entries are taken
+// in pairs: all 64 lanes stage the averaged enthalpy of both columns in LDS as
+// y_k = E_k / c_i + T0 - T_ref (up to 2 x 128 levels of both columns in flight),
+// then half wave hh takes column hh and lane m its zeta-cell m (the levels with
+// floor(32 z_k / thk) = m, level ks in cell 31 at most). Per level, with
+// pressures relative to the basal one, ph = p_k / p_b = (p_air + rho g d_k) / p_b:
+//   W_k = d_k (dz_k [k >= 1] + dz_{k+1} [k < ks]) / 2 + d_ks^2 / 2 [k = ks]
+//   (SIAFD.cc:677-684 regrouped per delta_k), G_k = W_k ph^n,
+//   x_k = T_pa - T_ref = y_k + beta p_b ph, cold iff x_k < T_melt - T_ref (E < E_cts)
+// g0, g1, g2: sums of G, G x, G x^2 over the cold levels; st: the exact
+// temperate sum of G A_T0 (1 + wf_coeff min(omega, wf_limit)). The outputs are
+// scaled back by C p_b^n in FP64. The cell epilogue evaluates the physics
+// baseline Qb = g0 A(T) (1 + (k^2 - 2k/T) var / 2), T = T_ref + g1/g0,
+// k = Q / (R T^2), with the hardware exp2 and Newton reciprocals.
 template <int NP>
-__device__ __forceinline__ void sg_cell_sums(const DevParams &P, const double2 *zsk, const float *yb, int klo,
-                                             int khi, int ks, double thk, double sk_top, double pc, double rhog,
-                                             double x_cts, double ta, double tcap, double &g0, double &g1,
-                                             double &g2, double &st) {
+__device__ __forceinline__ void sg_infer_features_block(const DevParams &P, const SgColumnArgs &C,
+                                                        const SgFeatureArgs &F, int bid, int nblk, double2 *zsk,
+                                                        int *bucket) {
+  const int Mz = P.Mz;
+  const int Mzp = (Mz + 1) & ~1;  // even: 16-byte aligned LDS regions, 8-byte aligned level pairs
+  // FP64 z_k (kBelowHeight, cell boundaries), FP32 (z_k, half spacing) (levels)
+  double *zd = reinterpret_cast<double *>(zsk);
+  float2 *zf = reinterpret_cast<float2 *>(zd + Mzp);
+  for (int k = threadIdx.x; k < Mz; k += blockDim.x) {
+    const double zk = C.z[k];
+    const double w = 0.5 * (((k >= 1) ? zk - C.z[k - 1] : 0.0) + ((k + 1 < Mz) ? C.z[k + 1] - zk : 0.0));
+    zd[k] = zk;
+    zf[k] = make_float2((float)zk, (float)w);
+  }
+  const double bscale = (double)SG_NBUCKET / C.z[Mz - 1];
+  for (int b = threadIdx.x; b < SG_NBUCKET; b += blockDim.x) {
+    const double lo = (double)b / bscale;  // first level at or above the bucket's lower edge
+    int a = 0, c = Mz;
+    while (a < c) {
+      const int mid = (a + c) >> 1;
+      if (C.z[mid] >= lo) c = mid; else a = mid + 1;
+    }
+    bucket[b] = a;
+  }
+  __syncthreads();
+  const int nwb = blockDim.x >> 6;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  float *ebw = reinterpret_cast<float *>(zf + Mzp) + (long)wv * 2 * Mzp;
+  float *oF = reinterpret_cast<float *>(zf + Mzp) + (long)nwb * 2 * Mzp + (long)wv * SG_FEAT_OUT;
+  float *oQ = oF + SG_FEAT_COLS * SG_FSTRIDE;
+  double *oS = reinterpret_cast<double *>(oQ + SG_FEAT_COLS * SG_NCELL);
+  const int n = *F.n_list;
+  const double rhog = P.rho_i * P.g;
+  const float T_ref = 263.15f;
+  const double y_off = P.T0 - 263.15;
+  const float x_cts = (float)(P.T_melt - 263.15);
+  // temperate softness A_T0 + ta min(x - x_cts, tcap), ta = A_T0 wf_coeff c_i / L, tcap = wf_limit L / c_i
+  const double ta = P.A_T0 * P.wf_coeff * P.ci * P.inv_L, tcap = P.wf_limit / (P.ci * P.inv_L);
+  const float t_a = (float)ta, t_b = (float)(P.A_T0 - ta * (P.T_melt - 263.15)), t_cap = (float)(P.A_T0 + ta * tcap);
+  const float QRc = (float)(P.Q_cold / P.R), QRw = (float)(P.Q_warm / P.R);
+  const float Ac = (float)P.A_cold, Aw = (float)P.A_warm, Tcrit = (float)P.T_crit;
+  const int hh = lane >> 5, m = lane & (SG_NCELL - 1);
+  for (int t0 = (bid * nwb + wv) * SG_FEAT_COLS; t0 < n; t0 += nblk * nwb * SG_FEAT_COLS) {
+    // ---- setup: lane c < 16 prepares entry t0 + c (c_ks = -1: no entry)
+    int c_ks = -1, c_q = 0, c_q2 = 0;
+    double c_thk = 0.0, c_s32 = 0.0, c_cs = 0.0;
+    float c_lgs = 0.0f, c_cz = 0.0f, c_pa = 0.0f, c_bp = 0.0f;
+    if (lane < SG_FEAT_COLS && t0 + lane < n) {
+      const int s = F.list[t0 + lane];
+      const int q = s >> 1, o = s & 1;
+      const int i = q / P.My, j = q - i * P.My;
+      const int q2 = (o == 0) ? wrapi(i + 1, P.Mx) * P.My + j : i * P.My + wrapi(j + 1, P.My);
+      const double thk = 0.5 * (C.thks[q] + C.thks[q2]);
+      const double hxs = C.hx[s], hys = C.hy[s];
+      const double alpha = sqrt(hxs * hxs + hys * hys);
+      c_lgs = (float)(log10(alpha) + 2.0);  // the slope input of the network
+      const double pb = P.p_air + rhog * thk;  // basal pressure
+      c_cs = P.e * (0.5 * (C.theta[q] + C.theta[q2])) * 2.0 * ipow_nm1(P, alpha) * pown<NP>(pb);
+      c_cz = (float)(rhog / pb);
+      c_pa = (float)(P.p_air / pb);
+      c_bp = (float)(P.beta * pb);
+      c_s32 = (double)SG_NCELL / thk;
+      c_ks = kbelow_bucket(zd, bucket, bscale, Mz, P.Lz, thk);  // >= 1 for listed columns
+      c_thk = thk;
+      c_q = q;
+      c_q2 = q2;
+    }
+    for (int cc = 0; cc < SG_FEAT_COLS; cc += 2) {
+      if (lane_value_i(c_ks, cc) < 0) break;  // entries are consecutive: nothing left
+      // ---- stage y of columns cc, cc + 1: lane l holds levels k, k + 1 with
+      // k = kb + 128 b + 2 l (16-byte loads; k <= ks <= Mz - 2 keeps k + 1
+      // inside the column)
+      int ksp[2];
+      const double *Ep[2][2];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        ksp[h] = lane_value_i(c_ks, cc + h);
+        Ep[h][0] = C.E + (long)lane_value_i(c_q, cc + h) * Mz;
+        Ep[h][1] = C.E + (long)lane_value_i(c_q2, cc + h) * Mz;
+      }
+      const int ksmax = max(ksp[0], ksp[1]);
+      const double hci = 0.5 * P.inv_ci;
+      for (int kb = 0; kb <= ksmax; kb += 128 * SG_STAGE_BATCHES) {
+        d2u ea[SG_STAGE_BATCHES][2], ebv[SG_STAGE_BATCHES][2];
+#pragma unroll
+        for (int bt = 0; bt < SG_STAGE_BATCHES; bt++) {
+          const int k = kb + 128 * bt + 2 * lane;
+#pragma unroll
+          for (int h = 0; h < 2; h++) {
+            if (k <= ksp[h]) {
+              ea[bt][h] = *reinterpret_cast<const d2u *>(Ep[h][0] + k);
+              ebv[bt][h] = *reinterpret_cast<const d2u *>(Ep[h][1] + k);
+            }
+          }
+        }
+#pragma unroll
+        for (int bt = 0; bt < SG_STAGE_BATCHES; bt++) {
+          const int k = kb + 128 * bt + 2 * lane;
+#pragma unroll
+          for (int h = 0; h < 2; h++) {
+            if (k <= ksp[h]) {
+              f2a y;
+              y.x = (float)__builtin_fma(ea[bt][h].x + ebv[bt][h].x, hci, y_off);
+              y.y = (float)__builtin_fma(ea[bt][h].y + ebv[bt][h].y, hci, y_off);
+              *reinterpret_cast<f2a *>(ebw + h * Mzp + k) = y;
+            }
+          }
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // ---- half wave hh: column cc + hh, lane m: zeta-cell m
+      const int ks = hh ? ksp[1] : ksp[0];
+      const bool live = ks >= 0;
+      const double thk = hh ? lane_value_d(c_thk, cc + 1) : lane_value_d(c_thk, cc);
+      const double s32 = hh ? lane_value_d(c_s32, cc + 1) : lane_value_d(c_s32, cc);
+      // cell m: levels [klo, khi), the first level with z_k s32 >= m from the
+      // bucket of height m thk / 32; khi is the next lane's klo
+      int klo = 0;
+      if (live) {
+        int b = (int)((float)m * (float)thk * (float)(bscale / SG_NCELL));
+        b = b > SG_NBUCKET - 1 ? SG_NBUCKET - 1 : b;
+        int k = min(bucket[b], ks + 1);
+        while (k > 0 && zd[k - 1] * s32 >= (double)m) k--;
+        while (k <= ks && zd[k] * s32 < (double)m) k++;
+        klo = k;
+      }
+      int khi = __shfl_down(klo, 1, 64);
+      if (m == SG_NCELL - 1) khi = ks + 1;
+      if (!live) khi = 0;
+      const float thkf = (float)thk;
+      const float cz = hh ? lane_value_f(c_cz, cc + 1) : lane_value_f(c_cz, cc);
+      const float pa = hh ? lane_value_f(c_pa, cc + 1) : lane_value_f(c_pa, cc);
+      const float bp = hh ? lane_value_f(c_bp, cc + 1) : lane_value_f(c_bp, cc);
+      // W_ks = d_ks (dz_ks + d_ks) / 2
+      const int kt = live ? ks : 0;
+      const float skt = 0.5f * ((kt >= 1 ? zf[kt].x - zf[kt - 1].x : 0.0f) + (thkf - zf[kt].x));
+      const float *yb = ebw + hh * Mzp;
+      float g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, st = 0.0f;
 #pragma unroll 2
-  for (int k = klo; k < khi; k++) {
-    const double2 zk = zsk[k];  // z_k, (dz_k [k >= 1] + dz_{k+1}) / 2
-    const double d = thk - zk.x;
-    const double p = __builtin_fma(-rhog, zk.x, pc);
-    const double W = d * ((k == ks) ? sk_top : zk.y);
-    const double G = W * pown<NP>(p);
-    const double x = __builtin_fma(P.beta, p, (double)yb[k]);
-    const double Gc = (x < x_cts) ? G : 0.0;
-    g0 += Gc;
-    const double gx = Gc * x;
-    g1 += gx;
-    g2 = __builtin_fma(gx, x, g2);
-    // temperate: A_T0 (1 + wf_coeff min(omega, wf_limit)), omega = c_i (x - x_cts) / L
-    const double om = fmin(x - x_cts, tcap);
-    st = __builtin_fma(G - Gc, __builtin_fma(ta, om, P.A_T0), st);
+      for (int k = klo; k < khi; k++) {
+        const float2 zw = zf[k];
+        const float d = thkf - zw.x;  // >= 0: rounding to FP32 keeps z_k <= thk
+        const float ph = __builtin_fmaf(cz, d, pa);
+        const float G = d * (k == ks ? skt : zw.y) * pownf<NP>(ph);
+        const float x = __builtin_fmaf(bp, ph, yb[k]);
+        const float Gc = (x < x_cts) ? G : 0.0f;
+        g0 += Gc;
+        const float gx = Gc * x;
+        g1 += gx;
+        g2 = __builtin_fmaf(gx, x, g2);
+        st = __builtin_fmaf(G - Gc, __builtin_fminf(__builtin_fmaf(t_a, x, t_b), t_cap), st);
+      }
+      // a cold top level with d = thk - z_ks > 0 whose G underflows in FP32 still
+      // populates its cell (the FP64 pass sees G > 0 of negligible weight)
+      if (live && klo <= ks && ks < khi && g0 == 0.0f && zd[ks] < thk) {
+        const float x = __builtin_fmaf(bp, __builtin_fmaf(cz, thkf - zf[ks].x, pa), yb[ks]);
+        if (x < x_cts) {
+          g0 = 1.0e-30f;
+          g1 = g0 * x;
+          g2 = g1 * x;
+        }
+      }
+      const float S_t = half_total_f(st, hh);
+      const bool has = live && g0 > 0.0f;
+      float Qh = 0.0f, feat = 2.0f;
+      if (has) {
+        const float ig0 = rcpf_nr(g0);
+        const float mx = g1 * ig0;
+        const float Tbar = T_ref + mx;
+        const float var = __builtin_fmaxf(__builtin_fmaf(g2, ig0, -mx * mx), 0.0f);
+        const float iT = rcpf_nr(Tbar);
+        const bool cold = Tbar < Tcrit;
+        const float QR = cold ? QRc : QRw;
+        const float kk = QR * iT * iT;
+        // Paterson-Budd softness at the cell mean (flowlaws.cc:90-95)
+        const float Ab = (cold ? Ac : Aw) * __builtin_amdgcn_exp2f(-1.4426950408889634f * QR * iT);
+        Qh = g0 * Ab * __builtin_fmaf(0.5f * (kk * kk - 2.0f * kk * iT), var, 1.0f);
+        feat = (Tbar - 250.0f) * (1.0f / 15.0f);
+      }
+      if (live) {
+        const int r = cc + hh;
+        const double cs = hh ? lane_value_d(c_cs, cc + 1) : lane_value_d(c_cs, cc);
+        oF[r * SG_FSTRIDE + m] = feat;
+        oQ[r * SG_NCELL + m] = (float)(cs * (double)Qh);
+        if (m < SG_FSTRIDE - SG_NCELL) {
+          const float lgs = hh ? lane_value_f(c_lgs, cc + 1) : lane_value_f(c_lgs, cc);
+          oF[r * SG_FSTRIDE + SG_NCELL + m] = m == 0 ? (float)(thk / 1000.0) : (m == 1 ? lgs : 0.0f);
+        }
+        if (m == 0) oS[r] = cs * (double)S_t;
+      }
+      // the next pair overwrites the enthalpy buffers
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    // rows t0 .. t0 + nr - 1 of F (36 floats), CQb (32 floats), CSt
+    const int nr = min(SG_FEAT_COLS, n - t0);
+    const float4 *sF = reinterpret_cast<const float4 *>(oF);
+    const float4 *sQ = reinterpret_cast<const float4 *>(oQ);
+    float4 *dF = reinterpret_cast<float4 *>(F.F + (long)t0 * SG_FSTRIDE);
+    float4 *dQ = reinterpret_cast<float4 *>(F.CQb + (long)t0 * SG_NCELL);
+    for (int v = lane; v < nr * (SG_FSTRIDE / 4); v += 64) dF[v] = sF[v];
+    for (int v = lane; v < nr * (SG_NCELL / 4); v += 64) dQ[v] = sQ[v];
+    if (lane < nr) F.CSt[t0 + lane] = oS[lane];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 
-// Block bid of nblk persistent blocks walking the list. LDS: zsk (dynamic):
-// (z_k, (dz_k [k >= 1] + dz_{k+1}) / 2) per level, then per wavefront the two
-// staged profiles: FP64 enthalpy for the exact training targets, in inference
-// FP32 y_k = E_k / c_i + T0 - T_ref (x_k without the beta p_k term), then the
-// inference output rows; bucket: SG_NBUCKET ints.
-template <bool TARGETS, int NP>
-__device__ __forceinline__ void sg_features_block(const DevParams &P, const SgColumnArgs &C, const SgFeatureArgs &F,
-                                                  int bid, int nblk, double2 *zsk, int *bucket) {
-  typedef typename std::conditional<TARGETS, double, float>::type EType;
+// Training samples, in FP64 throughout: the features as the inference pass
+// computes them and the exact per-cell cold sums of the plain per-level flow
+// law (in the reference's order). Block bid of nblk persistent blocks walking
+// every stride-th list entry. LDS: zsk (dynamic): (z_k, (dz_k [k >= 1] +
+// dz_{k+1}) / 2) per level, then per wavefront the two staged FP64 enthalpy
+// profiles; bucket: SG_NBUCKET ints.
+__device__ __forceinline__ void sg_sample_block(const DevParams &P, const SgColumnArgs &C, const SgFeatureArgs &F,
+                                                int bid, int nblk, double2 *zsk, int *bucket) {
   const int Mz = P.Mz;
   for (int k = threadIdx.x; k < Mz; k += blockDim.x) {
     const double zk = C.z[k];
@@ -374,22 +585,13 @@ __device__ __forceinline__ void sg_features_block(const DevParams &P, const SgCo
   const int nwb = blockDim.x >> 6;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int Mzp = (Mz + 1) & ~1;  // even stride: 8-byte aligned level pairs
-  EType *ebw = reinterpret_cast<EType *>(zsk + Mz) + (long)wv * 2 * Mzp;
-  // inference: the chunk's outputs gather in LDS and leave in one coalesced
-  // flush per 16 entries (no global store between the pairs of a chunk)
-  float *oF = reinterpret_cast<float *>(reinterpret_cast<EType *>(zsk + Mz) + (long)nwb * 2 * Mzp) +
-              (long)wv * SG_FEAT_OUT;
-  float *oQ = oF + SG_FEAT_COLS * SG_FSTRIDE;
-  double *oS = reinterpret_cast<double *>(oQ + SG_FEAT_COLS * SG_NCELL);
+  double *ebw = reinterpret_cast<double *>(zsk + Mz) + (long)wv * 2 * Mzp;
   const int n = *F.n_list;
-  const int stride = TARGETS ? F.stride : 1;
+  const int stride = F.stride;
   const int nsel = (n + stride - 1) / stride;
-  const long long base_sample = TARGETS ? *F.n_samples : 0;
+  const long long base_sample = *F.n_samples;
   const double T_ref = 263.15;
   const double rhog = P.rho_i * P.g, T_off = P.T0 + P.T_melt - T_ref;
-  const double y_off = P.T0 - T_ref, x_cts = P.T_melt - T_ref;
-  // temperate softness A_T0 (1 + wf_coeff min(omega, wf_limit)) with omega = (c_i / L) (x - x_cts)
-  const double ta = P.A_T0 * P.wf_coeff * P.ci * P.inv_L, tcap = P.wf_limit / (P.ci * P.inv_L);
   const double QR_cold = P.Q_cold / P.R, QR_warm = P.Q_warm / P.R;
   const int hh = lane >> 5, m = lane & (SG_NCELL - 1);
   for (int t0 = (bid * nwb + wv) * SG_FEAT_COLS; t0 < nsel; t0 += nblk * nwb * SG_FEAT_COLS) {
@@ -428,7 +630,6 @@ __device__ __forceinline__ void sg_features_block(const DevParams &P, const SgCo
       // read); the loads of SG_STAGE_BATCHES x 128 levels are all in flight
       // before the first is used
       const int ksmax = max(ksp[0], ksp[1]);
-      const double hci = 0.5 * P.inv_ci;
       for (int kb = 0; kb <= ksmax; kb += 128 * SG_STAGE_BATCHES) {
         d2u ea[SG_STAGE_BATCHES][2], ebv[SG_STAGE_BATCHES][2];
 #pragma unroll
@@ -448,17 +649,10 @@ __device__ __forceinline__ void sg_features_block(const DevParams &P, const SgCo
 #pragma unroll
           for (int h = 0; h < 2; h++) {
             if (k <= ksp[h]) {
-              if (TARGETS) {
-                d2u e;
-                e.x = 0.5 * (ea[bt][h].x + ebv[bt][h].x);
-                e.y = 0.5 * (ea[bt][h].y + ebv[bt][h].y);
-                *reinterpret_cast<d2u *>(ebw + h * Mzp + k) = e;
-              } else {
-                f2a y;
-                y.x = (float)__builtin_fma(ea[bt][h].x + ebv[bt][h].x, hci, y_off);
-                y.y = (float)__builtin_fma(ea[bt][h].y + ebv[bt][h].y, hci, y_off);
-                *reinterpret_cast<f2a *>(ebw + h * Mzp + k) = y;
-              }
+              d2u e;
+              e.x = 0.5 * (ea[bt][h].x + ebv[bt][h].x);
+              e.y = 0.5 * (ea[bt][h].y + ebv[bt][h].y);
+              *reinterpret_cast<d2u *>(ebw + h * Mzp + k) = e;
             }
           }
         }
@@ -479,15 +673,9 @@ __device__ __forceinline__ void sg_features_block(const DevParams &P, const SgCo
       int khi = __shfl_down(klo, 1, 64);
       if (m == SG_NCELL - 1) khi = ks + 1;
       if (!live) khi = 0;
-      const EType *eb = ebw + hh * Mzp;
+      const double *eb = ebw + hh * Mzp;
       double st = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0, qref = 0.0, dsum = 0.0, gs = 1.0;
-      if (!TARGETS) {
-        // W_ks = d_ks (dz_ks + d_ks) / 2 (SIAFD.cc:677-684 regrouped per delta_k)
-        const double zt = zsk[live ? ks : 0].x;
-        const double sk_top = 0.5 * ((ks >= 1 ? zt - zsk[ks - 1].x : 0.0) + (thk - zt));
-        sg_cell_sums<NP>(P, zsk, reinterpret_cast<const float *>(eb), klo, khi, ks, thk, sk_top,
-                         P.p_air + rhog * thk, rhog, x_cts, ta, tcap, g0, g1, g2, st);
-      } else {
+      {
         // G is accumulated relative to thk^2 p_base^n
         gs = live ? thk * thk * pow_n(P, P.p_air + rhog * thk) : 1.0;
         const double inv_gs = 1.0 / gs;
@@ -517,35 +705,25 @@ __device__ __forceinline__ void sg_features_block(const DevParams &P, const SgCo
           }
         }
       }
-      const double S_t = TARGETS ? half_sum(st, hh) * gs : half_total(st, hh);
+      const double S_t = half_sum(st, hh) * gs;
       const bool has = live && g0 > 0.0;
       double Qb = 0.0, feat = 2.0;
       if (has) {
-        const double inv_g0 = TARGETS ? 1.0 / g0 : rcp_nr(g0);
+        const double inv_g0 = 1.0 / g0;
         const double mx = g1 * inv_g0;
         const double Tbar = T_ref + mx;
         const double var = fmax(g2 * inv_g0 - mx * mx, 0.0);
-        const double inv_T = TARGETS ? 1.0 / Tbar : rcp_nr(Tbar);
+        const double inv_T = 1.0 / Tbar;
         const bool cold = Tbar < P.T_crit;
         const double QR = cold ? QR_cold : QR_warm;
         const double kk = QR * inv_T * inv_T;
         // Paterson-Budd softness at the cell mean (flowlaws.cc:90-95)
-        const double Ab = (cold ? P.A_cold : P.A_warm) * (TARGETS ? exp(-QR * inv_T) : exp_split(-QR * inv_T));
+        const double Ab = (cold ? P.A_cold : P.A_warm) * exp(-QR * inv_T);
         Qb = g0 * gs * Ab * (1.0 + 0.5 * (kk * kk - 2.0 * kk * inv_T) * var);
         feat = (Tbar - 250.0) * (1.0 / 15.0);
       }
       const int t = t0 + cc + hh;
-      if (!TARGETS) {
-        if (live) {
-          const int r = cc + hh;
-          oF[r * SG_FSTRIDE + m] = (float)feat;
-          oQ[r * SG_NCELL + m] = (float)(Cf * Qb);
-          if (m < SG_FSTRIDE - SG_NCELL) {
-            oF[r * SG_FSTRIDE + SG_NCELL + m] = m == 0 ? (float)(thk / 1000.0) : (m == 1 ? (float)lgs : 0.0f);
-          }
-          if (m == 0) oS[r] = Cf * S_t;
-        }
-      } else {
+      {
         const double qcell = qref * gs;  // exact cold sum of the cell
         const double Dref = Cf * (S_t + half_sum(dsum, hh) * gs);
         const unsigned long long hm = __ballot(has);
@@ -572,39 +750,29 @@ __device__ __forceinline__ void sg_features_block(const DevParams &P, const SgCo
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    if (!TARGETS) {
-      // rows t0 .. t0 + nr - 1 of F (36 floats), CQb (32 floats), CSt
-      const int nr = min(SG_FEAT_COLS, nsel - t0);
-      const float4 *sF = reinterpret_cast<const float4 *>(oF);
-      const float4 *sQ = reinterpret_cast<const float4 *>(oQ);
-      float4 *dF = reinterpret_cast<float4 *>(F.F + (long)t0 * SG_FSTRIDE);
-      float4 *dQ = reinterpret_cast<float4 *>(F.CQb + (long)t0 * SG_NCELL);
-      for (int v = lane; v < nr * (SG_FSTRIDE / 4); v += 64) dF[v] = sF[v];
-      for (int v = lane; v < nr * (SG_NCELL / 4); v += 64) dQ[v] = sQ[v];
-      if (lane < nr) F.CSt[t0 + lane] = oS[lane];
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
   }
 }
 
-// inference: at most 128 VGPRs (4 waves per SIMD; measured 2.69 -> 2.59 ms on
-// the 1501x2801x401 sheet against 3 waves with 3 load batches, a few spilled
-// registers outside the level loop); training samples: no cap
+// inference: at most 128 VGPRs (4 waves per SIMD); training samples: no cap
 template <bool TARGETS, int NP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(TARGETS ? 1 : 4)))
 void k_sg_features(DevParams P, SgColumnArgs C, SgFeatureArgs F) {
   extern __shared__ double2 zsk[];
   __shared__ int bucket[SG_NBUCKET];
-  sg_features_block<TARGETS, NP>(P, C, F, blockIdx.x, gridDim.x, zsk, bucket);
+  if constexpr (TARGETS) {
+    sg_sample_block(P, C, F, blockIdx.x, gridDim.x, zsk, bucket);
+  } else {
+    sg_infer_features_block<NP>(P, C, F, blockIdx.x, gridDim.x, zsk, bucket);
+  }
 }
 
-// dynamic LDS of k_sg_features: (z, half spacing) per level + per wavefront two
-// staged profiles (+ in inference the chunk's output rows)
+// dynamic LDS of k_sg_features: per level FP64 (z, half spacing) for the
+// training samples, FP64 z and FP32 (z, half spacing) in inference; per
+// wavefront two staged profiles (+ in inference the chunk's output rows)
 static size_t sg_features_lds(int Mz, int waves, bool targets) {
-  return sizeof(double2) * (size_t)Mz +
-         (size_t)waves * 2 * ((Mz + 1) & ~1) * (targets ? sizeof(double) : sizeof(float)) +
+  const size_t Mzp = (size_t)((Mz + 1) & ~1);
+  return (targets ? sizeof(double2) * (size_t)Mz : (sizeof(double) + sizeof(float2)) * Mzp) +
+         (size_t)waves * 2 * Mzp * (targets ? sizeof(double) : sizeof(float)) +
          (targets ? 0 : (size_t)waves * SG_FEAT_OUT * sizeof(float));
 }
 
@@ -928,7 +1096,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const int m = ens_member(start, nm, blockIdx.x);
   const SgColumnArgs C = global_view(T[m].C);
   const SgFeatureArgs F = global_view(T[m].F);
-  sg_features_block<false, NP>(T[m].P, C, F, blockIdx.x - start[m], start[m + 1] - start[m], zsk, bucket);
+  sg_infer_features_block<NP>(T[m].P, C, F, blockIdx.x - start[m], start[m + 1] - start[m], zsk, bucket);
 }
 
 __global__ __launch_bounds__(64 * SG_MLP_WAVES) void k_ens_sg_mlp(const EnsSgMember *__restrict__ T,
