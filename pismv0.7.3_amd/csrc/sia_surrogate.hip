@@ -251,6 +251,8 @@ constexpr int SG_FEAT_OUT = SG_FEAT_COLS * (SG_FSTRIDE + SG_NCELL) + 2 * SG_FEAT
 
 // buckets of the level search: bucket b holds the first level with z >= b Lz / SG_NBUCKET
 constexpr int SG_NBUCKET = 256;
+// flag beside ks of an inference entry: the top level lies below the surface
+constexpr int SG_TOP_POS = 1 << 16;
 
 // first level k in [0, ks] with z_k * s32 >= m (ks + 1 if none), i.e.
 // cell_of(z_k) >= m for m <= 31: start at the bucket of height m / s32 and
@@ -292,6 +294,16 @@ __device__ __forceinline__ float pownf(float p) {
 
 __device__ __forceinline__ float lane_value_f(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// v of the lane whose byte address (4 x lane) is src, per lane (ds_bpermute)
+__device__ __forceinline__ float lane_get_f(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(v)));
+}
+__device__ __forceinline__ double lane_get_d(double v, int src) {
+  const int lo = __builtin_amdgcn_ds_bpermute(src, __double2loint(v));
+  const int hi = __builtin_amdgcn_ds_bpermute(src, __double2hiint(v));
+  return __hiloint2double(hi, lo);
 }
 
 // sum of v over the 32 lanes of this lane's half wave (FP32, five DPP steps)
@@ -388,10 +400,11 @@ __device__ __forceinline__ void sg_infer_features_block(const DevParams &P, cons
   const float Ac = (float)P.A_cold, Aw = (float)P.A_warm, Tcrit = (float)P.T_crit;
   const int hh = lane >> 5, m = lane & (SG_NCELL - 1);
   for (int t0 = (bid * nwb + wv) * SG_FEAT_COLS; t0 < n; t0 += nblk * nwb * SG_FEAT_COLS) {
-    // ---- setup: lane c < 16 prepares entry t0 + c (c_ks = -1: no entry)
-    int c_ks = -1, c_q = 0, c_q2 = 0;
-    double c_thk = 0.0, c_s32 = 0.0, c_cs = 0.0;
-    float c_lgs = 0.0f, c_cz = 0.0f, c_pa = 0.0f, c_bp = 0.0f;
+    // ---- setup: lane c < 16 prepares entry t0 + c (c_kt = -1: no entry)
+    // c_kt: ks, plus SG_TOP_POS when z_ks < thk (d_ks > 0 in FP64)
+    int c_kt = -1, c_q = 0, c_q2 = 0;
+    double c_s32 = 0.0, c_cs = 0.0;
+    float c_thk = 0.0f, c_fthk = 0.0f, c_lgs = 0.0f, c_cz = 0.0f, c_pa = 0.0f, c_bp = 0.0f;
     if (lane < SG_FEAT_COLS && t0 + lane < n) {
       const int s = F.list[t0 + lane];
       const int q = s >> 1, o = s & 1;
@@ -401,40 +414,42 @@ __device__ __forceinline__ void sg_infer_features_block(const DevParams &P, cons
       const double hxs = C.hx[s], hys = C.hy[s];
       const double alpha = sqrt(hxs * hxs + hys * hys);
       c_lgs = (float)(log10(alpha) + 2.0);  // the slope input of the network
+      c_fthk = (float)(thk / 1000.0);       // the thickness input
       const double pb = P.p_air + rhog * thk;  // basal pressure
       c_cs = P.e * (0.5 * (C.theta[q] + C.theta[q2])) * 2.0 * ipow_nm1(P, alpha) * pown<NP>(pb);
       c_cz = (float)(rhog / pb);
       c_pa = (float)(P.p_air / pb);
       c_bp = (float)(P.beta * pb);
       c_s32 = (double)SG_NCELL / thk;
-      c_ks = kbelow_bucket(zd, bucket, bscale, Mz, P.Lz, thk);  // >= 1 for listed columns
-      c_thk = thk;
+      const int ks = kbelow_bucket(zd, bucket, bscale, Mz, P.Lz, thk);  // >= 1 for listed columns
+      c_kt = ks | ((zd[ks] < thk) ? SG_TOP_POS : 0);
+      c_thk = (float)thk;
       c_q = q;
       c_q2 = q2;
     }
     for (int cc = 0; cc < SG_FEAT_COLS; cc += 2) {
-      if (lane_value_i(c_ks, cc) < 0) break;  // entries are consecutive: nothing left
+      const int kt0 = lane_value_i(c_kt, cc), kt1 = lane_value_i(c_kt, cc + 1);
+      if (kt0 < 0) break;  // entries are consecutive: nothing left
       // ---- stage y of columns cc, cc + 1: lane l holds levels k, k + 1 with
-      // k = kb + 128 b + 2 l (16-byte loads; k <= ks <= Mz - 2 keeps k + 1
-      // inside the column)
-      int ksp[2];
+      // k = kb + 128 b + 2 l, up to the larger ks of the two (16-byte loads;
+      // k <= ks <= Mz - 2 keeps k + 1 inside the column)
+      const int ks0 = kt0 & (SG_TOP_POS - 1), ks1 = kt1 < 0 ? -1 : (kt1 & (SG_TOP_POS - 1));
       const double *Ep[2][2];
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        ksp[h] = lane_value_i(c_ks, cc + h);
-        Ep[h][0] = C.E + (long)lane_value_i(c_q, cc + h) * Mz;
-        Ep[h][1] = C.E + (long)lane_value_i(c_q2, cc + h) * Mz;
-      }
-      const int ksmax = max(ksp[0], ksp[1]);
+      Ep[0][0] = C.E + (long)lane_value_i(c_q, cc) * Mz;
+      Ep[0][1] = C.E + (long)lane_value_i(c_q2, cc) * Mz;
+      // a missing second entry stages the first column twice
+      Ep[1][0] = kt1 < 0 ? Ep[0][0] : C.E + (long)lane_value_i(c_q, cc + 1) * Mz;
+      Ep[1][1] = kt1 < 0 ? Ep[0][1] : C.E + (long)lane_value_i(c_q2, cc + 1) * Mz;
+      const int ksmax = max(ks0, ks1);
       const double hci = 0.5 * P.inv_ci;
       for (int kb = 0; kb <= ksmax; kb += 128 * SG_STAGE_BATCHES) {
         d2u ea[SG_STAGE_BATCHES][2], ebv[SG_STAGE_BATCHES][2];
 #pragma unroll
         for (int bt = 0; bt < SG_STAGE_BATCHES; bt++) {
           const int k = kb + 128 * bt + 2 * lane;
+          if (k <= ksmax) {
 #pragma unroll
-          for (int h = 0; h < 2; h++) {
-            if (k <= ksp[h]) {
+            for (int h = 0; h < 2; h++) {
               ea[bt][h] = *reinterpret_cast<const d2u *>(Ep[h][0] + k);
               ebv[bt][h] = *reinterpret_cast<const d2u *>(Ep[h][1] + k);
             }
@@ -443,9 +458,9 @@ __device__ __forceinline__ void sg_infer_features_block(const DevParams &P, cons
 #pragma unroll
         for (int bt = 0; bt < SG_STAGE_BATCHES; bt++) {
           const int k = kb + 128 * bt + 2 * lane;
+          if (k <= ksmax) {
 #pragma unroll
-          for (int h = 0; h < 2; h++) {
-            if (k <= ksp[h]) {
+            for (int h = 0; h < 2; h++) {
               f2a y;
               y.x = (float)__builtin_fma(ea[bt][h].x + ebv[bt][h].x, hci, y_off);
               y.y = (float)__builtin_fma(ea[bt][h].y + ebv[bt][h].y, hci, y_off);
@@ -458,15 +473,17 @@ __device__ __forceinline__ void sg_infer_features_block(const DevParams &P, cons
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       // ---- half wave hh: column cc + hh, lane m: zeta-cell m
-      const int ks = hh ? ksp[1] : ksp[0];
+      const int src = (cc + hh) * 4;
+      const int ks = hh ? ks1 : ks0;
       const bool live = ks >= 0;
-      const double thk = hh ? lane_value_d(c_thk, cc + 1) : lane_value_d(c_thk, cc);
-      const double s32 = hh ? lane_value_d(c_s32, cc + 1) : lane_value_d(c_s32, cc);
+      const bool top_pos = ((hh ? kt1 : kt0) & SG_TOP_POS) != 0;
+      const double s32 = lane_get_d(c_s32, src);
+      const float thkf = lane_get_f(c_thk, src);
       // cell m: levels [klo, khi), the first level with z_k s32 >= m from the
       // bucket of height m thk / 32; khi is the next lane's klo
       int klo = 0;
       if (live) {
-        int b = (int)((float)m * (float)thk * (float)(bscale / SG_NCELL));
+        int b = (int)((float)m * thkf * (float)(bscale / SG_NCELL));
         b = b > SG_NBUCKET - 1 ? SG_NBUCKET - 1 : b;
         int k = min(bucket[b], ks + 1);
         while (k > 0 && zd[k - 1] * s32 >= (double)m) k--;
@@ -476,10 +493,9 @@ __device__ __forceinline__ void sg_infer_features_block(const DevParams &P, cons
       int khi = __shfl_down(klo, 1, 64);
       if (m == SG_NCELL - 1) khi = ks + 1;
       if (!live) khi = 0;
-      const float thkf = (float)thk;
-      const float cz = hh ? lane_value_f(c_cz, cc + 1) : lane_value_f(c_cz, cc);
-      const float pa = hh ? lane_value_f(c_pa, cc + 1) : lane_value_f(c_pa, cc);
-      const float bp = hh ? lane_value_f(c_bp, cc + 1) : lane_value_f(c_bp, cc);
+      const float cz = lane_get_f(c_cz, src);
+      const float pa = lane_get_f(c_pa, src);
+      const float bp = lane_get_f(c_bp, src);
       // W_ks = d_ks (dz_ks + d_ks) / 2
       const int kt = live ? ks : 0;
       const float skt = 0.5f * ((kt >= 1 ? zf[kt].x - zf[kt - 1].x : 0.0f) + (thkf - zf[kt].x));
@@ -501,7 +517,7 @@ __device__ __forceinline__ void sg_infer_features_block(const DevParams &P, cons
       }
       // a cold top level with d = thk - z_ks > 0 whose G underflows in FP32 still
       // populates its cell (the FP64 pass sees G > 0 of negligible weight)
-      if (live && klo <= ks && ks < khi && g0 == 0.0f && zd[ks] < thk) {
+      if (live && top_pos && klo <= ks && ks < khi && g0 == 0.0f) {
         const float x = __builtin_fmaf(bp, __builtin_fmaf(cz, thkf - zf[ks].x, pa), yb[ks]);
         if (x < x_cts) {
           g0 = 1.0e-30f;
@@ -526,15 +542,13 @@ __device__ __forceinline__ void sg_infer_features_block(const DevParams &P, cons
         Qh = g0 * Ab * __builtin_fmaf(0.5f * (kk * kk - 2.0f * kk * iT), var, 1.0f);
         feat = (Tbar - 250.0f) * (1.0f / 15.0f);
       }
+      const double cs = lane_get_d(c_cs, src);
+      const float fthk = lane_get_f(c_fthk, src), lgs = lane_get_f(c_lgs, src);
       if (live) {
         const int r = cc + hh;
-        const double cs = hh ? lane_value_d(c_cs, cc + 1) : lane_value_d(c_cs, cc);
         oF[r * SG_FSTRIDE + m] = feat;
         oQ[r * SG_NCELL + m] = (float)(cs * (double)Qh);
-        if (m < SG_FSTRIDE - SG_NCELL) {
-          const float lgs = hh ? lane_value_f(c_lgs, cc + 1) : lane_value_f(c_lgs, cc);
-          oF[r * SG_FSTRIDE + SG_NCELL + m] = m == 0 ? (float)(thk / 1000.0) : (m == 1 ? lgs : 0.0f);
-        }
+        if (m < SG_FSTRIDE - SG_NCELL) oF[r * SG_FSTRIDE + SG_NCELL + m] = m == 0 ? fthk : (m == 1 ? lgs : 0.0f);
         if (m == 0) oS[r] = cs * (double)S_t;
       }
       // the next pair overwrites the enthalpy buffers
