@@ -246,8 +246,6 @@ constexpr int SG_FEAT_COLS = 16;
 // 128-level batches of a pair's enthalpy loads issued together (two: with the
 // loads in flight the inference pass fits 128 VGPRs, 4 waves per SIMD)
 constexpr int SG_STAGE_BATCHES = 2;
-// floats of a wavefront's output rows: F [16][36], CQb [16][32], CSt [16] (FP64)
-constexpr int SG_FEAT_OUT = SG_FEAT_COLS * (SG_FSTRIDE + SG_NCELL) + 2 * SG_FEAT_COLS;
 
 // buckets of the level search: bucket b holds the first level with z >= b Lz / SG_NBUCKET
 constexpr int SG_NBUCKET = 256;
@@ -317,7 +315,14 @@ __device__ __forceinline__ float half_total_f(float v, int half) {
   v += sg_dppf<0x114, 0xf, true>(v);
   v += sg_dppf<0x118, 0xf, true>(v);
   v += sg_dppf<0x142, 0xa, false>(v);
-  return half ? lane_value_f(v, 63) : lane_value_f(v, 31);
+  const float a = lane_value_f(v, 31), b = lane_value_f(v, 63);
+  return half ? b : a;
+}
+
+// a wave-uniform value pinned where it is computed (the compiler would
+// otherwise fold selects of such constants into per-lane FP64 divisions)
+__device__ __forceinline__ float pinned(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
 // 1 / x: hardware estimate refined by one Newton step
@@ -384,20 +389,18 @@ __device__ __forceinline__ void sg_infer_features_block(const DevParams &P, cons
   __syncthreads();
   const int nwb = blockDim.x >> 6;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  float *ebw = reinterpret_cast<float *>(zf + Mzp) + (long)wv * 2 * Mzp;
-  float *oF = reinterpret_cast<float *>(zf + Mzp) + (long)nwb * 2 * Mzp + (long)wv * SG_FEAT_OUT;
-  float *oQ = oF + SG_FEAT_COLS * SG_FSTRIDE;
-  double *oS = reinterpret_cast<double *>(oQ + SG_FEAT_COLS * SG_NCELL);
+  float *ebw = reinterpret_cast<float *>(zf + Mzp) + (long)wv * 4 * Mzp;  // two buffers of a pair's profiles
   const int n = *F.n_list;
   const double rhog = P.rho_i * P.g;
   const float T_ref = 263.15f;
   const double y_off = P.T0 - 263.15;
-  const float x_cts = (float)(P.T_melt - 263.15);
+  const float x_cts = pinned((float)(P.T_melt - 263.15));
   // temperate softness A_T0 + ta min(x - x_cts, tcap), ta = A_T0 wf_coeff c_i / L, tcap = wf_limit L / c_i
   const double ta = P.A_T0 * P.wf_coeff * P.ci * P.inv_L, tcap = P.wf_limit / (P.ci * P.inv_L);
-  const float t_a = (float)ta, t_b = (float)(P.A_T0 - ta * (P.T_melt - 263.15)), t_cap = (float)(P.A_T0 + ta * tcap);
-  const float QRc = (float)(P.Q_cold / P.R), QRw = (float)(P.Q_warm / P.R);
-  const float Ac = (float)P.A_cold, Aw = (float)P.A_warm, Tcrit = (float)P.T_crit;
+  const float t_a = pinned((float)ta), t_b = pinned((float)(P.A_T0 - ta * (P.T_melt - 263.15)));
+  const float t_cap = pinned((float)(P.A_T0 + ta * tcap));
+  const float QRc = pinned((float)(P.Q_cold / P.R)), QRw = pinned((float)(P.Q_warm / P.R));
+  const float Ac = pinned((float)P.A_cold), Aw = pinned((float)P.A_warm), Tcrit = pinned((float)P.T_crit);
   const int hh = lane >> 5, m = lane & (SG_NCELL - 1);
   for (int t0 = (bid * nwb + wv) * SG_FEAT_COLS; t0 < n; t0 += nblk * nwb * SG_FEAT_COLS) {
     // ---- setup: lane c < 16 prepares entry t0 + c (c_kt = -1: no entry)
@@ -417,8 +420,9 @@ __device__ __forceinline__ void sg_infer_features_block(const DevParams &P, cons
       c_fthk = (float)(thk / 1000.0);       // the thickness input
       const double pb = P.p_air + rhog * thk;  // basal pressure
       c_cs = P.e * (0.5 * (C.theta[q] + C.theta[q2])) * 2.0 * ipow_nm1(P, alpha) * pown<NP>(pb);
-      c_cz = (float)(rhog / pb);
-      c_pa = (float)(P.p_air / pb);
+      const double ipb = 1.0 / pb;
+      c_cz = (float)(rhog * ipb);
+      c_pa = (float)(P.p_air * ipb);
       c_bp = (float)(P.beta * pb);
       c_s32 = (double)SG_NCELL / thk;
       const int ks = kbelow_bucket(zd, bucket, bscale, Mz, P.Lz, thk);  // >= 1 for listed columns
@@ -427,51 +431,80 @@ __device__ __forceinline__ void sg_infer_features_block(const DevParams &P, cons
       c_q = q;
       c_q2 = q2;
     }
-    for (int cc = 0; cc < SG_FEAT_COLS; cc += 2) {
-      const int kt0 = lane_value_i(c_kt, cc), kt1 = lane_value_i(c_kt, cc + 1);
-      if (kt0 < 0) break;  // entries are consecutive: nothing left
-      // ---- stage y of columns cc, cc + 1: lane l holds levels k, k + 1 with
-      // k = kb + 128 b + 2 l, up to the larger ks of the two (16-byte loads;
-      // k <= ks <= Mz - 2 keeps k + 1 inside the column)
-      const int ks0 = kt0 & (SG_TOP_POS - 1), ks1 = kt1 < 0 ? -1 : (kt1 & (SG_TOP_POS - 1));
-      const double *Ep[2][2];
-      Ep[0][0] = C.E + (long)lane_value_i(c_q, cc) * Mz;
-      Ep[0][1] = C.E + (long)lane_value_i(c_q2, cc) * Mz;
+    // ---- the chunk's pairs (entries 2 pp, 2 pp + 1), software-pipelined: the
+    // loads of pair pp + 1 (its first SG_STAGE_BATCHES x 128 levels) are in
+    // flight while pair pp walks its levels; they are converted into the other
+    // profile buffer before pair pp's epilogue. Lane l stages levels k, k + 1
+    // with k = kb + 128 b + 2 l, up to the larger ks of the pair (16-byte loads;
+    // k <= ks <= Mz - 2 keeps k + 1 inside the column).
+    const int npair = (min(SG_FEAT_COLS, n - t0) + 1) / 2;
+    const double hci = 0.5 * P.inv_ci;
+    struct PairSrc {
+      const double *E[2][2];
+      int kt0, kt1, ksmax;
+    };
+    auto pair_src = [&](int pp) {
+      PairSrc r;
+      const int c = 2 * pp;
+      r.kt0 = lane_value_i(c_kt, c);
+      r.kt1 = lane_value_i(c_kt, c + 1);
+      const int ks0 = r.kt0 & (SG_TOP_POS - 1), ks1 = r.kt1 < 0 ? -1 : (r.kt1 & (SG_TOP_POS - 1));
+      r.ksmax = max(ks0, ks1);
+      r.E[0][0] = C.E + (long)lane_value_i(c_q, c) * Mz;
+      r.E[0][1] = C.E + (long)lane_value_i(c_q2, c) * Mz;
       // a missing second entry stages the first column twice
-      Ep[1][0] = kt1 < 0 ? Ep[0][0] : C.E + (long)lane_value_i(c_q, cc + 1) * Mz;
-      Ep[1][1] = kt1 < 0 ? Ep[0][1] : C.E + (long)lane_value_i(c_q2, cc + 1) * Mz;
-      const int ksmax = max(ks0, ks1);
-      const double hci = 0.5 * P.inv_ci;
-      for (int kb = 0; kb <= ksmax; kb += 128 * SG_STAGE_BATCHES) {
-        d2u ea[SG_STAGE_BATCHES][2], ebv[SG_STAGE_BATCHES][2];
+      r.E[1][0] = r.kt1 < 0 ? r.E[0][0] : C.E + (long)lane_value_i(c_q, c + 1) * Mz;
+      r.E[1][1] = r.kt1 < 0 ? r.E[0][1] : C.E + (long)lane_value_i(c_q2, c + 1) * Mz;
+      return r;
+    };
+    d2u ea[SG_STAGE_BATCHES][2], ebv[SG_STAGE_BATCHES][2];
+    auto issue = [&](const PairSrc &r, int kb) {
 #pragma unroll
-        for (int bt = 0; bt < SG_STAGE_BATCHES; bt++) {
-          const int k = kb + 128 * bt + 2 * lane;
-          if (k <= ksmax) {
+      for (int bt = 0; bt < SG_STAGE_BATCHES; bt++) {
+        const int k = kb + 128 * bt + 2 * lane;
+        if (k <= r.ksmax) {
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-              ea[bt][h] = *reinterpret_cast<const d2u *>(Ep[h][0] + k);
-              ebv[bt][h] = *reinterpret_cast<const d2u *>(Ep[h][1] + k);
-            }
-          }
-        }
-#pragma unroll
-        for (int bt = 0; bt < SG_STAGE_BATCHES; bt++) {
-          const int k = kb + 128 * bt + 2 * lane;
-          if (k <= ksmax) {
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-              f2a y;
-              y.x = (float)__builtin_fma(ea[bt][h].x + ebv[bt][h].x, hci, y_off);
-              y.y = (float)__builtin_fma(ea[bt][h].y + ebv[bt][h].y, hci, y_off);
-              *reinterpret_cast<f2a *>(ebw + h * Mzp + k) = y;
-            }
+          for (int h = 0; h < 2; h++) {
+            ea[bt][h] = *reinterpret_cast<const d2u *>(r.E[h][0] + k);
+            ebv[bt][h] = *reinterpret_cast<const d2u *>(r.E[h][1] + k);
           }
         }
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    auto convert = [&](const PairSrc &r, int kb, float *buf) {
+#pragma unroll
+      for (int bt = 0; bt < SG_STAGE_BATCHES; bt++) {
+        const int k = kb + 128 * bt + 2 * lane;
+        if (k <= r.ksmax) {
+#pragma unroll
+          for (int h = 0; h < 2; h++) {
+            f2a y;
+            y.x = (float)__builtin_fma(ea[bt][h].x + ebv[bt][h].x, hci, y_off);
+            y.y = (float)__builtin_fma(ea[bt][h].y + ebv[bt][h].y, hci, y_off);
+            *reinterpret_cast<f2a *>(buf + h * Mzp + k) = y;
+          }
+        }
+      }
+    };
+    PairSrc cur = pair_src(0);
+    for (int kb = 0; kb <= cur.ksmax; kb += 128 * SG_STAGE_BATCHES) {
+      issue(cur, kb);
+      convert(cur, kb, ebw);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int pp = 0; pp < npair; pp++) {
+      const int cc = 2 * pp;
+      const int kt0 = cur.kt0, kt1 = cur.kt1;
+      const int ks0 = kt0 & (SG_TOP_POS - 1), ks1 = kt1 < 0 ? -1 : (kt1 & (SG_TOP_POS - 1));
+      float *bcur = ebw + (pp & 1) * 2 * Mzp, *bnext = ebw + ((pp + 1) & 1) * 2 * Mzp;
+      const bool more = pp + 1 < npair;
+      PairSrc nxt = cur;
+      if (more) {
+        nxt = pair_src(pp + 1);
+        issue(nxt, 0);
+      }
       // ---- half wave hh: column cc + hh, lane m: zeta-cell m
       const int src = (cc + hh) * 4;
       const int ks = hh ? ks1 : ks0;
@@ -499,7 +532,7 @@ __device__ __forceinline__ void sg_infer_features_block(const DevParams &P, cons
       // W_ks = d_ks (dz_ks + d_ks) / 2
       const int kt = live ? ks : 0;
       const float skt = 0.5f * ((kt >= 1 ? zf[kt].x - zf[kt - 1].x : 0.0f) + (thkf - zf[kt].x));
-      const float *yb = ebw + hh * Mzp;
+      const float *yb = bcur + hh * Mzp;
       float g0 = 0.0f, g1 = 0.0f, g2 = 0.0f, st = 0.0f;
 #pragma unroll 2
       for (int k = klo; k < khi; k++) {
@@ -525,6 +558,14 @@ __device__ __forceinline__ void sg_infer_features_block(const DevParams &P, cons
           g2 = g1 * x;
         }
       }
+      // pair pp + 1 into the other buffer (its levels beyond the prefetch too)
+      if (more) {
+        convert(nxt, 0, bnext);
+        for (int kb = 128 * SG_STAGE_BATCHES; kb <= nxt.ksmax; kb += 128 * SG_STAGE_BATCHES) {
+          issue(nxt, kb);
+          convert(nxt, kb, bnext);
+        }
+      }
       const float S_t = half_total_f(st, hh);
       const bool has = live && g0 > 0.0f;
       float Qh = 0.0f, feat = 2.0f;
@@ -544,30 +585,19 @@ __device__ __forceinline__ void sg_infer_features_block(const DevParams &P, cons
       }
       const double cs = lane_get_d(c_cs, src);
       const float fthk = lane_get_f(c_fthk, src), lgs = lane_get_f(c_lgs, src);
-      if (live) {
-        const int r = cc + hh;
-        oF[r * SG_FSTRIDE + m] = feat;
-        oQ[r * SG_NCELL + m] = (float)(cs * (double)Qh);
-        if (m < SG_FSTRIDE - SG_NCELL) oF[r * SG_FSTRIDE + SG_NCELL + m] = m == 0 ? fthk : (m == 1 ? lgs : 0.0f);
-        if (m == 0) oS[r] = cs * (double)S_t;
+      if (live) {  // row t0 + cc + hh: F (36 floats), CQb (32 floats), CSt
+        const long r = t0 + cc + hh;
+        F.F[r * SG_FSTRIDE + m] = feat;
+        F.CQb[r * SG_NCELL + m] = (float)(cs * (double)Qh);
+        if (m < SG_FSTRIDE - SG_NCELL) F.F[r * SG_FSTRIDE + SG_NCELL + m] = m == 0 ? fthk : (m == 1 ? lgs : 0.0f);
+        if (m == 0) F.CSt[r] = cs * (double)S_t;
       }
-      // the next pair overwrites the enthalpy buffers
+      // pair pp + 1's profiles are complete; pair pp + 2 overwrites this pair's
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      cur = nxt;
     }
-    // rows t0 .. t0 + nr - 1 of F (36 floats), CQb (32 floats), CSt
-    const int nr = min(SG_FEAT_COLS, n - t0);
-    const float4 *sF = reinterpret_cast<const float4 *>(oF);
-    const float4 *sQ = reinterpret_cast<const float4 *>(oQ);
-    float4 *dF = reinterpret_cast<float4 *>(F.F + (long)t0 * SG_FSTRIDE);
-    float4 *dQ = reinterpret_cast<float4 *>(F.CQb + (long)t0 * SG_NCELL);
-    for (int v = lane; v < nr * (SG_FSTRIDE / 4); v += 64) dF[v] = sF[v];
-    for (int v = lane; v < nr * (SG_NCELL / 4); v += 64) dQ[v] = sQ[v];
-    if (lane < nr) F.CSt[t0 + lane] = oS[lane];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 
@@ -782,12 +812,12 @@ void k_sg_features(DevParams P, SgColumnArgs C, SgFeatureArgs F) {
 
 // dynamic LDS of k_sg_features: per level FP64 (z, half spacing) for the
 // training samples, FP64 z and FP32 (z, half spacing) in inference; per
-// wavefront two staged profiles (+ in inference the chunk's output rows)
+// wavefront the staged profiles of a pair of columns (FP64), in inference two
+// such buffers (FP32)
 static size_t sg_features_lds(int Mz, int waves, bool targets) {
   const size_t Mzp = (size_t)((Mz + 1) & ~1);
   return (targets ? sizeof(double2) * (size_t)Mz : (sizeof(double) + sizeof(float2)) * Mzp) +
-         (size_t)waves * 2 * Mzp * (targets ? sizeof(double) : sizeof(float)) +
-         (targets ? 0 : (size_t)waves * SG_FEAT_OUT * sizeof(float));
+         (size_t)waves * 2 * Mzp * (targets ? sizeof(double) : 2 * sizeof(float));
 }
 
 void launch_sg_features(const DevParams &P, const SgColumnArgs &C, const SgFeatureArgs &F, bool targets,
