@@ -56,6 +56,14 @@ __device__ __forceinline__ int wrapi(int i, int M) {
   return i < 0 ? i + M : (i >= M ? i - M : i);
 }
 
+// (i, j) of map point q = i My + j; grids hold fewer than 2^31 points, so the
+// division is 32-bit
+__device__ __forceinline__ void map_ij(long q, int My, int &i, int &j) {
+  const unsigned u = (unsigned)q;
+  i = (int)(u / (unsigned)My);
+  j = (int)(u - (unsigned)i * (unsigned)My);
+}
+
 // IceModelVec2Int::as_int (IceModelVec_inline.hh:94-100)
 __device__ __forceinline__ int as_int(double a) { return (int)floor(a + 0.5); }
 

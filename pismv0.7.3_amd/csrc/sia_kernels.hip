@@ -127,7 +127,8 @@ __global__ void k_smooth_mean(DevParams P, const double *__restrict__ b0, double
   const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int Mx = P.Mx, My = P.My;
   if (q >= (long)Mx * My) return;
-  const int i = (int)(q / My), j = (int)(q % My);
+  int i, j;
+  map_ij(q, My, i, j);
   double sum = 0.0, count = 0.0;
   for (int r = -P.Nx; r <= P.Nx; r++) {
     for (int s = -P.Ny; s <= P.Ny; s++) {
@@ -148,7 +149,8 @@ __global__ void k_smooth_coeffs(DevParams P, double s2, double s3, double s4,
   const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int Mx = P.Mx, My = P.My;
   if (q >= (long)Mx * My) return;
-  const int i = (int)(q / My), j = (int)(q % My);
+  int i, j;
+  map_ij(q, My, i, j);
   const double topgs = bs[q];
   double maxtltemp = 0.0, sum2 = 0.0, sum3 = 0.0, sum4 = 0.0, count = 0.0;
   for (int r = -P.Nx; r <= P.Nx; r++) {
@@ -207,7 +209,8 @@ __device__ __forceinline__ void haseloff_y(const DevParams &P, const Fields2D &F
 // surface gradient at map point q on both staggered grids
 __device__ __forceinline__ void gradient_point(const DevParams &P, const Fields2D &F, double *__restrict__ hx,
                                                double *__restrict__ hy, long q) {
-  const int i = (int)(q / P.My), j = (int)(q % P.My);
+  int i, j;
+  map_ij(q, P.My, i, j);
   const double dx = P.dx, dy = P.dy;
   double hx0, hy0, hx1, hy1;
   if (P.grad == 1) {  // mahaffy, SIAFD.cc:305-316
@@ -310,7 +313,8 @@ __global__ void k_regional_gradient(DevParams P, const double *__restrict__ nmm,
   const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int Mx = P.Mx, My = P.My;
   if (q >= (long)Mx * My) return;
-  const int i = (int)(q / My), j = (int)(q % My);
+  int i, j;
+  map_ij(q, My, i, j);
 #define NM(a, c) nmm[idx2(P, (a), (c))]
 #define HS(a, c) hst[idx2(P, (a), (c))]
   if (NM(i, j) > 0.5 || NM(i + 1, j) > 0.5) {
@@ -894,7 +898,8 @@ __global__ void k_surface_values(DevParams P, const double *__restrict__ z, cons
 __global__ void k_diffusivity_regular(DevParams P, const double *__restrict__ D, double *__restrict__ out) {
   const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (long)P.Mx * P.My) return;
-  const int i = (int)(q / P.My), j = (int)(q % P.My);
+  int i, j;
+  map_ij(q, P.My, i, j);
   out[q] = 0.25 * (D[2 * q] + D[2 * q + 1] + D[2 * idx2(P, i, j - 1) + 1] + D[2 * idx2(P, i - 1, j)]);
 }
 
@@ -910,7 +915,8 @@ __global__ void k_sliding_gradient(DevParams P, SlideArgs A) {
     gradient_point(P, A.F, A.hx, A.hy, q);
     return;
   }
-  const int i = (int)(q / P.My), j = (int)(q % P.My);
+  int i, j;
+  map_ij(q, P.My, i, j);
 #define HH(a, c) A.F.H[idx2(P, (a), (c))]
 #define BB(a, c) A.F.b[idx2(P, (a), (c))]
 #define SS(a, c) A.F.h[idx2(P, (a), (c))]
@@ -1002,7 +1008,8 @@ __global__ void k_sia_sliding(DevParams P, SlideArgs A) {
     A.heat[q] = 0.0;
     return;
   }
-  const int i = (int)(q / P.My), j = (int)(q % P.My);
+  int i, j;
+  map_ij(q, P.My, i, j);
   const long qw = idx2(P, i - 1, j), qs = idx2(P, i, j - 1);
   const double myhx = 0.25 * (A.hx[2 * q] + A.hx[2 * qw] + A.hx[2 * q + 1] + A.hx[2 * qs + 1]),
                myhy = 0.25 * (A.hy[2 * q] + A.hy[2 * qw] + A.hy[2 * q + 1] + A.hy[2 * qs + 1]);
@@ -1081,13 +1088,14 @@ __device__ __forceinline__ void adjust_flow_dir(int c, int nb, double &ssa, doub
   if (m_ice_free_ocean(c) && m_ice_free_ocean(nb)) { sia = 0.0; ssa = 0.0; return; }
 }
 
-// block bid (256 map points) of the mass-continuity grid; red: LDS [8][256]
-__device__ __forceinline__ void mass_block(const DevParams &P, const MassArgs &A, long bid, double (*red)[256]) {
+// block bid (256 map points) of the mass-continuity grid; red: LDS [8][4]
+__device__ __forceinline__ void mass_block(const DevParams &P, const MassArgs &A, long bid, double (*red)[4]) {
   const long q = bid * 256 + threadIdx.x;
   const int Mx = P.Mx, My = P.My;
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (q < (long)Mx * My) {
-    const int i = (int)(q / My), j = (int)(q % My);
+    int i, j;
+    map_ij(q, My, i, j);
     const double dt = *A.dt;
     const double dx = P.dx, dy = P.dy, ice_density = P.rho_i;
     const double cell_area = dx * dy;
@@ -1138,25 +1146,23 @@ __device__ __forceinline__ void mass_block(const DevParams &P, const MassArgs &A
     acc[4] = -divQ_SIA * meter_per_s_to_kg;
     acc[5] = -divQ_SSA * meter_per_s_to_kg;
   }
-  // fixed-order block reduction of the accounting terms
+  // fixed-order block reduction of the accounting terms: a DPP scan in each
+  // wavefront, then the four wavefront totals in order
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-  for (int c = 0; c < 8; c++) red[c][threadIdx.x] = acc[c];
-  __syncthreads();
-  for (int off = blockDim.x >> 1; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) {
-#pragma unroll
-      for (int c = 0; c < 8; c++) red[c][threadIdx.x] += red[c][threadIdx.x + off];
-    }
-    __syncthreads();
+  for (int c = 0; c < 8; c++) {
+    const double t = wave_inclusive_scan(acc[c]);
+    if (lane == 63) red[c][wv] = t;
   }
-  if (threadIdx.x == 0 && A.partials) {
-#pragma unroll
-    for (int c = 0; c < 8; c++) A.partials[bid * 8 + c] = red[c][0];
+  __syncthreads();
+  if (threadIdx.x < 8 && A.partials) {
+    const int c = threadIdx.x;
+    A.partials[bid * 8 + c] = ((red[c][0] + red[c][1]) + red[c][2]) + red[c][3];
   }
 }
 
 __global__ __launch_bounds__(256) void k_mass_continuity(DevParams P, MassArgs A) {
-  __shared__ double red[8][256];
+  __shared__ double red[8][4];
   mass_block(P, A, blockIdx.x, red);
 }
 
@@ -1444,7 +1450,7 @@ __global__ __launch_bounds__(1024) void k_ens_dt(const EnsMember *__restrict__ T
 
 __global__ __launch_bounds__(256) void k_ens_mass(const EnsMember *__restrict__ T, const int *__restrict__ start,
                                                   int nm) {
-  __shared__ double red[8][256];
+  __shared__ double red[8][4];
   const int m = ens_member(start, nm, blockIdx.x);
   const MassArgs M = global_view(T[m].M);
   mass_block(T[m].P, M, blockIdx.x - start[m], red);
