@@ -144,6 +144,8 @@ __device__ __forceinline__ void sg_scan_block(int nb, const SgListArgs &L, long 
   const int per = (nb + 1023) / 1024;
   const int b0 = t * per, b1 = min(nb, b0 + per);
   long long s = 0, z = 0, f = 0;
+  // unrolled: the loads of 8 blocks are in flight together
+#pragma unroll 8
   for (int b = b0; b < b1; b++) {
     z += L.block_count[3 * b + 0];
     s += L.block_count[3 * b + 1];
@@ -164,6 +166,7 @@ __device__ __forceinline__ void sg_scan_block(int nb, const SgListArgs &L, long 
     __syncthreads();
   }
   long long run = part[t] - s, runf = partf[t] - f;  // exclusive starts of this thread's chunk
+#pragma unroll 8
   for (int b = b0; b < b1; b++) {
     const int c = L.block_count[3 * b + 1], cf = L.block_count[3 * b + 2];
     L.block_count[3 * b + 1] = (int)run;
