@@ -1069,23 +1069,27 @@ __global__ void k_dt(DevParams P, const unsigned long long *dmax_bits,
 // ---------------------------------------------------------------- mass continuity
 // IceModel::massContExplicitStep with cell_interface_fluxes and adjust_flow
 // (iMgeometry.cc:172-415, 599-776), no part_grid, no Dirichlet BC.
-__device__ __forceinline__ void adjust_flow_dir(int c, int nb, double &ssa, double &sia) {
-  if (m_grounded_ice(c) && m_grounded_ice(nb)) return;
-  if ((m_grounded_ice(c) && m_floating_ice(nb)) || (m_floating_ice(c) && m_grounded_ice(nb))) return;
-  if ((m_grounded_ice(c) && m_ice_free_land(nb)) || (m_ice_free_land(c) && m_grounded_ice(nb))) return;
-  if ((m_grounded_ice(c) && m_ice_free_ocean(nb)) || (m_ice_free_ocean(c) && m_grounded_ice(nb))) return;
-  if (m_floating_ice(c) && m_floating_ice(nb)) { sia = 0.0; return; }
+// which of a face's fluxes adjust_flow zeroes: bit 0 the SIA flux, bit 1 the
+// SSA velocity (values, not references: stores through references to the four
+// faces' values were merged into one store to a selected address, which kept
+// them in scratch memory)
+constexpr int ZERO_SIA = 1, ZERO_SSA = 2;
+__device__ __forceinline__ int adjust_flow_dir(int c, int nb) {
+  if (m_grounded_ice(c) && m_grounded_ice(nb)) return 0;
+  if ((m_grounded_ice(c) && m_floating_ice(nb)) || (m_floating_ice(c) && m_grounded_ice(nb))) return 0;
+  if ((m_grounded_ice(c) && m_ice_free_land(nb)) || (m_ice_free_land(c) && m_grounded_ice(nb))) return 0;
+  if ((m_grounded_ice(c) && m_ice_free_ocean(nb)) || (m_ice_free_ocean(c) && m_grounded_ice(nb))) return 0;
+  if (m_floating_ice(c) && m_floating_ice(nb)) return ZERO_SIA;
   if ((m_floating_ice(c) && m_ice_free_land(nb)) || (m_ice_free_land(c) && m_floating_ice(nb))) {
-    sia = 0.0; ssa = 0.0; return;
+    return ZERO_SIA | ZERO_SSA;
   }
-  if ((m_floating_ice(c) && m_ice_free_ocean(nb)) || (m_ice_free_ocean(c) && m_floating_ice(nb))) {
-    sia = 0.0; return;
-  }
-  if (m_ice_free_land(c) && m_ice_free_land(nb)) { sia = 0.0; ssa = 0.0; return; }
+  if ((m_floating_ice(c) && m_ice_free_ocean(nb)) || (m_ice_free_ocean(c) && m_floating_ice(nb))) return ZERO_SIA;
+  if (m_ice_free_land(c) && m_ice_free_land(nb)) return ZERO_SIA | ZERO_SSA;
   if ((m_ice_free_land(c) && m_ice_free_ocean(nb)) || (m_ice_free_ocean(c) && m_ice_free_land(nb))) {
-    sia = 0.0; ssa = 0.0; return;
+    return ZERO_SIA | ZERO_SSA;
   }
-  if (m_ice_free_ocean(c) && m_ice_free_ocean(nb)) { sia = 0.0; ssa = 0.0; return; }
+  if (m_ice_free_ocean(c) && m_ice_free_ocean(nb)) return ZERO_SIA | ZERO_SSA;
+  return 0;
 }
 
 // block bid (256 map points) of the mass-continuity grid; red: LDS [8][4]
@@ -1106,8 +1110,9 @@ __device__ __forceinline__ void mass_block(const DevParams &P, const MassArgs &A
     // neighbours N, E, S, W
     const long qn = (long)i * My + wrapi(j + 1, My), qe = (long)wrapi(i + 1, Mx) * My + j;
     const long qs = (long)i * My + wrapi(j - 1, My), qw = (long)wrapi(i - 1, Mx) * My + j;
-    const long nb[4] = {qn, qe, qs, qw};
     const int mc = as_int(A.mask[q]);
+    // faces N, E, S, W: the SIA flux and the advective velocity after adjust_flow
+    const long nb[4] = {qn, qe, qs, qw};
     double Q[4] = {A.flux[2 * q + 1], A.flux[2 * q], A.flux[2 * qs + 1], A.flux[2 * qw]};
     double V[4];
 #pragma unroll
@@ -1116,11 +1121,14 @@ __device__ __forceinline__ void mass_block(const DevParams &P, const MassArgs &A
       const bool ew = (d == 1 || d == 3);
       const double *vel = ew ? A.ub : A.vb;
       const double vij = vel ? vel[q] : 0.0, vnb = vel ? vel[nb[d]] : 0.0;
-      if (m_icy(mc) && m_icy(mn)) V[d] = 0.5 * (vij + vnb);
-      else if (m_icy(mc) && m_ice_free(mn)) V[d] = vij;
-      else if (m_ice_free(mc) && m_icy(mn)) V[d] = vnb;
-      else V[d] = 0.0;
-      adjust_flow_dir(mc, mn, V[d], Q[d]);
+      double v;
+      if (m_icy(mc) && m_icy(mn)) v = 0.5 * (vij + vnb);
+      else if (m_icy(mc) && m_ice_free(mn)) v = vij;
+      else if (m_ice_free(mc) && m_icy(mn)) v = vnb;
+      else v = 0.0;
+      const int z = adjust_flow_dir(mc, mn);
+      V[d] = (z & ZERO_SSA) ? 0.0 : v;
+      Q[d] = (z & ZERO_SIA) ? 0.0 : Q[d];
     }
     const double Hc = A.H[q];
     const double divQ_SIA = (Q[1] - Q[3]) / dx + (Q[0] - Q[2]) / dy;
