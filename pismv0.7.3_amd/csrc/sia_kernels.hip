@@ -1178,6 +1178,8 @@ __global__ __launch_bounds__(256) void k_mass_continuity(DevParams P, MassArgs A
 __global__ void k_reduce_partials(const double *__restrict__ partials, long nblocks, double *out) {
   __shared__ double red[8][256];
   double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // unrolled: the loads of four rounds are in flight together (same order)
+#pragma unroll 4
   for (long b = threadIdx.x; b < nblocks; b += blockDim.x) {
 #pragma unroll
     for (int c = 0; c < 8; c++) acc[c] += partials[b * 8 + c];
