@@ -235,14 +235,14 @@ __device__ __forceinline__ double ipow_nm1(const DevParams &P, double s) {
 }
 __device__ __forceinline__ double pow_n(const DevParams &P, double p) { return ipow_nm1(P, p) * p; }
 
-// A wavefront owns 16 consecutive list entries. Lane c prepares entry c (2-D
-// inputs, kBelowHeight), so the setup of all 16 is in flight at once. The
-// entries are then taken in pairs: all 64 lanes stage the averaged enthalpy
-// of both columns in LDS with coalesced loads (up to 4 x 64 levels of both
-// columns in flight before the first store), then half wave hh takes column
-// hh of the pair and lane m its zeta-cell m, walking the cell's levels from
-// LDS: the level weight G_k, the exact temperate sum, and the moments of
-// x = T_pa - 263.15 over the cold levels. The level weights are the reference's:
+// Training samples: a wavefront owns 16 consecutive list entries. Lane c
+// prepares entry c (2-D inputs, kBelowHeight), so the setup of all 16 is in
+// flight at once. The entries are then taken in pairs: all 64 lanes stage the
+// averaged enthalpy of both columns in LDS with coalesced loads, then half wave
+// hh takes column hh of the pair and lane m its zeta-cell m, walking the
+// cell's levels from LDS: the level weight G_k, the exact temperate sum, and
+// the moments of x = T_pa - 263.15 over the cold levels. The level weights are
+// the reference's:
 // W_k = d_k (dz_k [k >= 1] + dz_{k+1} [k < ks]) / 2 + d_ks^2 / 2 [k == ks]
 // (SIAFD.cc:677-684 regrouped per delta_k), G_k = W_k p_k^n.
 constexpr int SG_FEAT_COLS = 16;
