@@ -330,12 +330,6 @@ __device__ __forceinline__ float pinned(float v) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
-// 1 / x: hardware estimate refined by one Newton step
-__device__ __forceinline__ float rcpf_nr(float x) {
-  const float r = __builtin_amdgcn_rcpf(x);
-  return __builtin_fmaf(r, __builtin_fmaf(-x, r, 1.0f), r);
-}
-
 // kBelowHeight (IceGrid.cc:379-391) clamped to Mz - 2, from the bucket of the
 // height: the last level below the bucket's lower edge, then 0-2 steps
 __device__ __forceinline__ int kbelow_bucket(const double *zd, const int *bucket, double bscale, int Mz, double Lz,
@@ -367,7 +361,7 @@ __device__ __forceinline__ int kbelow_bucket(const double *zd, const int *bucket
 // temperate sum of G A_T0 (1 + wf_coeff min(omega, wf_limit)). The outputs are
 // scaled back by C p_b^n in FP64. The cell epilogue evaluates the physics
 // baseline Qb = g0 A(T) (1 + (k^2 - 2k/T) var / 2), T = T_ref + g1/g0,
-// k = Q / (R T^2), with the hardware exp2 and Newton reciprocals.
+// k = Q / (R T^2), with the hardware exp2 and reciprocal (about 1 ulp).
 template <int NP>
 __device__ __forceinline__ void sg_infer_features_block(const DevParams &P, const SgColumnArgs &C,
                                                         const SgFeatureArgs &F, int bid, int nblk, double2 *zsk,
@@ -414,7 +408,7 @@ __device__ __forceinline__ void sg_infer_features_block(const DevParams &P, cons
     // c_kt: ks, plus SG_TOP_POS when z_ks < thk (d_ks > 0 in FP64)
     int c_kt = -1, c_q = 0, c_q2 = 0;
     double c_s32 = 0.0, c_cs = 0.0;
-    float c_thk = 0.0f, c_fthk = 0.0f, c_lgs = 0.0f, c_cz = 0.0f, c_pa = 0.0f, c_bp = 0.0f;
+    float c_thk = 0.0f, c_fthk = 0.0f, c_lgs = 0.0f, c_cz = 0.0f, c_pa = 0.0f, c_bp = 0.0f, c_csf = 0.0f;
     if (lane < SG_INFER_COLS && t0 + lane < n) {
       const int s = F.list[t0 + lane];
       const int q = s >> 1, o = s & 1;
@@ -427,6 +421,7 @@ __device__ __forceinline__ void sg_infer_features_block(const DevParams &P, cons
       c_fthk = (float)(thk / 1000.0);       // the thickness input
       const double pb = P.p_air + rhog * thk;  // basal pressure
       c_cs = P.e * (0.5 * (C.theta[q] + C.theta[q2])) * 2.0 * ipow_nm1(P, alpha) * pown<NP>(pb);
+      c_csf = (float)c_cs;
       const double ipb = 1.0 / pb;
       c_cz = (float)(rhog * ipb);
       c_pa = (float)(P.p_air * ipb);
@@ -530,7 +525,7 @@ __device__ __forceinline__ void sg_infer_features_block(const DevParams &P, cons
         while (k <= ks && zd[k] * s32 < (double)m) k++;
         klo = k;
       }
-      int khi = __shfl_down(klo, 1, 64);
+      int khi = __builtin_amdgcn_update_dpp(0, klo, 0x130, 0xf, 0xf, true);  // lane l + 1 (wave_shl:1)
       if (m == SG_NCELL - 1) khi = ks + 1;
       if (!live) khi = 0;
       const float cz = lane_get_f(c_cz, src);
@@ -577,11 +572,11 @@ __device__ __forceinline__ void sg_infer_features_block(const DevParams &P, cons
       const bool has = live && g0 > 0.0f;
       float Qh = 0.0f, feat = 2.0f;
       if (has) {
-        const float ig0 = rcpf_nr(g0);
+        const float ig0 = __builtin_amdgcn_rcpf(g0);
         const float mx = g1 * ig0;
         const float Tbar = T_ref + mx;
         const float var = __builtin_fmaxf(__builtin_fmaf(g2, ig0, -mx * mx), 0.0f);
-        const float iT = rcpf_nr(Tbar);
+        const float iT = __builtin_amdgcn_rcpf(Tbar);
         const bool cold = Tbar < Tcrit;
         const float QR = cold ? QRc : QRw;
         const float kk = QR * iT * iT;
@@ -591,11 +586,12 @@ __device__ __forceinline__ void sg_infer_features_block(const DevParams &P, cons
         feat = (Tbar - 250.0f) * (1.0f / 15.0f);
       }
       const double cs = lane_get_d(c_cs, src);
+      const float csf = lane_get_f(c_csf, src);
       const float fthk = lane_get_f(c_fthk, src), lgs = lane_get_f(c_lgs, src);
       if (live) {  // row t0 + cc + hh: F (36 floats), CQb (32 floats), CSt
         const long r = t0 + cc + hh;
         F.F[r * SG_FSTRIDE + m] = feat;
-        F.CQb[r * SG_NCELL + m] = (float)(cs * (double)Qh);
+        F.CQb[r * SG_NCELL + m] = csf * Qh;
         if (m < SG_FSTRIDE - SG_NCELL) F.F[r * SG_FSTRIDE + SG_NCELL + m] = m == 0 ? fthk : (m == 1 ? lgs : 0.0f);
         if (m == 0) F.CSt[r] = cs * (double)S_t;
       }
