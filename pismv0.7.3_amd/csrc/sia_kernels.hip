@@ -562,6 +562,10 @@ __device__ __forceinline__ double column_block(const DevParams &P, const ColumnA
 
 constexpr int COLS_PER_WAVE = 16;
 constexpr int COLS_PER_BLOCK = WAVES_PER_BLOCK * COLS_PER_WAVE;
+// the FP64 pass over a list (the surrogate's fallback columns, a small share of
+// deep columns) takes 4 columns per wavefront, so that more wavefronts share it
+constexpr int LIST_COLS_PER_WAVE = 4;
+constexpr int LIST_COLS_PER_BLOCK = WAVES_PER_BLOCK * LIST_COLS_PER_WAVE;
 
 template <int LAW, bool FULL>
 __global__ __launch_bounds__(256) void k_column(DevParams P, ColumnArgs A) {
@@ -581,8 +585,8 @@ __global__ __launch_bounds__(256) void k_column_list(DevParams P, ColumnArgs A, 
   __shared__ double wmax[WAVES_PER_BLOCK];
   const int n = *n_list;
   double m = 0.0;
-  for (int vb = blockIdx.x; vb * COLS_PER_BLOCK < n; vb += gridDim.x) {
-    const double mb = column_block<LAW, false, COLS_PER_WAVE>(P, A, vb, zs, wmax, list, n);
+  for (int vb = blockIdx.x; vb * LIST_COLS_PER_BLOCK < n; vb += gridDim.x) {
+    const double mb = column_block<LAW, false, LIST_COLS_PER_WAVE>(P, A, vb, zs, wmax, list, n);
     m = (m < mb) ? mb : m;
   }
   if (threadIdx.x == 0) A.dmax_part[blockIdx.x] = m;
@@ -1486,8 +1490,8 @@ __global__ __launch_bounds__(256) void k_ens_column_list(const EnsListMember *__
   const int n = *glb(T[m].n_list);
   const int bid = blockIdx.x - start[m], nblk = start[m + 1] - start[m];
   double mx = 0.0;
-  for (int vb = bid; vb * COLS_PER_BLOCK < n; vb += nblk) {
-    const double mb = column_block<LAW, false, COLS_PER_WAVE>(T[m].P, A, vb, zs, wmax, list, n);
+  for (int vb = bid; vb * LIST_COLS_PER_BLOCK < n; vb += nblk) {
+    const double mb = column_block<LAW, false, LIST_COLS_PER_WAVE>(T[m].P, A, vb, zs, wmax, list, n);
     mx = (mx < mb) ? mb : mx;
   }
   if (threadIdx.x == 0) A.dmax_part[bid] = mx;
