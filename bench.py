@@ -56,6 +56,8 @@ for _p in (os.path.join(REPO, "pismv0.7.3_amd"), os.path.join(REPO, "oracle")):
 
 import numpy as np  # noqa: E402
 
+from training import SG_FALLBACK_FRACTION, TRAIN_SEEDS, train_sheet_surrogate  # noqa: E402
+
 MX, MY, MZ, LZ, DX = 1501, 2801, 401, 4000.0, 1000.0
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
@@ -174,12 +176,6 @@ def cpu_baseline(full=True, n_windows=192, win=128, seed=7, workers=None):
     return rate, secs, nw, sample
 
 
-TRAIN_SEEDS = [101, 202, 303, 404, 505, 606, 707, 808, 909, 1010, 1111, 1212]
-# share of a training sheet's surrogate columns whose fallback score exceeds
-# the calibrated threshold (they are computed in FP64 instead)
-SG_FALLBACK_FRACTION = 0.005
-
-
 def clone_geometry(S, f):
     g = {k: f[k].clone() for k in ("H", "h", "mask", "b")}
     return g, S.geometry(g["H"], g["h"], g["mask"], g["b"])
@@ -212,29 +208,13 @@ def time_steps(S, torch, sia, f, steps, warmup, fast):
     return start.elapsed_time(end) / steps, stages, g
 
 
-ENS_SIZES = (201, 401, 601, 801)
-
-
-def ensemble_members(n=64, seed=2026, base=1000):
-    """The ensemble workload: n members with seeded sizes Mx, My in ENS_SIZES,
-    dx ~ U[2, 5] km, Mz ~ U{101..201} (Lz 4000 m), GPBLD with e ~ U[1, 5] and
-    n in {3, 4}, each a polythermal sheet of its own seed (base + member)."""
-    rng = np.random.default_rng(seed)
-    out = []
-    for m in range(n):
-        Mx, My = (int(v) for v in rng.choice(ENS_SIZES, 2))
-        out.append(dict(Mx=Mx, My=My, dx=float(rng.uniform(2e3, 5e3)), Mz=int(rng.integers(101, 202)),
-                        e=float(rng.uniform(1.0, 5.0)), n=float(rng.choice([3.0, 4.0])), seed=base + m))
-    return out
-
-
 def run_ensemble(S, torch, steps=50, reps=2):
     """One persistent-launch-per-stage ensemble step of 64 members, 50 steps
     captured in a HIP graph; beside it the same members stepped one by one
     (sia_step per member, its own launches)."""
     import sia_types as T
     import workloads as W
-    spec = ensemble_members()
+    spec = W.ensemble_members()
     members = []
     for m in spec:
         z = T.vertical_levels(LZ, m["Mz"])
@@ -302,7 +282,7 @@ def combined_members(S, torch):
     z = T.vertical_levels(LZ, MZ)
     out = [dict(grid=T.make_grid(MX, MY, DX, DX, z), params=W.gl_params(), n=3.0, z=z, Mz=MZ)]
     out[0]["f"] = S.generate_greenland_like(out[0]["grid"], out[0]["params"])
-    for m in ensemble_members()[:63]:
+    for m in W.ensemble_members()[:63]:
         zm = T.vertical_levels(LZ, m["Mz"])
         grid = T.make_grid(m["Mx"], m["My"], m["dx"], m["dx"], zm)
         params = W.gl_params(sia_enhancement_factor=m["e"], sia_Glen_exponent=m["n"])
@@ -329,7 +309,7 @@ def train_combined_surrogate(S, torch, n, train_steps, batch):
     if n == 3.0:
         z = T.vertical_levels(LZ, MZ)
         sources += [(T.make_grid(MX, MY, DX, DX, z), W.gl_params(), dict(seed=sd)) for sd in TRAIN_SEEDS[:6]]
-    for m in ensemble_members(n=64, seed=4242, base=50000):
+    for m in W.ensemble_members(n=64, seed=4242, base=50000):
         if m["n"] == n:
             zm = T.vertical_levels(LZ, m["Mz"])
             sources.append((T.make_grid(m["Mx"], m["My"], m["dx"], m["dx"], zm),
@@ -590,34 +570,8 @@ def main():
     out_extra = {}
     if args.path == "surrogate":
         log("training the surrogate on %d other sheets" % len(TRAIN_SEEDS))
-        t0 = time.perf_counter()
-        sg = S.Surrogate(seed=5, capacity=60_000_000, stream=stream.cuda_stream)
-        collector = S.SIAFD(grid, params, stream=stream.cuda_stream)
-        collector.set_sync_check(False)
-        n = 0
-        for k, seed in enumerate(TRAIN_SEEDS):
-            ft = S.generate_greenland_like(grid, params, seed=seed)
-            # each sheet has its own bed: a new bed state makes the handle smooth it again
-            n = sg.collect(collector, ft["H"], ft["b"], ft["h"], ft["mask"], ft["E"], stride=1,
-                           bed_state_counter=k + 1)
-            del ft
-        assert S.lib().sia_check_errors(collector.h) == 0, S.lib().sia_last_error(collector.h)
-        collector.close()
-        torch.cuda.synchronize()
-        t_collect = time.perf_counter() - t0
-        sg.fit_normalization()
-        t1 = time.perf_counter()
-        loss = sg.train(args.train_steps, batch=args.train_batch, lr_start=2e-3, lr_end=1e-5, seed=9)
-        torch.cuda.synchronize()
-        t_train = time.perf_counter() - t1
-        # score fallback threshold from the last training sheet (never the benchmark sheet)
-        ft = S.generate_greenland_like(grid, params, seed=TRAIN_SEEDS[-1])
-        cal = S.SIAFD(grid, params, stream=stream.cuda_stream)
-        tau = sg.calibrate_fallback(cal, ft["H"], ft["b"], ft["h"], ft["mask"], ft["E"], SG_FALLBACK_FRACTION)
-        cal.close()
-        del ft
-        log("  %d samples (%.1f s), %d steps of %d (%.1f s), loss %.3e -> %.3e, fallback score tau %.3e" % (
-            n, t_collect, args.train_steps, args.train_batch, t_train, loss[:100].mean(), loss[-100:].mean(), tau))
+        sg, train_info = train_sheet_surrogate(S, torch, grid, params, steps=args.train_steps,
+                                               batch=args.train_batch, stream=stream.cuda_stream, log=log)
         sur = S.SIAFD(grid, params, stream=stream.cuda_stream)
         sur.set_sync_check(False)
         sur.use_surrogate(sg)
@@ -642,11 +596,7 @@ def main():
                            for m in results},
             "speedup_vs_plain_fast": round(ms_plain_fast / ms_s, 3),
             "surrogate_error": err,
-            "training": {"samples": int(n), "sheets": len(TRAIN_SEEDS), "steps": args.train_steps,
-                         "batch": args.train_batch, "collect_s": round(t_collect, 2), "train_s": round(t_train, 2),
-                         "loss_first100": float(loss[:100].mean()), "loss_last100": float(loss[-100:].mean()),
-                         "fallback_score_tau": tau, "fallback_fraction_calibrated": SG_FALLBACK_FRACTION,
-                         "calibration_sheet_seed": TRAIN_SEEDS[-1]},
+            "training": train_info,
         }
         fast_mode = True
     else:

@@ -44,6 +44,8 @@ typedef struct sia_surrogate sia_surrogate;
 /* A model with Xavier-uniform weights (seeded) and room for `capacity`
    training samples on the current device. */
 int sia_surrogate_create(uint64_t seed, int64_t capacity, sia_surrogate **out);
+/* SIA_ERR_STATE (and nothing freed) while a handle is still bound to the model:
+   unbind with sia_use_surrogate(h, NULL) or sia_destroy(h) first. */
 int sia_surrogate_destroy(sia_surrogate *sg);
 int sia_surrogate_set_stream(sia_surrogate *sg, void *hip_stream);
 
@@ -73,7 +75,8 @@ int sia_surrogate_load(sia_surrogate *sg, const void *blob, size_t size);
 /* Training samples (device array of 416-byte records) and their count. */
 int sia_surrogate_samples(const sia_surrogate *sg, const void **samples, int64_t *n);
 
-/* Fast updates of h use the surrogate from now on (NULL: FP64 again).
+/* Fast updates of h use the surrogate from now on (NULL: FP64 again). The
+   model must outlive the binding: sia_surrogate_destroy refuses a bound model.
    SIA_ERR_CONFIG if the flow law, enthalpy converter or Glen exponent of h
    differ from the model's, or the model has no normalization yet. */
 int sia_use_surrogate(sia_handle *h, sia_surrogate *sg);

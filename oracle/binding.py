@@ -176,6 +176,14 @@ def theta(grid, params, sm, usurf):
     return th
 
 
+def thksmooth(grid, usurf, thk, mask, maxtl, topgsmooth):
+    """BedSmoother::get_smoothed_thk (PISMBedSmoother.cc:286-336) pointwise."""
+    arr = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (usurf, thk, mask, maxtl, topgsmooth)]
+    out = np.zeros(grid.Mx * grid.My)
+    _check(lib().or_thksmooth(C.byref(grid), *[_ptr(a) for a in arr], _ptr(out)))
+    return out
+
+
 def surface_gradient(grid, params, H, b, h, mask):
     n2 = grid.Mx * grid.My
     hx, hy = np.zeros(2 * n2), np.zeros(2 * n2)
@@ -278,17 +286,18 @@ def max_timestep(grid, params, D_max, time_to_end=0.0, ub=None, vb=None, mask=No
                                  _ptr(ub), _ptr(vb), _ptr(mask))
 
 
-def step(grid, params, state, E, full=True, smb=None, bmr=None, time_to_end=0.0):
+def step(grid, params, state, E, full=True, smb=None, bmr=None, time_to_end=0.0, ub=None, vb=None):
     """One SIA-only IceModel::step: update, dt, mass continuity, mask/surface.
 
     ``state`` is a dict with H, b, h, mask (flat arrays), updated in place.
-    Returns (dt, sia_outputs, accounting).
+    ub, vb: the sliding (SSA) velocity given to the update, the 2-D CFL and
+    the upwind flux of mass continuity. Returns (dt, sia_outputs, accounting).
     """
     out = sia_update(grid, params, state["H"], state["b"], state["h"], state["mask"], E,
-                     full=full, bmr=bmr)
-    dt = max_timestep(grid, params, out["D_max"], time_to_end)
+                     full=full, bmr=bmr, ub=ub, vb=vb)
+    dt = max_timestep(grid, params, out["D_max"], time_to_end, ub=ub, vb=vb, mask=state["mask"])
     Hn, acc = mass_continuity(grid, params, dt, state["H"], out["flux"], state["mask"],
-                              smb=smb, bmr=bmr)
+                              smb=smb, bmr=bmr, ub=ub, vb=vb)
     mask, h = update_mask_surface(grid, params, state["b"], Hn)
     state["H"], state["mask"], state["h"] = Hn, mask, h
     return dt, out, acc

@@ -39,7 +39,8 @@ struct sia_handle {
   int *ks = nullptr;
   // scalars / scratch
   unsigned long long *dmax_bits = nullptr, *cfl_bits = nullptr;
-  int *err = nullptr;
+  int *err = nullptr;      // flags of the current update
+  int *err_acc = nullptr;  // flags of the earlier updates since the last check
   double *dt_dev = nullptr, *acc_dev = nullptr, *partials = nullptr, *H_tmp = nullptr;
   double *dmax_part = nullptr;
   double *sl_hx = nullptr, *sl_hy = nullptr, *sl_x = nullptr, *sl_y = nullptr;  // SIA_Sliding work
@@ -76,6 +77,7 @@ struct sia_surrogate {
   double glen_n = 0.0;  // Glen exponent of the training samples (0: none yet)
   int version = 0;      // bumped when the normalization or tau changes (ensembles copy them)
   float tau = INFINITY; // fallback score threshold (calibrated; +inf: none)
+  int bound = 0;        // handles bound to this model (sia_use_surrogate); destroy refuses while > 0
   SgSample *samples = nullptr;
   long long *d_n = nullptr;
   int64_t cap = 0, n = 0;
@@ -231,10 +233,17 @@ void fill_dev_params(sia_handle *h) {
 // changes with grain size among the laws of flowlaw_factory.cc:77-83.
 bool law_uses_grain_size(int law) { return law == SIA_FLOW_GK; }
 
+// Reports the flags raised by every update since the last check (the
+// reference throws at once, SIAFD.cc:654; here a step that does not
+// synchronise leaves them to the next check) and clears them.
 int check_flags(sia_handle *h) {
-  int flags = 0;
-  HIP_TRY(h, hipMemcpyAsync(&flags, h->err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  int f[2] = {0, 0};
+  HIP_TRY(h, hipMemcpyAsync(&f[0], h->err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipMemcpyAsync(&f[1], h->err_acc, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  launch_clear_flags(h->err, h->err_acc, h->stream);
+  HIP_TRY(h, hipGetLastError());
   HIP_TRY(h, hipStreamSynchronize(h->stream));
+  const int flags = f[0] | f[1];
   if (flags & ERR_NEG_THK) {
     return set_err(h, SIA_ERR_NEGATIVE_THICKNESS, "negative ice thickness (BedSmoother::get_smoothed_thk)");
   }
@@ -307,7 +316,7 @@ int update_impl(sia_handle *h, const sia_inputs *in, int fast, bool allow_sync) 
   }
   hipStream_t st = h->stream;
   h->have_flux = false;
-  launch_reset(h->dmax_bits, h->err, h->cfl_bits, h->dp.max_dt, st);
+  launch_reset(h->dmax_bits, h->err, h->err_acc, h->cfl_bits, h->dp.max_dt, st);
   // SIAFD.cc:128-133: re-smooth the bed when its state counter increased
   if (in->bed_state_counter > h->bed_counter || !h->bed_ready) {
     Stage stage(h, SIA_EV_BED_SMOOTHER);
@@ -622,14 +631,17 @@ int sia_create(const sia_grid *grid, const sia_params *params, sia_handle **out)
       (rc = dalloc(h, &h->C4, n2)) || (rc = dalloc(h, &h->hx, 2 * n2)) || (rc = dalloc(h, &h->hy, 2 * n2)) ||
       (rc = dalloc(h, &h->theta, n2)) || (rc = dalloc(h, &h->thks, n2)) || (rc = dalloc(h, &h->flux, 2 * n2)) ||
       (rc = dalloc(h, &h->Dstag, 2 * n2)) || (rc = dalloc(h, &h->dmax_bits, 1)) ||
-      (rc = dalloc(h, &h->cfl_bits, 1)) || (rc = dalloc(h, &h->err, 1)) || (rc = dalloc(h, &h->dt_dev, 1)) ||
+      (rc = dalloc(h, &h->cfl_bits, 1)) || (rc = dalloc(h, &h->err, 1)) || (rc = dalloc(h, &h->err_acc, 1)) ||
+      (rc = dalloc(h, &h->dt_dev, 1)) ||
       (rc = dalloc(h, &h->acc_dev, 8)) || (rc = dalloc(h, &h->H_tmp, n2)) ||
       (rc = dalloc(h, &h->partials, (size_t)mass_partial_blocks(h->dp) * 8)) ||
       (rc = dalloc(h, &h->dmax_part, (size_t)(column_blocks(h->dp) + sg_mlp_blocks())))) {
     sia_destroy(h);
     return rc;
   }
-  if (hipMemcpy(h->d_z, h->z.data(), sizeof(double) * grid->Mz, hipMemcpyHostToDevice) != hipSuccess) {
+  if (hipMemcpy(h->d_z, h->z.data(), sizeof(double) * grid->Mz, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(h->err, 0, sizeof(int)) != hipSuccess || hipMemset(h->err_acc, 0, sizeof(int)) != hipSuccess ||
+      hipDeviceSynchronize() != hipSuccess) {
     sia_destroy(h);
     return SIA_ERR_HIP;
   }
@@ -639,10 +651,11 @@ int sia_create(const sia_grid *grid, const sia_params *params, sia_handle **out)
 
 int sia_destroy(sia_handle *h) {
   if (!h) return SIA_OK;
+  if (h->sg) h->sg->bound--;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   else (void)hipDeviceSynchronize();
   void *bufs[] = {h->d_z, h->topgs, h->maxtl, h->C2, h->C3, h->C4, h->hx, h->hy, h->theta, h->thks,
-                  h->flux, h->Dstag, h->I0, h->I1, h->u, h->v, h->w, h->Sigma, h->ks, h->dmax_bits, h->cfl_bits, h->err,
+                  h->flux, h->Dstag, h->I0, h->I1, h->u, h->v, h->w, h->Sigma, h->ks, h->dmax_bits, h->cfl_bits, h->err, h->err_acc,
                   h->dt_dev, h->acc_dev, h->partials, h->H_tmp, h->dmax_part, h->sl_hx, h->sl_hy,
                   h->sl_x, h->sl_y, h->sg_status,
                   h->sg_block_count, h->sg_list, h->sg_nlist, h->sg_counts, h->sg_F, h->sg_CQb, h->sg_CSt,
@@ -968,6 +981,8 @@ int sia_surrogate_create(uint64_t seed, int64_t capacity, sia_surrogate **out) {
 
 int sia_surrogate_destroy(sia_surrogate *sg) {
   if (!sg) return SIA_OK;
+  // a bound handle would read the freed model on its next fast update
+  if (sg->bound > 0) return SIA_ERR_STATE;
   (void)hipDeviceSynchronize();
   void *bufs[] = {sg->master, sg->m, sg->v, sg->grad, sg->frag, sg->samples, sg->d_n, sg->d_loss};
   for (void *b : bufs) dfree(b);
@@ -1187,10 +1202,17 @@ int sia_surrogate_samples(const sia_surrogate *sg, const void **samples, int64_t
   return SIA_OK;
 }
 
+// binds sg to h (nullptr: unbinds), keeping the models' bound-handle counts
+static void bind_surrogate(sia_handle *h, sia_surrogate *sg) {
+  if (h->sg) h->sg->bound--;
+  h->sg = sg;
+  if (sg) sg->bound++;
+}
+
 int sia_use_surrogate(sia_handle *h, sia_surrogate *sg) {
   if (!h) return SIA_ERR_ARGUMENT;
   if (!sg) {
-    h->sg = nullptr;
+    bind_surrogate(h, nullptr);
     return SIA_OK;
   }
   const sia_params &p = h->params;
@@ -1204,7 +1226,7 @@ int sia_use_surrogate(sia_handle *h, sia_surrogate *sg) {
   if (sg->device != h->device) return set_err(h, SIA_ERR_CONFIG, "the surrogate lives on another device");
   int rc;
   if ((rc = ensure_sg_buffers(h))) return rc;
-  h->sg = sg;
+  bind_surrogate(h, sg);
   return SIA_OK;
 }
 
@@ -1231,7 +1253,7 @@ int sia_surrogate_calibrate_fallback(sia_surrogate *sg, sia_handle *h, const sia
     rc = update_impl(h, in, 1, true);
     h->sg_calibrating = false;
   }
-  h->sg = saved;
+  bind_surrogate(h, saved);
   if (rc) return rc;
   int nlist = 0;
   HIP_TRY(h, hipMemcpyAsync(&nlist, h->sg_nlist, sizeof(int), hipMemcpyDeviceToHost, h->stream));

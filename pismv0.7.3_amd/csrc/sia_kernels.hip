@@ -106,13 +106,28 @@ __device__ __forceinline__ bool strip_column(const StripMap &m, int x, int Lp, i
 }
 
 // ---------------------------------------------------------------- resets
-__global__ void k_reset(unsigned long long *dmax_bits, int *err, unsigned long long *cfl_bits,
+// err_acc keeps the flags of every update since the last check: the flags of
+// the previous update move there before err is cleared for this one
+__global__ void k_reset(unsigned long long *dmax_bits, int *err, int *err_acc, unsigned long long *cfl_bits,
                         double max_dt) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     *dmax_bits = 0ull;  // +0.0
+    *err_acc |= *err;
     *err = 0;
     *cfl_bits = (unsigned long long)__double_as_longlong(max_dt);
   }
+}
+
+// clears the current and the accumulated error flags after they were read
+__global__ void k_clear_flags(int *err, int *err_acc) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    *err = 0;
+    *err_acc = 0;
+  }
+}
+
+void launch_clear_flags(int *err, int *err_acc, hipStream_t st) {
+  hipLaunchKernelGGL(k_clear_flags, dim3(1), dim3(64), 0, st, err, err_acc);
 }
 
 __global__ void k_fill(double *__restrict__ a, long n, double v) {
@@ -1289,9 +1304,9 @@ int column_blocks(const DevParams &P) { return (int)blocks_for(2L * P.Mx * P.My,
 
 int column_list_blocks(const DevParams &P) { return column_blocks(P) < 1024 ? column_blocks(P) : 1024; }
 
-void launch_reset(unsigned long long *dmax_bits, int *err, unsigned long long *cfl_bits, double max_dt,
-                  hipStream_t st) {
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, dmax_bits, err, cfl_bits, max_dt);
+void launch_reset(unsigned long long *dmax_bits, int *err, int *err_acc, unsigned long long *cfl_bits,
+                  double max_dt, hipStream_t st) {
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, dmax_bits, err, err_acc, cfl_bits, max_dt);
 }
 
 void launch_fill(double *a, long n, double v, hipStream_t st) {

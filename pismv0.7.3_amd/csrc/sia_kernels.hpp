@@ -59,8 +59,11 @@ struct MassArgs {
   double *partials;          // [nblocks][8] or nullptr
 };
 
-void launch_reset(unsigned long long *dmax_bits, int *err, unsigned long long *cfl_bits, double max_dt,
-                  hipStream_t st);
+// clears D_max and the per-update error flags (which are first ORed into
+// err_acc) and sets the CFL bound to max_dt
+void launch_reset(unsigned long long *dmax_bits, int *err, int *err_acc, unsigned long long *cfl_bits,
+                  double max_dt, hipStream_t st);
+void launch_clear_flags(int *err, int *err_acc, hipStream_t st);
 void launch_fill(double *a, long n, double v, hipStream_t st);
 void launch_bed_smoother(const DevParams &P, double s2, double s3, double s4, const double *b0, double *bs,
                          double *mt, double *c2, double *c3, double *c4, hipStream_t st);

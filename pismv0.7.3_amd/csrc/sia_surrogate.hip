@@ -94,9 +94,10 @@ constexpr int SG_CLS_BLOCK = 256;
 
 int sg_classify_blocks(const DevParams &P) { return (2 * P.Mx * P.My + SG_CLS_BLOCK - 1) / SG_CLS_BLOCK; }
 
-// status of staggered column s: SG_ZERO (thk == 0 or zero slope: D = 0,
-// SIAFD.cc:635-641), SG_FALLBACK (thk below the second level, or above Lz:
-// left to the FP64 kernel, which also reports the error), else SG_SURR.
+// status of staggered column s: SG_ZERO (thk == 0, SIAFD.cc:635-641, or zero
+// slope with thk <= Lz: D = 0), SG_FALLBACK (thk above Lz, whatever the slope:
+// left to the FP64 kernel, which reports the error as kBelowHeight does for
+// any thk != 0, SIAFD.cc:654; or thk below the second level), else SG_SURR.
 __device__ __forceinline__ void sg_classify_block(const DevParams &P, const SgColumnArgs &C, const SgListArgs &L,
                                                   int bid) {
   const int s = bid * SG_CLS_BLOCK + threadIdx.x;
@@ -109,12 +110,14 @@ __device__ __forceinline__ void sg_classify_block(const DevParams &P, const SgCo
     const double thk = 0.5 * (C.thks[q] + C.thks[q2]);
     const double hxs = C.hx[s], hys = C.hy[s];
     const double alpha = sqrt(hxs * hxs + hys * hys);
-    if (thk == 0.0 || alpha == 0.0) {
+    if (thk > P.Lz + 1.0e-6) {
+      st = SG_FALLBACK;
+    } else if (thk == 0.0 || alpha == 0.0) {
       st = SG_ZERO;
       const double slope = (o == 0) ? hxs : hys;
       L.flux[s] = (thk == 0.0) ? 0.0 : -0.0 * slope;
       L.Dstag[s] = 0.0;
-    } else if (thk < C.z[1] || thk > P.Lz + 1.0e-6) {
+    } else if (thk < C.z[1]) {
       st = SG_FALLBACK;
     } else {
       st = SG_SURR;

@@ -222,6 +222,12 @@ class SIAFD:
     def set_sync_check(self, on):
         self._check(lib().sia_set_sync_check(self.h, 1 if on else 0))
 
+    def check_errors(self):
+        """Raises SIAError for the device-side error flags of every update and
+        step since the last check (ParallelSection::check, error_handling.cc:153-192),
+        and clears them."""
+        self._check(lib().sia_check_errors(self.h))
+
     def update(self, H, b, h, mask, E, fast=False, age=None, ub=None, vb=None, bmr=None,
                bed_state_counter=0, no_model_mask=None, usurfstore=None):
         """SIAFD::update(vel_input, fast) (SIAFD.cc:119-148) + strain heating and
@@ -373,8 +379,12 @@ class Surrogate:
             raise SIAError(rc, what)
 
     def close(self):
+        """Frees the model; SIAError(SIA_ERR_STATE) while a handle is still
+        bound to it (the model is then kept)."""
         if getattr(self, "h", None):
-            lib().sia_surrogate_destroy(self.h)
+            rc = lib().sia_surrogate_destroy(self.h)
+            if rc != 0:
+                raise SIAError(rc, "sia_surrogate_destroy: a handle is still bound to the model")
             self.h = None
 
     def __del__(self):

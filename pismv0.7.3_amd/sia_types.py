@@ -19,6 +19,10 @@ FLOW_LAWS = {  # sia_flow_law keywords, cdl:1093; flowlaw_factory.cc:77-83
 GRADIENT_METHODS = {"haseloff": 0, "mahaffy": 1, "eta": 2}  # cdl:1272; SIAFD.cc:187-207
 ENTHALPY_CONVERTERS = {"default": 0, "cold": 1, "kirchhoff": 2}
 
+# status codes of include/pism_sia.h (enum sia_status)
+SIA_OK, SIA_ERR_ARGUMENT, SIA_ERR_HIP, SIA_ERR_NEGATIVE_THICKNESS, SIA_ERR_THICKNESS_ABOVE_LZ, \
+    SIA_ERR_OMEGA_NEGATIVE, SIA_ERR_CONFIG, SIA_ERR_OUT_OF_MEMORY, SIA_ERR_STATE = range(9)
+
 # mask codes, Mask.hh:28-34
 MASK_ICE_FREE_BEDROCK = 0
 MASK_GROUNDED = 2

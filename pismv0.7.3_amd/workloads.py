@@ -199,3 +199,78 @@ def enthalpy_columns(P, H, f_t, omega, T_s, z):
         Ek = np.where(zk > H, E_air, Ek)
         E[..., k] = np.where(H > 0.0, Ek, E_air)
     return E
+
+
+# ------------------------------------------------------------------ marine window
+def marine_window(i0=140, i1=204, j0=1350, j1=1422, float_quantile=0.15, Mz=401, **kw):
+    """A window of the polythermal sheet with is_dry_simulation = 0 and a sea
+    level that floats part of the ice: the sea level is the float_quantile
+    quantile, over the icy cells, of the level at which a cell starts to float
+    (b + (rho_i / rho_w) H + 1 m, Mask.hh:82-118). The default window holds
+    grounded and floating ice, ice-free ocean and ice-free land."""
+    base = greenland_like(i0, i1, j0, j1, Mz=Mz, **kw)
+    H, b = base.fields["H"], base.fields["b"]
+    P = base.params
+    icy = H > 0.0
+    fl = b + (P.ice_density / P.sea_water_density) * H + P.mask_is_floating_thickness_standard
+    sea_level = float(np.quantile(fl[icy], float_quantile))
+    return greenland_like(i0, i1, j0, j1, Mz=Mz, is_dry_simulation=0, sea_level=sea_level, **kw)
+
+
+def sliding_field(w, speed=100.0 / T.SECONDS_PER_YEAR):
+    """A smooth synthetic sliding (SSA) velocity (u_b, v_b) of the given
+    speed scale on every cell of workload w, flat arrays in m s-1: it drives
+    the upwind flux and the 2-D CFL limit of mass continuity."""
+    Mx, My = w.grid.Mx, w.grid.My
+    ii, jj = np.meshgrid(np.arange(Mx), np.arange(My), indexing="ij")
+    u = speed * np.sin(2.0 * math.pi * ii / Mx + 0.3) * np.cos(2.0 * math.pi * jj / My)
+    v = speed * np.cos(2.0 * math.pi * ii / Mx) * np.sin(2.0 * math.pi * jj / My + 0.7)
+    return u.ravel(), v.ravel()
+
+
+# ------------------------------------------------------------------ config-1 grid, polythermal
+def dome_polythermal(seed, Mx=61, My=61, L=750e3, Mz=31, Lz=4000.0, H0=3600.0, R=750e3, **param_overrides):
+    """The 61x61x31 dome geometry (flat bed, Vialov profile, dx = 25 km, 31
+    equal levels to 4000 m) with GPBLD and a seeded polythermal enthalpy of
+    the greenland_like kind (temperate basal layer of U[0, 0.1] H with water
+    fraction U[0, 0.01], cold above from a surface temperature U[238, 248] K):
+    the surrogate's form of the dome, whose isothermal Glen law it does not
+    model."""
+    dx = T.horizontal_spacing(L, Mx)
+    dy = T.horizontal_spacing(L, My)
+    x = T.horizontal_coordinates(L, Mx)
+    y = T.horizontal_coordinates(L, My)
+    z = T.vertical_levels(Lz, Mz)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    r = np.sqrt(X * X + Y * Y)
+    H = np.where(r < R, H0 * np.power(np.maximum(1.0 - np.power(r / R, 4.0 / 3.0), 0.0), 3.0 / 8.0), 0.0)
+    opts = dict(sia_flow_law="gpbld", sia_enhancement_factor=1.0, is_dry_simulation=1)
+    opts.update(param_overrides)
+    P = T.make_params(**opts)
+    b = np.zeros_like(H)
+    mask, h = geometry_mask_surface(P, b, H)
+    col = np.arange(Mx * My, dtype=np.uint64).reshape(Mx, My)
+    f_t = 0.1 * uniform01(seed, S_TEMPERATE, col)
+    omega = 0.01 * uniform01(seed, S_OMEGA, col)
+    T_s = 238.15 + 10.0 * uniform01(seed, S_TSURF, col)
+    E = enthalpy_columns(P, H, f_t, omega, T_s, z)
+    grid = T.make_grid(Mx, My, dx, dy, z)
+    fields = dict(H=H.ravel(), b=b.ravel(), h=h.ravel(), mask=mask.ravel(), E=E.ravel())
+    return Workload("dome_polythermal", grid, P, fields, (Mx, My, Mz), (0, Mx, 0, My))
+
+
+# ------------------------------------------------------------------ ensemble workload
+ENS_SIZES = (201, 401, 601, 801)
+
+
+def ensemble_members(n=64, seed=2026, base=1000):
+    """The ensemble workload: n members with seeded sizes Mx, My in ENS_SIZES,
+    dx ~ U[2, 5] km, Mz ~ U{101..201} (Lz 4000 m), GPBLD with e ~ U[1, 5] and
+    n in {3, 4}, each a polythermal sheet of its own seed (base + member)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for m in range(n):
+        Mx, My = (int(v) for v in rng.choice(ENS_SIZES, 2))
+        out.append(dict(Mx=Mx, My=My, dx=float(rng.uniform(2e3, 5e3)), Mz=int(rng.integers(101, 202)),
+                        e=float(rng.uniform(1.0, 5.0)), n=float(rng.choice([3.0, 4.0])), seed=base + m))
+    return out

@@ -22,14 +22,13 @@ Tolerances (stated per check):
   keeps the surrogate's D (1e-4 to the oracle forward); tau survives the
   model blob.
 """
-import struct
-
 import numpy as np
 import pytest
 
 import surrogate_oracle as SO
 import workloads as W
 from parity_util import say
+from surrogate_util import eligible, parse_norm, plain_fast, surrogate_fast
 
 pytestmark = pytest.mark.gpu
 
@@ -49,21 +48,6 @@ def S():
 def window_fields(S, win, **kw):
     w = W.greenland_like(*win, **kw)
     return w, {k: S.to_device(v) for k, v in w.fields.items()}
-
-
-def plain_fast(S, w, dev):
-    sia = S.SIAFD(w.grid, w.params)
-    sia.update(dev["H"], dev["b"], dev["h"], dev["mask"], dev["E"], fast=True)
-    out = dict(D=sia.diffusivity_staggered(), flux=sia.diffusive_flux(), D_max=sia.max_diffusivity())
-    out["theta"], out["thks"] = sia.theta_thksmooth()
-    out["hx"], out["hy"] = sia.surface_gradient()
-    sia.close()
-    return out
-
-
-def eligible(w, st):
-    z = np.array([w.grid.z[k] for k in range(w.grid.Mz)])
-    return (st["thk"] > 0) & (st["alpha"] > 0) & (st["thk"] >= z[1]) & (st["thk"] <= z[-1] + 1e-6)
 
 
 def test_collected_samples_match_oracle(S):
@@ -137,24 +121,6 @@ def test_training_reduces_loss(trained):
     # the loss of the untrained network (output 0) is the weighted mean square of
     # the targets; training must cut it severalfold (accuracy is checked below)
     assert loss[-200:].mean() < 0.3 * loss[:20].mean()
-
-
-def parse_norm(blob):
-    hdr = struct.calcsize("8sii d ii")
-    vals = np.frombuffer(blob, dtype=np.float32, count=4 * 34 + 1, offset=hdr)
-    mean, inv_std, lo, hi = (vals[34 * i:34 * (i + 1)] for i in range(4))
-    w = np.frombuffer(blob, dtype=np.float32, offset=hdr + 4 * (4 * 34 + 1))
-    return dict(mean=mean, inv_std=inv_std, lo=lo, hi=hi, rscale=float(vals[-1])), w
-
-
-def surrogate_fast(S, sg, w, dev):
-    sia = S.SIAFD(w.grid, w.params)
-    sia.use_surrogate(sg)
-    sia.update(dev["H"], dev["b"], dev["h"], dev["mask"], dev["E"], fast=True)
-    out = dict(D=sia.diffusivity_staggered(), flux=sia.diffusive_flux(), D_max=sia.max_diffusivity(),
-               counts=sia.surrogate_counts())
-    sia.close()
-    return out
 
 
 # the second window (an ice margin) lists 4625 columns: an odd length, not a
