@@ -4,6 +4,7 @@
 #include "../../include/pism_sia.h"
 #include "../../include/sia_surrogate.h"
 #include "../../include/sia_ensemble.h"
+#include "sia_host.hpp"
 #include "sia_kernels.hpp"
 #include "sia_surrogate.hpp"
 
@@ -89,6 +90,120 @@ struct sia_surrogate {
   int64_t d_loss_len = 0;
 };
 
+namespace pism_sia {
+
+static double softness_pb_host(const sia_params &p, double T_pa) {
+  if (T_pa < p.Paterson_Budd_critical_temperature) {
+    return p.Paterson_Budd_A_cold * std::exp(-p.Paterson_Budd_Q_cold / (p.ideal_gas_constant * T_pa));
+  }
+  return p.Paterson_Budd_A_warm * std::exp(-p.Paterson_Budd_Q_warm / (p.ideal_gas_constant * T_pa));
+}
+
+void make_dev_params(const sia_grid &g, const sia_params &p, DevParams &d) {
+  d = DevParams{};
+  d.Mx = g.Mx;
+  d.My = g.My;
+  d.Mz = g.Mz;
+  d.law = p.sia_flow_law;
+  d.ec = p.enthalpy_converter;
+  d.grad = p.surface_gradient_method;
+  d.is_dry = p.is_dry_simulation;
+  d.include_bmr = p.include_bmr_in_continuity;
+  d.dx = g.dx;
+  d.dy = g.dy;
+  d.Lz = g.z[g.Mz - 1];
+  d.e = p.sia_enhancement_factor;
+  d.n = p.sia_Glen_exponent;
+  d.nm1 = p.sia_Glen_exponent - 1;
+  d.nm1_int = (d.nm1 == 1.0) ? 1 : (d.nm1 == 2.0) ? 2 : (d.nm1 == 3.0) ? 3 : -1;
+  d.gs = p.ice_grain_size;
+  d.p_air = p.surface_pressure;
+  d.rho_i = p.ice_density;
+  d.g = p.standard_gravity;
+  d.beta = p.beta_CC;
+  d.ci = p.ice_specific_heat_capacity;
+  d.T_melt = p.water_melting_point_temperature;
+  d.T0 = p.enthalpy_converter_reference_temperature;
+  d.L = p.water_latent_heat_fusion;
+  d.cw = p.water_specific_heat_capacity;
+  d.A_cold = p.Paterson_Budd_A_cold;
+  d.A_warm = p.Paterson_Budd_A_warm;
+  d.Q_cold = p.Paterson_Budd_Q_cold;
+  d.Q_warm = p.Paterson_Budd_Q_warm;
+  d.R = p.ideal_gas_constant;
+  d.T_crit = p.Paterson_Budd_critical_temperature;
+  {
+    const double beta_CC_grad = p.beta_CC * p.ice_density * p.standard_gravity;  // flowlaws.cc:71
+    d.pb_coeff = beta_CC_grad / (p.ice_density * p.standard_gravity);            // flowlaws.cc:242
+  }
+  d.A_T0 = softness_pb_host(p, p.water_melting_point_temperature);  // flowlaws.cc:219
+  d.ln_A_T0 = std::log(d.A_T0);
+  d.wf_coeff = p.gpbld_water_frac_coeff;
+  d.wf_limit = p.gpbld_water_frac_observed_limit;
+  d.A_glen = p.ice_softness;
+  d.hooke_A = p.Hooke_A;
+  d.hooke_Q = p.Hooke_Q;
+  d.hooke_C = p.Hooke_C;
+  d.hooke_k = p.Hooke_k;
+  d.hooke_Tr = p.Hooke_Tr;
+  d.sec_to_ka = 1.0 / (1000.0 * p.seconds_per_year);
+  d.hard_pow = -1.0 / p.sia_Glen_exponent;                         // flowlaws.cc:76
+  d.e_pow = std::pow(p.sia_enhancement_factor, -1.0 / p.sia_Glen_exponent);  // PISMStressBalance.cc:388
+  d.heat_exp = 0.5 * (1.0 / p.sia_Glen_exponent + 1.0);             // PISMStressBalance.cc:387
+  d.B_glen = std::pow(p.ice_softness, d.hard_pow);                  // flowlaws.cc:252
+  d.ln_A_cold = std::log(p.Paterson_Budd_A_cold);
+  d.ln_A_warm = std::log(p.Paterson_Budd_A_warm);
+  d.ln_A_glen = std::log(p.ice_softness);
+  d.ln_hooke_A = std::log(p.Hooke_A);
+  d.inv_ci = 1.0 / p.ice_specific_heat_capacity;
+  d.inv_L = 1.0 / p.water_latent_heat_fusion;
+  d.alpha_float = 1 - p.ice_density / p.sea_water_density;
+  d.sea_level = p.sea_level;
+  d.icefree_thk = p.mask_icefree_thickness_standard;
+  d.floating_thk = p.mask_is_floating_thickness_standard;
+  d.max_dt = p.maximum_time_step;
+  d.adapt_ratio = p.adaptive_timestepping_ratio;
+  d.Nx = -1;
+  d.Ny = -1;
+  d.use_age = 0;
+}
+
+int check_grid_params(const sia_grid *grid, const sia_params *params) {
+  if (!grid || !params || !grid->z) return SIA_ERR_ARGUMENT;
+  if (grid->Mx < 3 || grid->My < 3 || grid->Mz < 2 || !(grid->dx > 0) || !(grid->dy > 0)) return SIA_ERR_ARGUMENT;
+  // indices of staggered columns are 32-bit; z levels are staged in LDS
+  if (2.0 * grid->Mx * grid->My >= 2147483647.0 || grid->Mz > MAX_MZ) return SIA_ERR_ARGUMENT;
+  if (params->sia_flow_law < 0 || params->sia_flow_law > SIA_FLOW_GK) return SIA_ERR_CONFIG;
+  if (params->surface_gradient_method < 0 || params->surface_gradient_method > SIA_GRADIENT_ETA) return SIA_ERR_CONFIG;
+  if (params->enthalpy_converter < 0 || params->enthalpy_converter > SIA_EC_KIRCHHOFF) return SIA_ERR_CONFIG;
+  for (int k = 1; k < grid->Mz; k++) {
+    if (!(grid->z[k] > grid->z[k - 1])) return SIA_ERR_ARGUMENT;
+  }
+  return SIA_OK;
+}
+
+// BedSmoother's half-widths and moment scales (PISMBedSmoother.cc:96-149,
+// 247-263): Nx = Ny = -1 when the range is not positive (no smoothing)
+int bed_smoother_setup(const sia_grid &g, const sia_params &p, DevParams &d, double s[3]) {
+  d.Nx = -1;
+  d.Ny = -1;
+  if (p.bed_smoother_range <= 0.0) return SIA_OK;
+  int Nx = (int)std::ceil(p.bed_smoother_range / g.dx);
+  int Ny = (int)std::ceil(p.bed_smoother_range / g.dy);
+  if (Nx < 1) Nx = 1;
+  if (Ny < 1) Ny = 1;
+  if (Nx >= g.Mx || Ny >= g.My) return SIA_ERR_CONFIG;
+  d.Nx = Nx;
+  d.Ny = Ny;
+  const double n = p.sia_Glen_exponent, k = (n + 2) / n;
+  s[0] = k * (2 * n + 2) / (2 * n);
+  s[1] = s[0] * (3 * n + 2) / (3 * n);
+  s[2] = s[1] * (4 * n + 2) / (4 * n);
+  return SIA_OK;
+}
+
+}  // namespace pism_sia
+
 namespace {
 
 int set_err(sia_handle *h, int code, const std::string &msg) {
@@ -152,86 +267,6 @@ void dfree(void *p) {
   if (p) (void)hipFree(p);
 }
 
-double softness_pb_host(const sia_params &p, double T_pa) {
-  if (T_pa < p.Paterson_Budd_critical_temperature) {
-    return p.Paterson_Budd_A_cold * std::exp(-p.Paterson_Budd_Q_cold / (p.ideal_gas_constant * T_pa));
-  }
-  return p.Paterson_Budd_A_warm * std::exp(-p.Paterson_Budd_Q_warm / (p.ideal_gas_constant * T_pa));
-}
-
-void fill_dev_params(sia_handle *h) {
-  const sia_params &p = h->params;
-  DevParams &d = h->dp;
-  d.Mx = h->grid.Mx;
-  d.My = h->grid.My;
-  d.Mz = h->grid.Mz;
-  d.law = p.sia_flow_law;
-  d.ec = p.enthalpy_converter;
-  d.grad = p.surface_gradient_method;
-  d.is_dry = p.is_dry_simulation;
-  d.include_bmr = p.include_bmr_in_continuity;
-  d.dx = h->grid.dx;
-  d.dy = h->grid.dy;
-  d.Lz = h->z[h->grid.Mz - 1];
-  d.e = p.sia_enhancement_factor;
-  d.n = p.sia_Glen_exponent;
-  d.nm1 = p.sia_Glen_exponent - 1;
-  d.nm1_int = (d.nm1 == 1.0) ? 1 : (d.nm1 == 2.0) ? 2 : (d.nm1 == 3.0) ? 3 : -1;
-  d.gs = p.ice_grain_size;
-  d.p_air = p.surface_pressure;
-  d.rho_i = p.ice_density;
-  d.g = p.standard_gravity;
-  d.beta = p.beta_CC;
-  d.ci = p.ice_specific_heat_capacity;
-  d.T_melt = p.water_melting_point_temperature;
-  d.T0 = p.enthalpy_converter_reference_temperature;
-  d.L = p.water_latent_heat_fusion;
-  d.cw = p.water_specific_heat_capacity;
-  d.A_cold = p.Paterson_Budd_A_cold;
-  d.A_warm = p.Paterson_Budd_A_warm;
-  d.Q_cold = p.Paterson_Budd_Q_cold;
-  d.Q_warm = p.Paterson_Budd_Q_warm;
-  d.R = p.ideal_gas_constant;
-  d.T_crit = p.Paterson_Budd_critical_temperature;
-  {
-    const double beta_CC_grad = p.beta_CC * p.ice_density * p.standard_gravity;  // flowlaws.cc:71
-    d.pb_coeff = beta_CC_grad / (p.ice_density * p.standard_gravity);            // flowlaws.cc:242
-  }
-  d.A_T0 = softness_pb_host(p, p.water_melting_point_temperature);  // flowlaws.cc:219
-  d.ln_A_T0 = std::log(d.A_T0);
-  d.wf_coeff = p.gpbld_water_frac_coeff;
-  d.wf_limit = p.gpbld_water_frac_observed_limit;
-  d.A_glen = p.ice_softness;
-  d.hooke_A = p.Hooke_A;
-  d.hooke_Q = p.Hooke_Q;
-  d.hooke_C = p.Hooke_C;
-  d.hooke_k = p.Hooke_k;
-  d.hooke_Tr = p.Hooke_Tr;
-  d.sec_to_ka = 1.0 / (1000.0 * p.seconds_per_year);
-  d.hard_pow = -1.0 / p.sia_Glen_exponent;                         // flowlaws.cc:76
-  d.e_pow = std::pow(p.sia_enhancement_factor, -1.0 / p.sia_Glen_exponent);  // PISMStressBalance.cc:388
-  d.heat_exp = 0.5 * (1.0 / p.sia_Glen_exponent + 1.0);             // PISMStressBalance.cc:387
-  d.B_glen = std::pow(p.ice_softness, d.hard_pow);                  // flowlaws.cc:252
-  d.ln_A_cold = std::log(p.Paterson_Budd_A_cold);
-  d.ln_A_warm = std::log(p.Paterson_Budd_A_warm);
-  d.ln_A_glen = std::log(p.ice_softness);
-  d.ln_hooke_A = std::log(p.Hooke_A);
-  d.inv_ci = 1.0 / p.ice_specific_heat_capacity;
-  d.inv_L = 1.0 / p.water_latent_heat_fusion;
-  d.alpha_float = 1 - p.ice_density / p.sea_water_density;
-  d.sea_level = p.sea_level;
-  d.icefree_thk = p.mask_icefree_thickness_standard;
-  d.floating_thk = p.mask_is_floating_thickness_standard;
-  d.max_dt = p.maximum_time_step;
-  d.adapt_ratio = p.adaptive_timestepping_ratio;
-  d.Nx = -1;
-  d.Ny = -1;
-  d.use_age = 0;
-}
-
-// FlowLawUsesGrainSize (flowlaws.cc:33-42): only the Goldsby-Kohlstedt law
-// changes with grain size among the laws of flowlaw_factory.cc:77-83.
-bool law_uses_grain_size(int law) { return law == SIA_FLOW_GK; }
 
 // Reports the flags raised by every update since the last check (the
 // reference throws at once, SIAFD.cc:654; here a step that does not
@@ -272,18 +307,12 @@ int preprocess_bed(sia_handle *h, const double *bed) {
     h->dp.Ny = -1;
     return SIA_OK;
   }
-  int Nx = (int)std::ceil(p.bed_smoother_range / h->grid.dx);
-  int Ny = (int)std::ceil(p.bed_smoother_range / h->grid.dy);
-  if (Nx < 1) Nx = 1;
-  if (Ny < 1) Ny = 1;
-  if (Nx >= h->grid.Mx || Ny >= h->grid.My) {
+  double sc[3];
+  if (bed_smoother_setup(h->grid, p, h->dp, sc)) {
     return set_err(h, SIA_ERR_CONFIG,
                    "input Nx, Ny in bed smoother is too large because domain of smoothing exceeds IceGrid domain");
   }
-  h->dp.Nx = Nx;
-  h->dp.Ny = Ny;
-  const double n = p.sia_Glen_exponent, k = (n + 2) / n, s2 = k * (2 * n + 2) / (2 * n),
-               s3 = s2 * (3 * n + 2) / (3 * n), s4 = s3 * (4 * n + 2) / (4 * n);
+  const double s2 = sc[0], s3 = sc[1], s4 = sc[2];
   launch_bed_smoother(h->dp, s2, s3, s4, bed, h->topgs, h->maxtl, h->C2, h->C3, h->C4, h->stream);
   HIP_TRY(h, hipGetLastError());
   return SIA_OK;
@@ -604,15 +633,8 @@ void sia_params_default(sia_params *p) {
 int sia_create(const sia_grid *grid, const sia_params *params, sia_handle **out) {
   if (!grid || !params || !out || !grid->z) return SIA_ERR_ARGUMENT;
   *out = nullptr;
-  if (grid->Mx < 3 || grid->My < 3 || grid->Mz < 2 || !(grid->dx > 0) || !(grid->dy > 0)) return SIA_ERR_ARGUMENT;
-  // indices of staggered columns are 32-bit; z levels are staged in LDS
-  if (2.0 * grid->Mx * grid->My >= 2147483647.0 || grid->Mz > pism_sia::MAX_MZ) return SIA_ERR_ARGUMENT;
-  if (params->sia_flow_law < 0 || params->sia_flow_law > SIA_FLOW_GK) return SIA_ERR_CONFIG;
-  if (params->surface_gradient_method < 0 || params->surface_gradient_method > SIA_GRADIENT_ETA) return SIA_ERR_CONFIG;
-  if (params->enthalpy_converter < 0 || params->enthalpy_converter > SIA_EC_KIRCHHOFF) return SIA_ERR_CONFIG;
-  for (int k = 1; k < grid->Mz; k++) {
-    if (!(grid->z[k] > grid->z[k - 1])) return SIA_ERR_ARGUMENT;
-  }
+  const int rc0 = check_grid_params(grid, params);
+  if (rc0) return rc0;
   sia_handle *h = new (std::nothrow) sia_handle();
   if (!h) return SIA_ERR_OUT_OF_MEMORY;
   h->grid = *grid;
@@ -623,7 +645,7 @@ int sia_create(const sia_grid *grid, const sia_params *params, sia_handle **out)
     delete h;
     return SIA_ERR_HIP;
   }
-  fill_dev_params(h);
+  make_dev_params(h->grid, h->params, h->dp);
   const size_t n2 = (size_t)grid->Mx * grid->My;
   int rc;
   if ((rc = dalloc(h, &h->d_z, (size_t)grid->Mz)) || (rc = dalloc(h, &h->topgs, n2)) ||
