@@ -316,6 +316,33 @@ const char *sia_event_name(int event);
 /* Human-readable message for the last failing call on this handle. */
 const char *sia_last_error(const sia_handle *h);
 
+/* ---- flux-only update that allocates nothing ---------------------------------
+   No counterpart in the reference. For callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc): the fast (flux-only) SIAFD::update of sia_update(h, in, 1)
+   without a handle. The bed smoother, gradient, theta/thksmooth and the column
+   pass run on every call (no state is kept) with their intermediate fields in
+   one scratch array that the library holds on each GPU it runs on, for the
+   whole process (about 55 MB per GPU); the call makes no hipMalloc/hipFree.
+   Results are bitwise those of sia_update(fast = 1) on a handle with the same
+   grid and parameters (sia_diffusive_flux, sia_max_diffusivity_device).
+     grid: at most SIA_NOALLOC_MAX_COLUMNS map columns (Mx * My) and 4096
+           levels; larger grids return SIA_ERR_ARGUMENT. grid->z (host) is
+           copied on the stream.
+     in:   as for sia_update (device fields; bed_state_counter is ignored).
+     flux: device [Mx][My][2], written.  D_max: device scalar, written.
+     err_flags: device int or NULL: the update's error flags (bit 1: negative
+           thickness, 2: thickness above Lz, 4: negative omega; the
+           SIA_ERR_NEGATIVE_THICKNESS, _THICKNESS_ABOVE_LZ, _OMEGA_NEGATIVE
+           cases), read by the caller after the stream completes.
+     hip_stream: hipStream_t or NULL. The call does not synchronise.
+   Every call on a GPU uses that GPU's one scratch array: calls that overlap
+   on one GPU (other streams or threads) race and must be serialised by the
+   caller. */
+#define SIA_NOALLOC_MAX_COLUMNS 524288
+int sia_flux_update_noalloc(const sia_grid *grid, const sia_params *params, const sia_inputs *in, double *flux,
+                            double *D_max, int32_t *err_flags, void *hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,6 +61,8 @@ SIGNATURES = {
                                            C.POINTER(T.SiaMassAccounting)]),
     "sia_step": (C.c_int, [_vp, C.POINTER(T.SiaGeometry), _vp, C.c_int, C.c_double, _vp, _vp]),
     "sia_last_error": (C.c_char_p, [_vp]),
+    "sia_flux_update_noalloc": (C.c_int, [C.POINTER(T.SiaGrid), C.POINTER(T.SiaParams), C.POINTER(T.SiaInputs),
+                                          _vp, _vp, _vp, _vp]),
     "sia_profiling": (C.c_int, [_vp, C.c_int]),
     "sia_profiling_read": (C.c_int, [_vp, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     "sia_event_name": (C.c_char_p, [C.c_int]),
@@ -554,3 +556,18 @@ def generate_greenland_like(grid, params, i0=0, j0=0, Mx_full=1501, My_full=2801
     if rc != 0:
         raise SIAError(rc, "sia_workload_generate")
     return dict(H=H, b=b, h=h, mask=mask, E=E)
+
+
+# largest grid (Mx * My) of sia_flux_update_noalloc (include/pism_sia.h)
+NOALLOC_MAX_COLUMNS = 524288
+
+
+def flux_update_noalloc(grid, params, H, b, h, mask, E, flux, D_max, err_flags=None, age=None,
+                        no_model_mask=None, usurfstore=None, stream=None):
+    """sia_flux_update_noalloc: the flux-only update without a handle, written
+    into the caller's device tensors flux [Mx*My*2], D_max [1] (float64) and
+    err_flags [1] (int32, optional). Returns the status code (0 = OK)."""
+    inp = T.SiaInputs(ptr(H), ptr(b), ptr(h), ptr(mask), ptr(E), ptr(age), None, None, None, 0,
+                      ptr(no_model_mask), ptr(usurfstore))
+    return lib().sia_flux_update_noalloc(C.byref(grid), C.byref(params), C.byref(inp), ptr(flux), ptr(D_max),
+                                         ptr(err_flags), C.c_void_p(stream) if stream else None)
