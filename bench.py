@@ -24,10 +24,11 @@ algorithmic bytes (see algorithmic_bytes) over its average launch time; its
 "traffic" is the copy-calibrated FETCH_SIZE + WRITE_SIZE per launch from the
 committed rocprofv3 summary, when present.
 
-The CPU baseline times the oracle (the CPU restatement of the reference) on
-the host's cores, one worker process per core (at most 16), on seeded random
-windows of the same sheet, in the same update mode as value; it is a reported
-baseline, not the target.
+The CPU baseline times the oracle (the CPU restatement of the reference) as
+PISM runs on MPI ranks: oracle/sia_mpi under mpiexec, one rank per host core
+(at most 16), a 2-D block decomposition with width-2 periodic ghosts and a
+global max of D, on the whole sheet (fast updates) in the same update mode as
+value; it is a reported baseline, not the target.
 
 The default run also measures the batched ensemble (64 members of uneven
 sizes, one kernel launch per stage for all members, 50 steps captured in a
@@ -136,44 +137,56 @@ def stage_traffic(stage):
     return None
 
 
-def _cpu_windows(job):
-    """Worker of cpu_baseline: the oracle step (update, dt, mass continuity,
-    mask) on the given windows of the sheet, each regenerated on the CPU by
-    workloads.greenland_like; returns (columns, seconds in the oracle)."""
-    windows, win, full = job
-    import binding as oracle
-    import workloads as W
-    cols, secs = 0, 0.0
-    for i0, j0 in windows:
-        w = W.greenland_like(i0, i0 + win, j0, j0 + win)
-        st = {k: w.fields[k].copy() for k in ("H", "b", "h", "mask")}
-        t0 = time.perf_counter()
-        oracle.step(w.grid, w.params, st, w.fields["E"], full=full)
-        secs += time.perf_counter() - t0
-        cols += win * win
-    return cols, secs
+MPIEXEC = "/opt/conda/bin/mpiexec"
+MPI_DRIVER = os.path.join(REPO, "oracle", "sia_mpi")
+# window of the sheet timed by a full-update baseline (the whole sheet's 3-D
+# work fields would not fit a host): a quarter of the sheet around its centre
+FULL_WINDOW = (375, 700, 750, 1400)  # i0, j0, wx, wy
 
 
-def cpu_baseline(full=True, n_windows=192, win=128, seed=7, workers=None):
-    """The oracle (CPU restatement of the reference) on the host's cores: one
-    worker process per core (at most 16, the host share of one GPU), each
-    timing the oracle step on its share of seeded random windows of the sheet.
-    The workers run at once; the rate is the sum of their column-updates per
-    second of oracle time. Returns (rate, CPU seconds, cores, sample)."""
-    import multiprocessing as mp
-    import binding as oracle
-    oracle.build()
-    rng = np.random.default_rng(seed)
-    wins = [(int(rng.integers(0, MX - win)), int(rng.integers(0, MY - win))) for _ in range(n_windows)]
-    nw = workers or min(16, os.cpu_count() or 1)
-    jobs = [(wins[k::nw], win, full) for k in range(nw)]
-    with mp.get_context("spawn").Pool(nw) as pool:  # fresh interpreters: the workers never touch the GPU
-        res = pool.map(_cpu_windows, jobs)
-    rate = sum(c / t for c, t in res if t > 0)
-    secs = sum(t for _, t in res)
-    sample = "%d windows of %dx%d columns x %d levels at seeded random positions of the sheet (regenerated on " \
-             "the CPU), %s updates, %d worker processes" % (n_windows, win, win, MZ, "full" if full else "fast", nw)
-    return rate, secs, nw, sample
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
+def cpu_baseline(full=False, steps=3, warmup=1, ranks=None):
+    """The oracle (CPU restatement of the reference) as PISM runs on MPI ranks:
+    oracle/sia_mpi under MPICH's mpiexec, one rank per host core (at most 16,
+    the host share of one GPU), a 2-D block decomposition of the map plane with
+    periodic ghosts of width 2, the gradient's width-1 ghost update and
+    MPI_Allreduce(MAX) of D_max, each rank generating its own block. One step
+    is the SIA update (fast: the whole sheet; full: a quarter window, whose 3-D
+    work fields fit the host) + dt + mass continuity + mask, the median of
+    `steps` after `warmup`. Returns (rate, seconds, ranks, sample) or None
+    with the reason when MPICH or the driver is missing."""
+    import subprocess
+    if not os.path.exists(MPI_DRIVER):
+        subprocess.run(["make", "-s", "-C", os.path.join(REPO, "oracle")], capture_output=True)
+    if not (os.path.exists(MPIEXEC) and os.path.exists(MPI_DRIVER)):
+        return None, "MPICH (mpiexec) or the oracle's MPI driver is not available on this host"
+    n = ranks or min(16, os.cpu_count() or 1)
+    cmd = [MPIEXEC, "-n", str(n), MPI_DRIVER, "--steps", str(steps), "--warmup", str(warmup)]
+    if full:
+        i0, j0, wx, wy = FULL_WINDOW
+        cmd += ["--full", "1", "--i0", str(i0), "--j0", str(j0), "--wx", str(wx), "--wy", str(wy)]
+    t0 = time.perf_counter()
+    res = subprocess.run(cmd, capture_output=True, text=True, timeout=900, cwd="/tmp")
+    wall = time.perf_counter() - t0
+    line = [ln for ln in res.stdout.splitlines() if ln.startswith("{")]
+    if res.returncode != 0 or not line:
+        return None, "the MPI driver failed: " + (res.stdout + res.stderr)[-300:]
+    r = json.loads(line[-1])
+    sample = ("%s %dx%dx%d, %s updates; one step = SIA update + dt + mass continuity + mask; %d MPI ranks "
+              "(MPICH) in a %dx%d block decomposition with width-2 periodic ghosts, MPI_Allreduce(MAX) of D_max; "
+              "median of %d steps after %d warm-up (%.2f s/step); %s; %.1f s wall including generation"
+              % ("quarter window of the sheet" if full else "whole sheet", r["Mx"], r["My"], r["Mz"], r["mode"],
+                 r["ranks"], r["px"], r["py"], steps, warmup, r["median_step_s"], _cpu_model(), wall))
+    return (r["column_updates_per_s"], r["median_step_s"] * steps * r["ranks"], r["ranks"], sample), None
 
 
 def clone_geometry(S, f):
@@ -638,10 +651,15 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        log("CPU baseline (oracle, one worker process per core)")
-        v, secs, cores, sample = cpu_baseline(full=not fast_mode)
-        cpu = {"value": round(v, 1), "unit": "column-updates/s", "cores": cores, "kind": "port",
-               "sample": sample + "; %.1f s of CPU work" % secs}
+        log("CPU baseline (the oracle under mpiexec, one rank per host core)")
+        res, why = cpu_baseline(full=not fast_mode)
+        if res is None:
+            cpu = {"value": None, "unit": "column-updates/s", "cores": 0, "kind": "port", "sample": why}
+        else:
+            v, secs, cores, sample = res
+            cpu = {"value": round(v, 1), "unit": "column-updates/s", "cores": cores, "kind": "port",
+                   "sample": sample + "; %.1f s of CPU work timed" % secs}
+        log("  %s" % cpu)
     path = ("surrogate (BF16 MFMA MLP over FP64 per-level moments) on fast updates"
             if args.path == "surrogate" else "plain FP64 %s updates" % args.mode)
     out = {

@@ -1,0 +1,505 @@
+/*
+ * sia_mpi.c -- MPI driver of the CPU oracle (TEST INFRASTRUCTURE ONLY): the
+ * CPU baseline of bench.py and a check of the oracle's decomposition
+ * independence. The product never runs or links it.
+ *
+ * It runs the SIA-only time step of the oracle (sia_oracle.c) the way PISM
+ * runs it on MPI ranks: a 2-D block decomposition of the map plane with
+ * periodic ghosts of width 2 (IceGrid.cc:436-510; iceModelVec.cc:628-632),
+ * the surface gradient's ghost update of width 1 (SIAFD.cc:501-502), the
+ * global maximum of D (SIAFD.cc:721, GlobalMax) and the thickness ghost
+ * update after mass continuity (iMgeometry.cc:812). Per step, on every rank:
+ *   exchange H (width 2); mask and surface on the ghosted block
+ *   (iMgeometry.cc:56-149); theta and thksmooth (PISMBedSmoother.cc:286-420);
+ *   surface gradient; exchange h_x, h_y (width 1); diffusive flux (fast update,
+ *   SIAFD.cc:620-721) with D = 0 on the domain's last row and column;
+ *   MPI_Allreduce(MAX) of D_max; dt (iMadaptive.cc:157-171, 214-327); mass
+ *   continuity (iMgeometry.cc:497-822).
+ * The bed smoother runs once before the timed steps, as in PISM (the bed does
+ * not change). Each rank generates its own block of the 1501x2801x401
+ * polythermal sheet (or of a window of it) with the counter-based generator of
+ * pismv0.7.3_amd/workloads.py:greenland_like, restated here in C.
+ *
+ * Usage: mpiexec -n N sia_mpi [--Mx 1501 --My 2801 --Mz 401] [--i0 I --j0 J
+ *        --wx W --wy W] [--px P --py Q] [--steps K] [--warmup W] [--full 1] [--dump FILE]
+ * --full 1 makes every update a full one: also I, u, v (with the width-1
+ * ghost update of u, v), w and the strain heating (PISMStressBalance.cc:80-116).
+ * Rank 0 prints one JSON line: ranks, decomposition, columns, per-step times
+ * (max over ranks), column-updates/s of the median step.
+ */
+#include <math.h>
+#include <mpi.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "sia_oracle.h"
+
+#define G 2 /* ghost width of H, h, b, mask, E (PISM's grid_max_stencil_width) */
+
+/* ---------------------------------------------------------------- generator */
+static const uint64_t M1 = 0xBF58476D1CE4E5B9ull, M2 = 0x94D049BB133111EBull, G1 = 0x9E3779B97F4A7C15ull,
+                      G2 = 0xD1B54A32D192ED03ull;
+enum { S_BUMP_A, S_BUMP_W, S_BUMP_CX, S_BUMP_CY, S_DOME, S_TEMPERATE, S_OMEGA, S_TSURF };
+
+static double uniform01(uint64_t seed, uint64_t stream, uint64_t idx) {
+  uint64_t z = seed * G1 + stream * G2 + idx;
+  z = (z ^ (z >> 30)) * M1;
+  z = (z ^ (z >> 27)) * M2;
+  z = z ^ (z >> 31);
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+typedef struct {
+  int Mx, My, Mz, nb;
+  uint64_t seed;
+  double dx, Hmax, Lz;
+  double *a, *w, *cx, *cy;
+  double x0f, y0f, ph1, ph2;
+} sheet;
+
+static void sheet_init(sheet *s) {
+  const double Lx = 0.5 * (s->Mx - 1) * s->dx, Ly = 0.5 * (s->My - 1) * s->dx;
+  s->a = malloc(sizeof(double) * s->nb);
+  s->w = malloc(sizeof(double) * s->nb);
+  s->cx = malloc(sizeof(double) * s->nb);
+  s->cy = malloc(sizeof(double) * s->nb);
+  for (int m = 0; m < s->nb; m++) {
+    s->a[m] = -500.0 + 2000.0 * uniform01(s->seed, S_BUMP_A, m);
+    s->w[m] = 5e3 + 75e3 * uniform01(s->seed, S_BUMP_W, m);
+    s->cx[m] = -Lx + 2 * Lx * uniform01(s->seed, S_BUMP_CX, m);
+    s->cy[m] = -Ly + 2 * Ly * uniform01(s->seed, S_BUMP_CY, m);
+  }
+  s->x0f = -0.05 + 0.1 * uniform01(s->seed, S_DOME, 0);
+  s->y0f = -0.05 + 0.1 * uniform01(s->seed, S_DOME, 1);
+  s->ph1 = 2 * M_PI * uniform01(s->seed, S_DOME, 2);
+  s->ph2 = 2 * M_PI * uniform01(s->seed, S_DOME, 3);
+}
+
+/* bed and thickness at global map point (gi, gj) */
+static void sheet_point(const sheet *s, int gi, int gj, double *b_out, double *H_out) {
+  const double Lx = 0.5 * (s->Mx - 1) * s->dx, Ly = 0.5 * (s->My - 1) * s->dx;
+  const double X = -Lx + gi * s->dx, Y = -Ly + gj * s->dx;
+  double b = 0.0;
+  for (int m = 0; m < s->nb; m++) {
+    const double d2 = (X - s->cx[m]) * (X - s->cx[m]) + (Y - s->cy[m]) * (Y - s->cy[m]);
+    b += s->a[m] * exp(-d2 / (2.0 * s->w[m] * s->w[m]));
+  }
+  const double Rx = 0.78 * Lx, Ry = 0.78 * Ly;
+  const double xn = (X - s->x0f * Lx) / Rx, yn = (Y - s->y0f * Ly) / Ry;
+  const double rho = sqrt(xn * xn + yn * yn), phi = atan2(yn, xn);
+  const double sc = 1.0 + 0.06 * sin(3.0 * phi + s->ph1) + 0.04 * sin(5.0 * phi + s->ph2);
+  const double r = rho / sc;
+  double H = 0.0;
+  if (r < 1.0) {
+    const double t = 1.0 - pow(r, 4.0 / 3.0);
+    H = s->Hmax * pow(t > 0.0 ? t : 0.0, 0.375);
+  }
+  *b_out = b;
+  *H_out = H;
+}
+
+/* enthalpy column of global point (gi, gj) with thickness H (workloads.py:enthalpy_columns) */
+static void sheet_enthalpy(const sheet *s, const sia_params *P, const double *z, int gi, int gj, double H,
+                           double *E) {
+  const uint64_t col = (uint64_t)gi * (uint64_t)s->My + (uint64_t)gj;
+  const double f_t = 0.1 * uniform01(s->seed, S_TEMPERATE, col);
+  const double omega = 0.01 * uniform01(s->seed, S_OMEGA, col);
+  const double T_s = 238.15 + 10.0 * uniform01(s->seed, S_TSURF, col);
+  const double ci = P->ice_specific_heat_capacity, T0 = P->enthalpy_converter_reference_temperature,
+               L = P->water_latent_heat_fusion;
+  const double rho_g = P->ice_density * P->standard_gravity, beta = P->beta_CC,
+               Tmelt = P->water_melting_point_temperature, pair = P->surface_pressure;
+  const double z_t = f_t * H;
+  const double p_t = (H - z_t > 0.0) ? pair + rho_g * (H - z_t) : pair;
+  const double T_base = Tmelt - beta * p_t - 1.0;
+  const double E_air = ci * (T_s - T0);
+  const double denom = (H - z_t > 0.0) ? H - z_t : 1.0;
+  for (int k = 0; k < s->Mz; k++) {
+    const double depth = H - z[k];
+    const double p = (depth > 0.0) ? pair + rho_g * depth : pair;
+    const double E_cts = ci * ((Tmelt - beta * p) - T0);
+    const double E_temp = E_cts + omega * L;
+    double frac = depth / denom;
+    frac = frac < 0.0 ? 0.0 : (frac > 1.0 ? 1.0 : frac);
+    const double T_cold = T_s + (T_base - T_s) * frac * frac;
+    const double E_cold = ci * (T_cold - T0);
+    double Ek = (z[k] <= z_t) ? E_temp : E_cold;
+    if (z[k] > H) Ek = E_air;
+    E[k] = (H > 0.0) ? Ek : E_air;
+  }
+}
+
+/* the pism_config.cdl defaults of the parameters (SI units), as
+   sia_params_default of include/pism_sia.h gives them */
+static void params_default(sia_params *p) {
+  memset(p, 0, sizeof(*p));
+  const double secpera = 3.15569259747e7;  /* UDUNITS "year" */
+  p->sia_flow_law = SIA_FLOW_GPBLD;  /* cdl:1094 */
+  p->surface_gradient_method = SIA_GRADIENT_HASELOFF;  /* cdl:1272 */
+  p->enthalpy_converter = SIA_EC_DEFAULT;
+  p->compute_grain_size_using_age = 0;  /* cdl:309 */
+  p->do_age = 0;  /* cdl:34 */
+  p->is_dry_simulation = 0;  /* cdl:136 */
+  p->include_bmr_in_continuity = 1;  /* cdl:148 */
+  p->sia_enhancement_factor = 1.0;  /* cdl:281 */
+  p->sia_Glen_exponent = 3.0;  /* cdl:1009 */
+  p->bed_smoother_range = 5.0e3;  /* cdl:560 */
+  p->ice_grain_size = 1.0;  /* cdl:304 */
+  p->ice_softness = 3.1689e-24;  /* cdl:1020 */
+  p->Paterson_Budd_A_cold = 3.61e-13;  /* cdl:1060 */
+  p->Paterson_Budd_A_warm = 1.73e3;  /* cdl:1065 */
+  p->Paterson_Budd_Q_cold = 6.0e4;  /* cdl:1070 */
+  p->Paterson_Budd_Q_warm = 13.9e4;  /* cdl:1075 */
+  p->Paterson_Budd_critical_temperature = 263.15;  /* cdl:1080 */
+  p->gpbld_water_frac_coeff = 181.25;  /* cdl:1115 */
+  p->gpbld_water_frac_observed_limit = 0.01;  /* cdl:1120 */
+  p->Hooke_A = 4.42165e-9;  /* cdl:1025 */
+  p->Hooke_Q = 7.88e4;  /* cdl:1030 */
+  p->Hooke_C = 0.16612;  /* cdl:1035 */
+  p->Hooke_k = 1.17;  /* cdl:1040 */
+  p->Hooke_Tr = 273.39;  /* cdl:1045 */
+  p->ideal_gas_constant = 8.31441;  /* cdl:1193 */
+  p->beta_CC = 7.9e-8;  /* cdl:963 */
+  p->ice_specific_heat_capacity = 2009.0;  /* cdl:1003 */
+  p->water_specific_heat_capacity = 4170.0;  /* cdl:988 */
+  p->water_latent_heat_fusion = 3.34e5;  /* cdl:983 */
+  p->surface_pressure = 0.0;  /* cdl:968 */
+  p->ice_density = 910.0;  /* cdl:993 */
+  p->standard_gravity = 9.81;  /* cdl:1188 */
+  p->water_melting_point_temperature = 273.15;  /* cdl:973 */
+  p->enthalpy_converter_reference_temperature = 223.15;  /* cdl:978 */
+  p->sea_water_density = 1028.0;  /* cdl:1140 */
+  p->sea_level = 0.0;
+  p->mask_icefree_thickness_standard = 0.01;  /* cdl:1225 */
+  p->mask_is_floating_thickness_standard = 1.0;  /* cdl:1230 */
+  p->adaptive_timestepping_ratio = 0.12;  /* cdl:330 */
+  p->maximum_time_step = 60.0 * secpera;  /* cdl:341 */
+  p->seconds_per_year = secpera;
+}
+
+/* ---------------------------------------------------------------- decomposition */
+typedef struct {
+  MPI_Comm comm;
+  int rank, px, py, ri, rj;
+  int nbr_im, nbr_ip, nbr_jm, nbr_jp; /* ranks of the blocks at i-1, i+1, j-1, j+1 (periodic) */
+  int gi0, gj0, mx, my;               /* owned block: global [gi0, gi0+mx) x [gj0, gj0+my) */
+  int Mxp, Myp;                       /* padded local sizes mx + 2G, my + 2G */
+  double *buf_s, *buf_r;
+} decomp;
+
+static int block_start(int M, int p, int r) { return (int)(((long)M * r) / p); }
+
+/* ncmax: the most values per point of a field that exchange() will move */
+static void decomp_init(decomp *d, int Mx, int My, int px, int py, int ncmax) {
+  MPI_Comm_rank(MPI_COMM_WORLD, &d->rank);
+  int n;
+  MPI_Comm_size(MPI_COMM_WORLD, &n);
+  if (px <= 0 || py <= 0) {
+    /* PETSc-like split: px / py close to Mx / My */
+    int best = 1;
+    double bestc = 1e300;
+    for (int a = 1; a <= n; a++) {
+      if (n % a) continue;
+      const double c = fabs(log(((double)Mx / a) / ((double)My / (n / a))));
+      if (c < bestc) {
+        bestc = c;
+        best = a;
+      }
+    }
+    px = best;
+    py = n / best;
+  }
+  d->comm = MPI_COMM_WORLD;
+  d->px = px;
+  d->py = py;
+  d->ri = d->rank / py;
+  d->rj = d->rank % py;
+  d->gi0 = block_start(Mx, px, d->ri);
+  d->mx = block_start(Mx, px, d->ri + 1) - d->gi0;
+  d->gj0 = block_start(My, py, d->rj);
+  d->my = block_start(My, py, d->rj + 1) - d->gj0;
+  d->Mxp = d->mx + 2 * G;
+  d->Myp = d->my + 2 * G;
+  d->nbr_im = ((d->ri + px - 1) % px) * py + d->rj;
+  d->nbr_ip = ((d->ri + 1) % px) * py + d->rj;
+  d->nbr_jm = d->ri * py + (d->rj + py - 1) % py;
+  d->nbr_jp = d->ri * py + (d->rj + 1) % py;
+  /* the j-direction exchange packs (mx + 2 w) w nc values per side, w <= G */
+  const size_t nbuf = (size_t)2 * d->Mxp * G * ncmax;
+  d->buf_s = malloc(sizeof(double) * nbuf);
+  d->buf_r = malloc(sizeof(double) * nbuf);
+}
+
+/* periodic ghost exchange of width w of a padded field with nc values per
+   point (a[((i * Myp) + j) * nc + c]): rows in i first, then columns in j over
+   the full padded height (which fills the corners) */
+static void exchange(decomp *d, double *a, int nc, int w) {
+  const int Myp = d->Myp, mx = d->mx, my = d->my;
+  const int row = Myp * nc;
+  MPI_Request rq[4];
+  /* i direction: owned rows [G, G+w) go to i-1, [G+mx-w, G+mx) to i+1 */
+  MPI_Irecv(a + (size_t)(G + mx) * row, w * row, MPI_DOUBLE, d->nbr_ip, 0, d->comm, &rq[0]);
+  MPI_Irecv(a + (size_t)(G - w) * row, w * row, MPI_DOUBLE, d->nbr_im, 1, d->comm, &rq[1]);
+  MPI_Isend(a + (size_t)G * row, w * row, MPI_DOUBLE, d->nbr_im, 0, d->comm, &rq[2]);
+  MPI_Isend(a + (size_t)(G + mx - w) * row, w * row, MPI_DOUBLE, d->nbr_ip, 1, d->comm, &rq[3]);
+  MPI_Waitall(4, rq, MPI_STATUSES_IGNORE);
+  /* j direction over padded rows [G - w, G + mx + w) */
+  const int i0 = G - w, ni = mx + 2 * w, cnt = ni * w * nc;
+  double *s_lo = d->buf_s, *s_hi = d->buf_s + cnt, *r_lo = d->buf_r, *r_hi = d->buf_r + cnt;
+  for (int i = 0; i < ni; i++) {
+    for (int t = 0; t < w; t++) {
+      for (int c = 0; c < nc; c++) {
+        s_lo[(i * w + t) * nc + c] = a[((size_t)(i0 + i) * Myp + G + t) * nc + c];
+        s_hi[(i * w + t) * nc + c] = a[((size_t)(i0 + i) * Myp + G + my - w + t) * nc + c];
+      }
+    }
+  }
+  MPI_Irecv(r_hi, cnt, MPI_DOUBLE, d->nbr_jp, 2, d->comm, &rq[0]);
+  MPI_Irecv(r_lo, cnt, MPI_DOUBLE, d->nbr_jm, 3, d->comm, &rq[1]);
+  MPI_Isend(s_lo, cnt, MPI_DOUBLE, d->nbr_jm, 2, d->comm, &rq[2]);
+  MPI_Isend(s_hi, cnt, MPI_DOUBLE, d->nbr_jp, 3, d->comm, &rq[3]);
+  MPI_Waitall(4, rq, MPI_STATUSES_IGNORE);
+  for (int i = 0; i < ni; i++) {
+    for (int t = 0; t < w; t++) {
+      for (int c = 0; c < nc; c++) {
+        a[((size_t)(i0 + i) * Myp + G + my + t) * nc + c] = r_hi[(i * w + t) * nc + c];
+        a[((size_t)(i0 + i) * Myp + G - w + t) * nc + c] = r_lo[(i * w + t) * nc + c];
+      }
+    }
+  }
+}
+
+static double now(void) { return MPI_Wtime(); }
+
+static int cmp_double(const void *x, const void *y) {
+  const double a = *(const double *)x, b = *(const double *)y;
+  return (a > b) - (a < b);
+}
+
+int main(int argc, char **argv) {
+  MPI_Init(&argc, &argv);
+  int Mx = 1501, My = 2801, Mz = 401, i0 = 0, j0 = 0, wx = -1, wy = -1, px = 0, py = 0, steps = 5, warmup = 1,
+      full = 0;
+  const char *dump = NULL;
+  for (int a = 1; a + 1 < argc; a += 2) {
+    const char *k = argv[a];
+    const int v = atoi(argv[a + 1]);
+    if (!strcmp(k, "--Mx")) Mx = v;
+    else if (!strcmp(k, "--My")) My = v;
+    else if (!strcmp(k, "--Mz")) Mz = v;
+    else if (!strcmp(k, "--i0")) i0 = v;
+    else if (!strcmp(k, "--j0")) j0 = v;
+    else if (!strcmp(k, "--wx")) wx = v;
+    else if (!strcmp(k, "--wy")) wy = v;
+    else if (!strcmp(k, "--px")) px = v;
+    else if (!strcmp(k, "--py")) py = v;
+    else if (!strcmp(k, "--steps")) steps = v;
+    else if (!strcmp(k, "--warmup")) warmup = v;
+    else if (!strcmp(k, "--dump")) dump = argv[a + 1];
+    else if (!strcmp(k, "--full")) full = v;
+  }
+  /* the simulated grid: the whole sheet, or the window [i0, i0+wx) x [j0, j0+wy) of it */
+  const int Gx = wx > 0 ? wx : Mx, Gy = wy > 0 ? wy : My;
+  sheet sh = {Mx, My, Mz, 200, 20261203ull, 1000.0, 3200.0, 4000.0, NULL, NULL, NULL, NULL, 0, 0, 0, 0};
+  sheet_init(&sh);
+  sia_params P;
+  params_default(&P);
+  P.sia_flow_law = SIA_FLOW_GPBLD; /* workloads.gl_params: GPBLD, e = 3, n = 3, dry */
+  P.sia_enhancement_factor = 3.0;
+  P.sia_Glen_exponent = 3.0;
+  P.is_dry_simulation = 1;
+  double *z = malloc(sizeof(double) * Mz);
+  for (int k = 0; k < Mz; k++) z[k] = sh.Lz / (Mz - 1) * k; /* IceGrid.cc:333-376, EQUAL */
+  z[Mz - 1] = sh.Lz;
+
+  decomp d;
+  decomp_init(&d, Gx, Gy, px, py, full ? Mz : 2);
+  const int Mxp = d.Mxp, Myp = d.Myp;
+  const size_t n2 = (size_t)Mxp * Myp;
+  sia_grid g = {Mxp, Myp, Mz, 0, sh.dx, sh.dx, z};
+
+  /* bed smoother (once): box of half-width N over in-domain points, computed
+     on this rank's block from the bed on a halo of width G + N */
+  const int N = (int)ceil(P.bed_smoother_range / sh.dx);
+  const int hb = G + N, Bx = d.mx + 2 * hb, By = d.my + 2 * hb;
+  double *bb = malloc(sizeof(double) * (size_t)Bx * By), *Hb = malloc(sizeof(double) * (size_t)Bx * By);
+  for (int i = 0; i < Bx; i++) {
+    for (int j = 0; j < By; j++) {
+      const int gi = d.gi0 + i - hb, gj = d.gj0 + j - hb; /* in the simulated grid */
+      const int wi = ((gi % Gx) + Gx) % Gx, wj = ((gj % Gy) + Gy) % Gy;
+      sheet_point(&sh, i0 + wi, j0 + wj, &bb[(size_t)i * By + j], &Hb[(size_t)i * By + j]);
+    }
+  }
+  double *H = malloc(sizeof(double) * n2), *b = malloc(sizeof(double) * n2), *h = malloc(sizeof(double) * n2),
+         *mask = malloc(sizeof(double) * n2), *E = malloc(sizeof(double) * n2 * Mz);
+  double *topgs = malloc(sizeof(double) * n2), *maxtl = malloc(sizeof(double) * n2),
+         *C2 = malloc(sizeof(double) * n2), *C3 = malloc(sizeof(double) * n2), *C4 = malloc(sizeof(double) * n2);
+  {
+    /* BedSmoother::smooth_the_bed_on_proc0 / compute_coefficients_on_proc0
+       (PISMBedSmoother.cc:162-269), restricted to the padded block */
+    const double n = P.sia_Glen_exponent, kk = (n + 2) / n, s2 = kk * (2 * n + 2) / (2 * n),
+                 s3 = s2 * (3 * n + 2) / (3 * n), s4 = s3 * (4 * n + 2) / (4 * n);
+    for (int i = 0; i < Mxp; i++) {
+      for (int j = 0; j < Myp; j++) {
+        const int gi = d.gi0 + i - G, gj = d.gj0 + j - G;
+        const int wi = ((gi % Gx) + Gx) % Gx, wj = ((gj % Gy) + Gy) % Gy;
+        double sum = 0.0, count = 0.0;
+        for (int r = -N; r <= N; r++) {
+          for (int s = -N; s <= N; s++) {
+            if (wi + r >= 0 && wi + r < Gx && wj + s >= 0 && wj + s < Gy) {
+              sum += bb[(size_t)(i + N + r) * By + (j + N + s)];
+              count += 1.0;
+            }
+          }
+        }
+        const double bs = sum / count;
+        double mt = 0.0, q2 = 0.0, q3 = 0.0, q4 = 0.0;
+        for (int r = -N; r <= N; r++) {
+          for (int s = -N; s <= N; s++) {
+            if (wi + r >= 0 && wi + r < Gx && wj + s >= 0 && wj + s < Gy) {
+              const double tl = bb[(size_t)(i + N + r) * By + (j + N + s)] - bs;
+              mt = (mt < tl) ? tl : mt;
+              const double tl2 = tl * tl;
+              q2 += tl2;
+              q3 += tl2 * tl;
+              q4 += tl2 * tl2;
+            }
+          }
+        }
+        const size_t q = (size_t)i * Myp + j;
+        topgs[q] = bs;
+        maxtl[q] = mt;
+        C2[q] = (q2 / count) * s2;
+        C3[q] = (q3 / count) * s3;
+        C4[q] = (q4 / count) * s4;
+        b[q] = bb[(size_t)(i + N) * By + (j + N)];
+        H[q] = Hb[(size_t)(i + N) * By + (j + N)];
+        sheet_enthalpy(&sh, &P, z, i0 + wi, j0 + wj, H[q], E + q * Mz);
+      }
+    }
+  }
+  free(bb);
+  free(Hb);
+  double *theta = malloc(sizeof(double) * n2), *thks = malloc(sizeof(double) * n2),
+         *hx = malloc(sizeof(double) * 2 * n2), *hy = malloc(sizeof(double) * 2 * n2),
+         *flux = malloc(sizeof(double) * 2 * n2), *Dst = malloc(sizeof(double) * 2 * n2),
+         *Hn = malloc(sizeof(double) * n2);
+  double acc[8];
+  /* full updates: delta, I, u, v, w and the strain heating on the padded block */
+  double *d0 = NULL, *d1 = NULL, *I0 = NULL, *I1 = NULL, *u = NULL, *v = NULL, *w = NULL, *Sig = NULL;
+  if (full) {
+    double **f3[] = {&d0, &d1, &I0, &I1, &u, &v, &w, &Sig};
+    for (int t = 0; t < 8; t++) *f3[t] = malloc(sizeof(double) * n2 * Mz);
+  }
+  int rc = or_update_mask_surface(&g, &P, b, H, mask, h);
+  double *ts = malloc(sizeof(double) * (steps + warmup)), tsum = 0.0;
+  for (int s = 0; s < warmup + steps && rc == 0; s++) {
+    MPI_Barrier(d.comm);
+    const double t0 = now();
+    if (s > 0) {
+      exchange(&d, H, 1, G); /* H.update_ghosts() after mass continuity */
+      rc |= or_update_mask_surface(&g, &P, b, H, mask, h);
+    }
+    rc |= or_theta(&g, &P, N, N, h, topgs, maxtl, C2, C3, C4, theta);
+    rc |= or_thksmooth(&g, h, H, mask, maxtl, topgs, thks);
+    rc |= or_surface_gradient(&g, &P, H, b, h, mask, hx, hy);
+    exchange(&d, hx, 2, 1); /* h_x.update_ghosts(), h_y.update_ghosts() */
+    exchange(&d, hy, 2, 1);
+    double dmax_local = 0.0;
+    rc |= or_diffusive_flux(&g, &P, hx, hy, thks, theta, E, NULL, full, flux, Dst, d0, d1, &dmax_local);
+    if (full) {
+      /* SIAFD::compute_I, compute_3d_horizontal_velocity and u, v ghosts
+         (SIAFD.cc:819-963), then StressBalance's w and strain heating
+         (PISMStressBalance.cc:192-287, 370-500) */
+      rc |= or_compute_I(&g, thks, d0, d1, I0, I1);
+      rc |= or_horizontal_velocity(&g, hx, hy, I0, I1, NULL, NULL, u, v);
+      exchange(&d, u, Mz, 1);
+      exchange(&d, v, Mz, 1);
+      rc |= or_vertical_velocity(&g, mask, u, v, NULL, w);
+      rc |= or_strain_heating(&g, &P, H, mask, E, u, v, Sig);
+    }
+    /* D = 0 on the last row and column of the domain (SIAFD.cc:694-697), max over owned faces */
+    dmax_local = 0.0;
+    for (int i = 0; i < Mxp; i++) {
+      for (int j = 0; j < Myp; j++) {
+        const int gi = (((d.gi0 + i - G) % Gx) + Gx) % Gx, gj = (((d.gj0 + j - G) % Gy) + Gy) % Gy;
+        const size_t q = (size_t)i * Myp + j;
+        if (gi >= Gx - 1 || gj >= Gy - 1) {
+          flux[2 * q] = 0.0;
+          flux[2 * q + 1] = 0.0;
+        } else if (i >= G && i < G + d.mx && j >= G && j < G + d.my) {
+          for (int o = 0; o < 2; o++) dmax_local = dmax_local < Dst[2 * q + o] ? Dst[2 * q + o] : dmax_local;
+        }
+      }
+    }
+    double D_max = 0.0;
+    MPI_Allreduce(&dmax_local, &D_max, 1, MPI_DOUBLE, MPI_MAX, d.comm); /* GlobalMax */
+    const double dt = or_max_timestep(&g, &P, D_max, 0.0, NULL, NULL, NULL);
+    rc |= or_mass_continuity(&g, &P, dt, H, flux, NULL, NULL, mask, NULL, NULL, Hn, acc);
+    memcpy(H, Hn, sizeof(double) * n2);
+    const double t1 = now() - t0;
+    double tmax = 0.0;
+    MPI_Allreduce(&t1, &tmax, 1, MPI_DOUBLE, MPI_MAX, d.comm);
+    ts[s] = tmax;
+    if (s >= warmup) tsum += tmax;
+  }
+  int rc_all = 0;
+  MPI_Allreduce(&rc, &rc_all, 1, MPI_INT, MPI_BOR, d.comm);
+  if (dump && rc_all == 0) {
+    /* the owned thickness of every rank, gathered into a [Gx][Gy] file on rank 0 */
+    exchange(&d, H, 1, G);
+    double *own = malloc(sizeof(double) * (size_t)d.mx * d.my);
+    for (int i = 0; i < d.mx; i++) {
+      for (int j = 0; j < d.my; j++) own[(size_t)i * d.my + j] = H[(size_t)(i + G) * Myp + j + G];
+    }
+    int nr;
+    MPI_Comm_size(d.comm, &nr);
+    if (d.rank == 0) {
+      double *all = malloc(sizeof(double) * (size_t)Gx * Gy);
+      for (int r = 0; r < nr; r++) {
+        const int ri = r / d.py, rj = r % d.py;
+        const int a0 = block_start(Gx, d.px, ri), am = block_start(Gx, d.px, ri + 1) - a0;
+        const int b0 = block_start(Gy, d.py, rj), bm = block_start(Gy, d.py, rj + 1) - b0;
+        double *blk = own;
+        if (r != 0) {
+          blk = malloc(sizeof(double) * (size_t)am * bm);
+          MPI_Recv(blk, am * bm, MPI_DOUBLE, r, 9, d.comm, MPI_STATUS_IGNORE);
+        }
+        for (int i = 0; i < am; i++) memcpy(all + (size_t)(a0 + i) * Gy + b0, blk + (size_t)i * bm, sizeof(double) * bm);
+        if (r != 0) free(blk);
+      }
+      FILE *f = fopen(dump, "wb");
+      if (f) {
+        fwrite(all, sizeof(double), (size_t)Gx * Gy, f);
+        fclose(f);
+      }
+      free(all);
+    } else {
+      MPI_Send(own, d.mx * d.my, MPI_DOUBLE, 0, 9, d.comm);
+    }
+    free(own);
+  }
+  if (d.rank == 0) {
+    int nr;
+    MPI_Comm_size(d.comm, &nr);
+    double med = 0.0;
+    if (steps > 0) {
+      double *tt = malloc(sizeof(double) * steps);
+      memcpy(tt, ts + warmup, sizeof(double) * steps);
+      qsort(tt, steps, sizeof(double), cmp_double);
+      med = (steps % 2) ? tt[steps / 2] : 0.5 * (tt[steps / 2 - 1] + tt[steps / 2]);
+      free(tt);
+    }
+    printf("{\"status\": %d, \"ranks\": %d, \"px\": %d, \"py\": %d, \"Mx\": %d, \"My\": %d, \"Mz\": %d, "
+           "\"columns\": %ld, \"mode\": \"%s\", \"steps\": %d, \"warmup\": %d, \"median_step_s\": %.6f, "
+           "\"mean_step_s\": %.6f, \"column_updates_per_s\": %.1f}\n",
+           rc_all, nr, d.px, d.py, Gx, Gy, Mz, (long)Gx * Gy, full ? "full" : "fast", steps, warmup, med,
+           steps ? tsum / steps : 0.0,
+           med > 0 ? (double)Gx * Gy / med : 0.0);
+    fflush(stdout);
+  }
+  MPI_Finalize();
+  return rc_all ? 1 : 0;
+}
