@@ -67,50 +67,28 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def staggered_ks(thks, z):
-    """kBelowHeight (IceGrid.cc:379-391, clamped to Mz-2) of every staggered
-    column, from the smoothed thickness the library used (thk = mean of the
-    two cells, SIAFD.cc:631-633); -1 where thk == 0 (delta = 0, only I[0])."""
-    Mx, My = thks.shape
-    out = []
-    for q2 in (np.roll(thks, -1, axis=0), np.roll(thks, -1, axis=1)):
-        thk = 0.5 * (thks + q2)
-        ks = np.clip(np.searchsorted(z, thk, side="right") - 1, 0, len(z) - 2)
-        out.append(np.where(thk > 0.0, ks, -1))
-    return np.stack(out, axis=-1)  # [Mx, My, 2]
-
-
-def algorithmic_bytes(thks, z, Mz, full=True, H=None):
-    """Bytes each stage must move at least once, for one update of the sheet.
-
-    SIA flux (column kernel): E of each map column up to the largest ks of the
-    four staggered columns that read it (8 B a level); I of each staggered
-    column up to its ks (8 B a level, full updates); 2-D: thksmooth, theta,
-    h_x, h_y read, flux, D_stag, ks written (88 B a map column).
-    SIA 3D hor. vel. (one pass for u, v, w and the strain heating): u, v, w
-    and Sigma written at every level (32 Mz B a map column); I of every
-    staggered column up to its ks; E of each map column up to kBelowHeight(H)
-    (8 B a level); ks, h_x, h_y, mask and H read (56 B a map column)."""
-    ks = staggered_ks(thks, z)  # [Mx, My, 2]
-    lev_I = np.where(ks < 0, 1, ks + 1)  # thk == 0 stores I[0] only
-    # E levels of map column (i,j): faces (i,j,0), (i,j,1), (i-1,j,0), (i,j-1,1)
-    kE = np.maximum.reduce([ks[:, :, 0], ks[:, :, 1], np.roll(ks[:, :, 0], 1, axis=0),
-                            np.roll(ks[:, :, 1], 1, axis=1)])
-    e_bytes = 8.0 * float(np.where(kE < 0, 0, kE + 1).sum())
-    i_bytes = 8.0 * float(lev_I.sum())
-    n = thks.size
-    flux = e_bytes + 88.0 * n + (i_bytes if full else 0.0)
-    out = {"SIA flux": flux}
+def algorithmic_bytes(H, z, Mz, full=True):
+    """The bytes an update must move at least once (the algorithmic bytes of
+    the column-update unit): per map column c, with ks_c = kBelowHeight(H_c)
+    (IceGrid.cc:379-391, clamped to Mz-2),
+        B_fast(c) = 8 (ks_c + 1) + 88,   B_full(c) = B_fast(c) + 24 Mz:
+    E read up to ks_c; 88 B of 2-D fields (H, b, h, mask and the five bed
+    smoother fields read, the two fluxes written); u, v, w written at every
+    level. Intermediates (delta, I, the surrogate's feature rows) are not
+    algorithmic. Returns the step's bytes ("step") and the share each kernel
+    stage must move:
+      SIA flux (FP64 column kernel): E to ks_c + 88 B;
+      SIA 3D hor. vel. (u, v, w and the strain heating Sigma in one pass):
+        32 Mz B written + E to ks_c (Sigma) + 56 B of 2-D inputs;
+      surrogate features: E to ks_c + 48 B (thksmooth, theta, h_x, h_y);
+      surrogate MLP: 32 B (D and the flux of both staggered columns)."""
+    ks = np.clip(np.searchsorted(z, H, side="right") - 1, 0, len(z) - 2)
+    n = float(H.size)
+    e_bytes = 8.0 * float((ks + 1).sum())
+    out = {"step": e_bytes + 88.0 * n + (24.0 * Mz * n if full else 0.0), "SIA flux": e_bytes + 88.0 * n,
+           "surrogate features": e_bytes + 48.0 * n, "surrogate MLP": 32.0 * n}
     if full:
-        ksH = np.clip(np.searchsorted(z, H, side="right") - 1, 0, len(z) - 2)
-        out["SIA 3D hor. vel."] = 32.0 * Mz * n + i_bytes + 8.0 * float((ksH + 1).sum()) + 56.0 * n
-    else:
-        # surrogate: the feature pass reads E of the listed columns and writes
-        # 272 B per staggered column (34 inputs, 32 cell baselines, C S_t);
-        # the network reads those back and writes D and the flux
-        nstag = float((ks >= 0).sum())
-        out["surrogate features"] = e_bytes + 40.0 * nstag + 272.0 * nstag
-        out["surrogate MLP"] = (272.0 + 4.0 + 16.0) * nstag
+        out["SIA 3D hor. vel."] = 32.0 * Mz * n + e_bytes + 56.0 * n
     return out
 
 
@@ -400,9 +378,7 @@ def run_combined_leg(S, torch, sgs, steps, check_every, rng=None, min_mz=0):
                 m["sia"].use_surrogate(m["sg"])
     alg = 0.0
     for m in members:
-        _, thks = m["sia"].theta_thksmooth()
-        g = m["grid"]
-        alg += algorithmic_bytes(thks.reshape(g.Mx, g.My), m["z"], m["Mz"], full=False)["SIA flux"]
+        alg += algorithmic_bytes(m["f"]["H"].cpu().numpy(), m["z"], m["Mz"], full=False)["step"]
     ens = S.Ensemble([m["sia"] for m in members], [m["geom"] for m in members], [m["f"]["E"] for m in members])
     chosen, _ = ens.autotune(steps=3)
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -626,13 +602,21 @@ def main():
         del a, b
         log("copy calibration: %d bytes read and written" % (8 * nel))
 
-    # roofline of the dominant kernel stage; algorithmic bytes from the
-    # smoothed thickness of the initial state
-    plain.update(f["H"], f["b"], f["h"], f["mask"], f["E"], fast=True)
-    _, thks = plain.theta_thksmooth()
-    alg = algorithmic_bytes(thks.reshape(MX, MY), z, MZ, full=not fast_mode, H=f["H"].cpu().numpy().reshape(MX, MY))
-    prefer = ["surrogate features", "surrogate MLP"] if args.path == "surrogate" else list(alg)
+    # roofline of the dominant kernel stage and of the whole step; algorithmic
+    # bytes from the thickness of the initial state
+    H0 = f["H"].cpu().numpy()
+    alg = algorithmic_bytes(H0, z, MZ, full=not fast_mode)
+    prefer = ["surrogate features", "surrogate MLP"] if args.path == "surrogate" else \
+        [k for k in alg if k != "step"]
     roofline = roofline_of(stages, alg, prefer)
+    step_roofline = {}
+    for name, (ms_k, full_k) in {"value": (ms, not fast_mode),
+                                 **({"plain_fp64_fast": (results["fast"][0], False),
+                                     "plain_fp64_full": (results["full"][0], True)} if "full" in results and
+                                    "fast" in results else {})}.items():
+        b = algorithmic_bytes(H0, z, MZ, full=full_k)["step"]
+        gbs = b / (ms_k / 1000.0) / 1e9
+        step_roofline[name] = {"bytes": round(b), "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
     if args.path == "surrogate":
         mlp_ms = stages["surrogate MLP"][0] / stages["surrogate MLP"][1]
         ncol = out_extra["surrogate_error"]["columns"]["surrogate"]
@@ -675,6 +659,7 @@ def main():
                    "path": path, "Mx": MX, "My": MY, "Mz": MZ, "mode": "fast" if fast_mode else "full"},
         "stages_ms": {k: round(v[0] / v[1], 4) for k, v in stages.items() if v[1]},
         "roofline": roofline,
+        "step_roofline": step_roofline,
         "cpu_baseline": cpu,
     }
     out.update(out_extra)
