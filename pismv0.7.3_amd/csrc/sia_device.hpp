@@ -41,7 +41,8 @@ struct DevParams {
   double A_glen;            // ice_softness
   double hooke_A, hooke_Q, hooke_C, hooke_k, hooke_Tr;
   double sec_to_ka;
-  double inv_ci, inv_L;      // 1 / c_i, 1 / L (the surrogate's per-level pass)
+  double inv_ci, inv_L;      // 1 / c_i, 1 / L (the surrogate's per-level pass, flow_gpbld_fast)
+  double QR_cold, QR_warm;   // Q / R of the Paterson-Budd cold and warm branches
   // strain heating (PISMStressBalance.cc:384-388; flowlaws.cc:76, 249-253)
   double hard_pow;           // -1 / n
   double e_pow;              // e^(-1/n)
@@ -284,6 +285,80 @@ __device__ __forceinline__ double log_softness(const DevParams &P, double E, dou
   } else {  // LAW_ARRWARM
     return P.ln_A_warm - P.Q_warm / (P.R * ec_pressure_adjusted_temperature(P, E, p));
   }
+}
+
+// ---------------------------------------------------------------- fast FP64 math
+// 1 / t for t in the normal range: the hardware reciprocal estimate refined by
+// two Newton steps (each doubles the correct bits); within 2 ulp of 1 / t
+__device__ __forceinline__ double rcp_nr(double t) {
+  double y = __builtin_amdgcn_rcp(t);
+  double e = fma(-t, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-t, y, 1.0);
+  return fma(y, e, y);
+}
+
+// e^x for |x| <= 700 (the Arrhenius factors exp(-Q / (R T)), the strain heating):
+// x = n ln 2 + r with n = rint(x / ln 2) and |r| <= ln 2 / 2, the product
+// n ln 2 taken in two parts (a high part with trailing zero bits, exact for
+// |n| < 2^11, and the remainder); e^r by its Taylor series to degree 12,
+// whose truncation is below 2.5e-16 relative for |r| <= 0.347; scaled by 2^n.
+// Within a few ulp of e^x.
+__device__ __forceinline__ double exp_fast(double x) {
+  const double n = rint(x * 1.4426950408889634074);
+  double r = fma(-n, 6.93147180369123816490e-01, x);
+  r = fma(-n, 1.90821492927058770002e-10, r);
+  double q = 1.0 / 479001600.0;                 // 1/12!
+  q = fma(q, r, 1.0 / 39916800.0);              // 1/11!
+  q = fma(q, r, 1.0 / 3628800.0);
+  q = fma(q, r, 1.0 / 362880.0);
+  q = fma(q, r, 1.0 / 40320.0);
+  q = fma(q, r, 1.0 / 5040.0);
+  q = fma(q, r, 1.0 / 720.0);
+  q = fma(q, r, 1.0 / 120.0);
+  q = fma(q, r, 1.0 / 24.0);
+  q = fma(q, r, 1.0 / 6.0);
+  q = fma(q, r, 0.5);
+  q = fma(q, r, 1.0);
+  q = fma(q, r, 1.0);
+  return ldexp(q, (int)n);
+}
+
+// GPBLD's flow (flowlaws.cc:209-221 through FlowLaw::flow, 98-101) with the
+// default enthalpy converter (enthalpyConverter.cc:92-234) and an integer
+// n - 1 in {1, 2, 3}, branch-free: the cold and the temperate softness are
+// both evaluated and the one E selects is kept. Against flow<LAW_GPBLD> the
+// divisions by c_i and L become products with their reciprocals, -Q / (R T_pa)
+// becomes -(Q / R) (1 / T_pa) and exp is exp_fast: the softness agrees to a
+// few ulp (a relative 1e-14 at most through the exponent's size).
+__device__ __forceinline__ double flow_gpbld_fast(const DevParams &P, double stress, double E, double p) {
+  const double Tm = P.T_melt - P.beta * p;                 // melting_temperature_impl
+  const double E_s = P.ci * (Tm - P.T0);                   // enthalpy_cts_impl
+  const double T_pa = ((E * P.inv_ci) + P.T0) - Tm + P.T_melt;  // temperature_impl, pressure_adjusted_temperature
+  const bool cold_pb = T_pa < P.T_crit;                    // softness_parameter_paterson_budd
+  const double A_c = (cold_pb ? P.A_cold : P.A_warm) * exp_fast(-(cold_pb ? P.QR_cold : P.QR_warm) * rcp_nr(T_pa));
+  double omega = (E - E_s) * P.inv_L;                      // water_fraction_impl
+  omega = (P.wf_limit < omega) ? P.wf_limit : omega;
+  const double A_t = P.A_T0 * (1.0 + P.wf_coeff * omega);
+  const double A = (E < E_s) ? A_c : A_t;
+  const double s2 = stress * stress;
+  const double sp = (P.nm1_int == 1) ? stress : ((P.nm1_int == 2) ? s2 : s2 * stress);
+  return A * sp;
+}
+
+// log_softness<LAW_GPBLD> with the default enthalpy converter, the divisions
+// replaced as in flow_gpbld_fast (ln A agrees to 1e-14 absolute)
+__device__ __forceinline__ double log_softness_gpbld_fast(const DevParams &P, double E, double p) {
+  const double Tm = P.T_melt - P.beta * p;
+  const double E_s = P.ci * (Tm - P.T0);
+  if (E < E_s) {
+    const double T_pa = ((E * P.inv_ci) + P.T0) - Tm + P.T_melt;
+    const bool cold = T_pa < P.T_crit;
+    return (cold ? P.ln_A_cold : P.ln_A_warm) - (cold ? P.QR_cold : P.QR_warm) * rcp_nr(T_pa);
+  }
+  double omega = (E - E_s) * P.inv_L;
+  if (P.wf_limit < omega) omega = P.wf_limit;
+  return P.ln_A_T0 + log(1.0 + P.wf_coeff * omega);
 }
 
 // SIAFD::grainSizeVostok (SIAFD.cc:980-1018)

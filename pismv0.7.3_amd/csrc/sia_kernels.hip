@@ -411,17 +411,22 @@ __global__ void k_theta_thk(DevParams P, Fields2D F, const double *__restrict__ 
 // (SIAFD.cc:845-878). Each wavefront owns COLS_PER_WAVE consecutive
 // staggered columns s = 2*(i*My+j)+o. Lane c first prepares column c (2-D
 // inputs, kBelowHeight), so the 2-D loads of all its columns are in flight
-// together; the wave then walks the columns, one lane per level, with the
-// first 64 levels of E of the next column loaded one column ahead. D, flux
-// and ks leave as one coalesced store per wave. D_max is reduced per block
-// into dmax_part[block]; a one-block kernel takes the max of the partials.
+// together; the wave then walks the columns in chunks of 64 levels, one lane
+// per level, and the enthalpy of the next chunk (of this column or of the
+// next one) is loaded while the current chunk is evaluated. D, flux and ks
+// leave as one coalesced store per wave. D_max is reduced per block into
+// dmax_part[block]; a one-block kernel takes the max of the partials.
+// GPBLD with the default enthalpy converter and an integer Glen exponent
+// takes flow_gpbld_fast, every other configuration the reference-form flow<LAW>
+// (the choice depends on the parameters only, so a column gets the same
+// arithmetic from every kernel that computes it).
 // block bid of the column grid with COLS_PER_WAVE = CPW; zs: Mz doubles of
 // LDS, wmax: WAVES_PER_BLOCK doubles of LDS. With a list, the block's columns
 // are list entries bid * COLS_PER_BLOCK .. (n_list of them in all) instead of
 // staggered columns. Returns the block's D_max in thread 0.
-template <int LAW, bool FULL, int CPW>
-__device__ __forceinline__ double column_block(const DevParams &P, const ColumnArgs &A, int bid, double *zs,
-                                               double *wmax, const int *list = nullptr, int n_list = 0) {
+template <int LAW, bool FULL, int CPW, bool FASTGP>
+__device__ __forceinline__ double column_block_impl(const DevParams &P, const ColumnArgs &A, int bid, double *zs,
+                                                    double *wmax, const int *list, int n_list) {
   constexpr int COLS_PER_WAVE = CPW;
   constexpr int COLS_PER_BLOCK = WAVES_PER_BLOCK * COLS_PER_WAVE;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -473,87 +478,120 @@ __device__ __forceinline__ double column_block(const DevParams &P, const ColumnA
     }
   }
 
-  // first chunk of E (and age) of column c, levels 0..min(ks, 63)
-  double nEa = 0.0, nEb = 0.0, nAa = 0.0, nAb = 0.0;
-  auto prefetch = [&](int c) {
+  auto Icol_of = [&](int c) -> double * {
+    const int s_c = list ? lane_value(s, c) : s0 + c;
+    return ((s_c & 1) ? A.I1 : A.I0) + (long)(s_c >> 1) * Mz;
+  };
+  // first column >= c with a level to evaluate; the columns passed over get
+  // I = 0 at every level (readers use I[ks] with ks stored as 0)
+  auto next_valid = [&](int c) -> int {
+    for (; c < COLS_PER_WAVE; c++) {
+      const int kk = lane_value(c_ks, c);
+      if (kk == -3) return COLS_PER_WAVE;
+      if (kk >= 0) return c;
+      if (FULL && kk != -4 && lane == 0) Icol_of(c)[0] = 0.0;
+    }
+    return COLS_PER_WAVE;
+  };
+  const bool with_age = !FASTGP && P.use_age;
+  // enthalpy (and age) of the chunk (c, k0): level k0 + lane of both cells
+  auto load = [&](int c, int k0, double &ea, double &eb, double &ga, double &gb) {
+    if (c >= COLS_PER_WAVE) return;
     const int ks = lane_value(c_ks, c);
-    if (ks >= 0 && lane <= ks) {
+    const int k = k0 + lane;
+    if (k <= ks) {
       const long qa = lane_value(c_qa, c), qb = lane_value(c_qb, c);
-      nEa = A.E[qa * Mz + lane];
-      nEb = A.E[qb * Mz + lane];
-      if (P.use_age) {
-        nAa = A.age[qa * Mz + lane];
-        nAb = A.age[qb * Mz + lane];
+      ea = A.E[qa * Mz + k];
+      eb = A.E[qb * Mz + k];
+      if (with_age) {
+        ga = A.age[qa * Mz + k];
+        gb = A.age[qb * Mz + k];
       }
     }
   };
-  prefetch(0);
+  // the chunk after (c, k0): the rest of column c, else the next column's first
+  auto advance = [&](int &c, int &k0) {
+    if (c >= COLS_PER_WAVE) return;
+    k0 += 64;
+    if (k0 > lane_value(c_ks, c)) {
+      c = next_valid(c + 1);
+      k0 = 0;
+    }
+  };
 
   double myD = 0.0;  // lane c: D of column c before the edge override
-  for (int c = 0; c < COLS_PER_WAVE; c++) {
+  // three chunks in flight: the one evaluated and the next two, whose loads
+  // are issued two chunks ahead of their use
+  int c = next_valid(0), k0 = 0;
+  int c1 = c, k1 = 0;
+  advance(c1, k1);
+  int c2 = c1, k2 = k1;
+  advance(c2, k2);
+  double ea = 0.0, eb = 0.0, ga = 0.0, gb = 0.0;
+  double e1a = 0.0, e1b = 0.0, g1a = 0.0, g1b = 0.0;
+  load(c, 0, ea, eb, ga, gb);
+  load(c1, k1, e1a, e1b, g1a, g1b);
+  double Dpart = 0.0, dprev_carry = 0.0, Icarry = 0.0;
+  while (c < COLS_PER_WAVE) {
     const int ks = lane_value(c_ks, c);
-    if (ks == -3) break;
-    if (ks == -4) {
-      if (c + 1 < COLS_PER_WAVE) prefetch(c + 1);
-      continue;
-    }
-    const double Ea0 = nEa, Eb0 = nEb, Aa0 = nAa, Ab0 = nAb;
-    if (c + 1 < COLS_PER_WAVE) prefetch(c + 1);
-    const long qa = lane_value(c_qa, c), qb = lane_value(c_qb, c);
-    const int s_c = list ? lane_value(s, c) : s0 + c;
-    double *Icol = FULL ? ((s_c & 1) ? A.I1 : A.I0) + (long)(s_c >> 1) * Mz : nullptr;
-    if (ks < 0) {
-      // I = 0 at every level; above ks readers use I[ks] with ks stored as 0
-      if (FULL && lane == 0) Icol[0] = 0.0;
-      continue;
-    }
+    const int cn = c1;
+    double na = 0.0, nb = 0.0, nga = 0.0, ngb = 0.0;
+    load(c2, k2, na, nb, nga, ngb);
+
     const double thk = lane_value(c_thk, c), alpha = lane_value(c_alpha, c), c0 = lane_value(c_c0, c);
-    const double *__restrict__ Ea = A.E + qa * Mz;
-    const double *__restrict__ Eb = A.E + qb * Mz;
-    const double *__restrict__ aa = P.use_age ? A.age + qa * Mz : nullptr;
-    const double *__restrict__ ab = P.use_age ? A.age + qb * Mz : nullptr;
-    double Dpart = 0.0, dprev_carry = 0.0, Icarry = 0.0;
-    for (int k0 = 0; k0 <= ks; k0 += 64) {
-      const int k = k0 + lane;
-      const bool act = k <= ks;
-      double ea = Ea0, eb = Eb0, ga = Aa0, gb = Ab0;
-      if (k0 > 0 && act) {
-        ea = Ea[k];
-        eb = Eb[k];
-        if (P.use_age) {
-          ga = aa[k];
-          gb = ab[k];
-        }
+    const int k = k0 + lane;
+    const bool act = k <= ks;
+    double delta = 0.0, depth = 0.0, dz = 0.0;
+    if (act) {
+      depth = thk - zs[k];
+      const double pressure = ec_pressure(P, depth);
+      const double Ek = 0.5 * (ea + eb);
+      double fl;
+      if (FASTGP) {
+        fl = flow_gpbld_fast(P, alpha * pressure, Ek, pressure);
+      } else {
+        const double gs = with_age ? grain_size_vostok(P, 0.5 * (ga + gb)) : P.gs;
+        fl = flow<LAW>(P, alpha * pressure, Ek, pressure, gs);
       }
-      double delta = 0.0, depth = 0.0, dz = 0.0;
-      if (act) {
-        depth = thk - zs[k];
-        const double pressure = ec_pressure(P, depth);
-        const double gs = P.use_age ? grain_size_vostok(P, 0.5 * (ga + gb)) : P.gs;
-        const double Ek = 0.5 * (ea + eb);
-        const double fl = flow<LAW>(P, alpha * pressure, Ek, pressure, gs);
-        delta = c0 * pressure * fl;
-        if (k > 0) dz = zs[k] - zs[k - 1];
-      }
-      double dprev = wave_shift_up1(delta);
-      if (lane == 0) dprev = dprev_carry;
-      if (act && k > 0) {
-        Dpart += 0.5 * dz * ((depth + dz) * dprev + depth * delta);  // SIAFD.cc:677-680
-      }
-      if (k == ks) {
-        const double dzt = thk - zs[ks];  // SIAFD.cc:682-684
-        Dpart += 0.5 * dzt * dzt * delta;
-      }
-      if (FULL) {
-        const double t = (act && k > 0) ? 0.5 * dz * (dprev + delta) : 0.0;  // SIAFD.cc:863-867
-        const double Ik = Icarry + wave_inclusive_scan(t);
-        if (act) Icol[k] = Ik;
-        Icarry = lane_value(Ik, 63);
-      }
-      dprev_carry = lane_value(delta, 63);
+      delta = c0 * pressure * fl;
+      if (k > 0) dz = zs[k] - zs[k - 1];
     }
-    const double D = lane_value(wave_inclusive_scan(Dpart), 63);
-    if (lane == c) myD = D;
+    double dprev = wave_shift_up1(delta);
+    if (lane == 0) dprev = dprev_carry;
+    if (act && k > 0) {
+      Dpart += 0.5 * dz * ((depth + dz) * dprev + depth * delta);  // SIAFD.cc:677-680
+    }
+    if (k == ks) {
+      const double dzt = thk - zs[ks];  // SIAFD.cc:682-684
+      Dpart += 0.5 * dzt * dzt * delta;
+    }
+    if (FULL) {
+      const double t = (act && k > 0) ? 0.5 * dz * (dprev + delta) : 0.0;  // SIAFD.cc:863-867
+      const double Ik = Icarry + wave_inclusive_scan(t);
+      if (act) Icol_of(c)[k] = Ik;
+      Icarry = lane_value(Ik, 63);
+    }
+    dprev_carry = lane_value(delta, 63);
+    if (cn != c) {
+      const double D = lane_value(wave_inclusive_scan(Dpart), 63);
+      if (lane == c) myD = D;
+      Dpart = 0.0;
+      dprev_carry = 0.0;
+      Icarry = 0.0;
+    }
+    c = c1;
+    k0 = k1;
+    c1 = c2;
+    k1 = k2;
+    advance(c2, k2);
+    ea = e1a;
+    eb = e1b;
+    ga = g1a;
+    gb = g1b;
+    e1a = na;
+    e1b = nb;
+    g1a = nga;
+    g1b = ngb;
   }
 
   double dmax = 0.0;
@@ -573,6 +611,20 @@ __device__ __forceinline__ double column_block(const DevParams &P, const ColumnA
     for (int w = 1; w < WAVES_PER_BLOCK; w++) m = (m < wmax[w]) ? wmax[w] : m;
   }
   return m;
+}
+
+// flow_gpbld_fast applies (see column_block_impl)
+__device__ __forceinline__ bool use_gpbld_fast(const DevParams &P) {
+  return P.ec == EC_DEFAULT && P.nm1_int > 0 && !P.use_age;
+}
+
+template <int LAW, bool FULL, int CPW>
+__device__ __forceinline__ double column_block(const DevParams &P, const ColumnArgs &A, int bid, double *zs,
+                                               double *wmax, const int *list = nullptr, int n_list = 0) {
+  if (LAW == LAW_GPBLD && use_gpbld_fast(P)) {
+    return column_block_impl<LAW, FULL, CPW, true>(P, A, bid, zs, wmax, list, n_list);
+  }
+  return column_block_impl<LAW, FULL, CPW, false>(P, A, bid, zs, wmax, list, n_list);
 }
 
 constexpr int COLS_PER_WAVE = 16;
@@ -879,6 +931,264 @@ __global__ __launch_bounds__(256) void k_velocity_uvw(DevParams P, VelArgs A, St
     if (act) __builtin_nontemporal_store(w0 - S, wo + k);
     carry_sum = lane_value(S, 63);
     carry_int = lane_value(integrand, 63);
+  }
+}
+
+// ---------------------------------------------------------------- u, v, w by tiles
+// The same outputs as k_velocity_uvw, computed by tiles of TI x TJ map columns:
+// per chunk of KC levels, u and v of the tile and of a ring of one column
+// around it go to LDS once (each from the four I columns of its faces, with
+// the reference's formula, so a ring value is bitwise the one its own tile
+// stores); the w integrand, w (a prefix scan over the wavefront's lanes, one
+// level each) and the strain heating read them there.
+// A chunk holds the levels k0-1 .. k0+KC of u and v (u_z at k0 and k0+KC-1);
+// its last two slots become the next chunk's first two. Above the tile's
+// largest kBelowHeight (of its faces and of H) u, v and the integrand are
+// constant and the strain heating is 0, so those levels are written directly,
+// w by the trapezoid of a constant integrand (S + int (z_k - z_top)). Each
+// column's KC levels leave as runs of 512 bytes (shorter runs of this layout,
+// whose columns are not aligned to cache lines, write HBM at a fraction of
+// the rate).
+// workgroup barrier that orders LDS only: the global stores in flight are
+// not waited for (__syncthreads would wait for them at every chunk)
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+namespace vtile {
+constexpr int TI = 4, TJ = 4, KC = 64;
+constexpr int RJ = TJ + 2, NR = (TI + 2) * RJ;                       // u, v columns: tile + ring
+constexpr int FJ0 = TJ + 2, NF0 = (TI + 3) * FJ0;                    // I0 faces (a, b): a in [-2, TI], b in [-1, TJ]
+constexpr int FJ1 = TJ + 3, NF1 = (TI + 2) * FJ1;                    // I1 faces (a, b): a in [-1, TI], b in [-2, TJ]
+constexpr int NF = NF0 + NF1, NC = TI * TJ, NS = KC + 2;
+__device__ __forceinline__ int f0(int a, int b) { return (a + 2) * FJ0 + (b + 1); }
+__device__ __forceinline__ int f1(int a, int b) { return NF0 + (a + 1) * FJ1 + (b + 2); }
+__device__ __forceinline__ int rr(int a, int b) { return (a + 1) * RJ + (b + 1); }
+__device__ __forceinline__ int wrapm(int i, int M) { const int r = i % M; return r < 0 ? r + M : r; }
+}  // namespace vtile
+
+struct TileMap {
+  int ti, tj, per_xcd;  // tiles along i and j; blocks reserved per XCD
+};
+
+__host__ __device__ inline TileMap tile_map(int Mx, int My) {
+  TileMap m;
+  m.ti = (Mx + vtile::TI - 1) / vtile::TI;
+  m.tj = (My + vtile::TJ - 1) / vtile::TJ;
+  m.per_xcd = ((m.tj + N_XCD - 1) / N_XCD) * m.ti;
+  return m;
+}
+
+template <int LAW, bool FASTLN>
+__device__ __forceinline__ double log_softness_sel(const DevParams &P, double E, double p) {
+  if (FASTLN) return log_softness_gpbld_fast(P, E, p);
+  return log_softness<LAW>(P, E, p);
+}
+
+template <int LAW, bool FASTLN>
+__global__ __launch_bounds__(256) void k_velocity_tile(DevParams P, VelArgs A, TileMap tm) {
+  using namespace vtile;
+  constexpr bool HEAT = LAW >= 0;
+  extern __shared__ double zs[];  // z [Mz], then (HEAT) 1 / (z_{k+1} - z_{k-1}) [Mz] (k = 0: 1 / (z_1 - z_0))
+  __shared__ const double *fcol[NF];
+  __shared__ int fks[NF];
+  __shared__ double fhx[NF], fhy[NF];
+  __shared__ double rub[NR], rvb[NR];
+  __shared__ double sU[NR][NS], sV[NR][NS];
+  __shared__ double cDx[NC], cDy[NC], cwe[NC][4], cw0[NC], cH[NC], cS[NC], cI[NC];
+  __shared__ int cks[NC], cq[NC];
+  __shared__ int s_kend;
+  const int tid = threadIdx.x;
+  const int Mx = P.Mx, My = P.My, Mz = P.Mz;
+  double *idz = zs + Mz;
+  // tile of this block: XCD x walks the tile columns x, x + 8, ... down the tile rows
+  const int x = blockIdx.x % N_XCD, L = blockIdx.x / N_XCD;
+  const int ss = L / tm.ti, tj = x + N_XCD * ss, ti = L - ss * tm.ti;
+  if (tj >= tm.tj) return;
+  const int i0 = ti * TI, j0 = tj * TJ;
+  for (int k = tid; k < Mz; k += blockDim.x) {
+    zs[k] = A.z[k];
+    if (HEAT) idz[k] = (k == 0) ? 1.0 / (A.z[1] - A.z[0]) : ((k + 1 < Mz) ? 1.0 / (A.z[k + 1] - A.z[k - 1]) : 0.0);
+  }
+  if (tid == 0) s_kend = 0;
+  __syncthreads();
+  int kend = 0;
+  // faces: I column, kBelowHeight, h_x, h_y
+  for (int f = tid; f < NF; f += blockDim.x) {
+    int a, b, o;
+    if (f < NF0) { a = f / FJ0 - 2; b = f % FJ0 - 1; o = 0; }
+    else { a = (f - NF0) / FJ1 - 1; b = (f - NF0) % FJ1 - 2; o = 1; }
+    const int gi = wrapm(i0 + a, Mx), gj = wrapm(j0 + b, My);
+    const int sid = 2 * (gi * My + gj) + o;
+    fcol[f] = (o ? A.I1 : A.I0) + (long)(gi * My + gj) * Mz;
+    const int k = A.ks[sid];
+    fks[f] = k;
+    fhx[f] = A.hx[sid];
+    fhy[f] = A.hy[sid];
+    kend = max(kend, k);
+  }
+  // ring columns: basal velocity
+  for (int r = tid; r < NR; r += blockDim.x) {
+    const int a = r / RJ - 1, b = r % RJ - 1;
+    const long q = (long)wrapm(i0 + a, Mx) * My + wrapm(j0 + b, My);
+    rub[r] = A.ub ? A.ub[q] : 0.0;
+    rvb[r] = A.vb ? A.vb[q] : 0.0;
+  }
+  // tile columns: margin-aware differences (PISMStressBalance.cc:223-260), w_0, H and its kBelowHeight
+  if (tid < NC) {
+    const int a = tid / TJ, b = tid % TJ;
+    const int gi = i0 + a, gj = j0 + b;
+    int ksH = -1;
+    if (gi < Mx && gj < My) {
+      const int q = gi * My + gj;
+      cq[tid] = q;
+      const int im = wrapm(gi - 1, Mx), ip = wrapm(gi + 1, Mx), jm = wrapm(gj - 1, My), jp = wrapm(gj + 1, My);
+      const int m = as_int(A.mask[q]);
+      const int me = as_int(A.mask[ip * My + gj]), mw = as_int(A.mask[im * My + gj]);
+      const int mn = as_int(A.mask[gi * My + jp]), ms = as_int(A.mask[gi * My + jm]);
+      double west = 1, east = 1, south = 1, north = 1;
+      if ((m_icy(m) && m_ice_free(me)) || (m_ice_free(m) && m_icy(me))) east = 0;
+      if ((m_icy(m) && m_ice_free(mw)) || (m_ice_free(m) && m_icy(mw))) west = 0;
+      if ((m_icy(m) && m_ice_free(mn)) || (m_ice_free(m) && m_icy(mn))) north = 0;
+      if ((m_icy(m) && m_ice_free(ms)) || (m_ice_free(m) && m_icy(ms))) south = 0;
+      cDx[tid] = (east + west > 0) ? 1.0 / (P.dx * (east + west)) : 0.0;
+      cDy[tid] = (north + south > 0) ? 1.0 / (P.dy * (north + south)) : 0.0;
+      cwe[tid][0] = west;
+      cwe[tid][1] = east;
+      cwe[tid][2] = south;
+      cwe[tid][3] = north;
+      cw0[tid] = A.bmr ? -A.bmr[q] : 0.0;
+      if (HEAT) {
+        const double Hq = A.H[q];
+        cH[tid] = Hq;
+        ksH = kbelow(zs, Mz, P.Lz, Hq);  // throws above Lz (IceGrid.cc:381-389)
+        if (ksH < 0) atomicOr(A.err, ERR_THK_LZ);
+        kend = max(kend, ksH);
+      }
+    } else {
+      cq[tid] = -1;
+    }
+    cks[tid] = ksH;
+    cS[tid] = 0.0;
+    cI[tid] = 0.0;
+  }
+  kend = max(kend, __shfl_xor(kend, 1));  // cheap partial reduction before the LDS atomic
+  atomicMax(&s_kend, kend);
+  __syncthreads();
+  kend = min(s_kend, Mz - 1);
+
+  int k0 = 0;
+  for (; k0 <= kend; k0 += KC) {
+    // enthalpy of this chunk's levels of the wavefront's tile columns (phase
+    // C), loaded now so that phase B hides its latency
+    double Ec[NC / WAVES_PER_BLOCK];
+    {
+      const int lane = tid & 63;
+#pragma unroll
+      for (int t = 0; t < NC / WAVES_PER_BLOCK; t++) {
+        const int c = (tid >> 6) + t * WAVES_PER_BLOCK;
+        const int k = k0 + lane;
+        Ec[t] = (HEAT && cq[c] >= 0 && k <= cks[c]) ? glb(A.E)[(long)cq[c] * Mz + k] : 0.0;
+      }
+    }
+    // phase B: u, v at levels k0 .. k0+KC-1 (slots 1 .. KC), one wavefront
+    // per ring column, one lane per level; then level k0+KC (slot KC+1) of
+    // every ring column. Slot 0 (level k0-1) came from the last chunk.
+    {
+      const int lane = tid & 63;
+      auto uv_at = [&](int r, int sl) {
+        const int kk = min(k0 - 1 + sl, Mz - 1);
+        const int a = r / RJ - 1, b = r - (a + 1) * RJ - 1;
+        const int fe = f0(a, b), fw = f0(a - 1, b), fn = f1(a, b), fs = f1(a, b - 1);
+        const double Ie = glb(fcol[fe])[min(kk, fks[fe])], Iw = glb(fcol[fw])[min(kk, fks[fw])];
+        const double In = glb(fcol[fn])[min(kk, fks[fn])], Is = glb(fcol[fs])[min(kk, fks[fs])];
+        sU[r][sl] = uv_formula(Ie, fhx[fe], Iw, fhx[fw], In, fhx[fn], Is, fhx[fs], rub[r]);
+        sV[r][sl] = uv_formula(Ie, fhy[fe], Iw, fhy[fw], In, fhy[fn], Is, fhy[fs], rvb[r]);
+      };
+#pragma unroll 3
+      for (int r = tid >> 6; r < NR; r += WAVES_PER_BLOCK) uv_at(r, lane + 1);
+      if (tid < NR) uv_at(tid, KC + 1);
+    }
+    lds_barrier();
+    // phase C: integrand, w and the strain heating of levels k0 .. k0+KC-1;
+    // one wavefront per tile column, one lane per level
+#pragma unroll
+    for (int tc = 0; tc < NC / WAVES_PER_BLOCK; tc++) {
+      const int c = (tid >> 6) + tc * WAVES_PER_BLOCK;
+      const int lane = tid & 63;
+      const int a = c / TJ, b = c - a * TJ;
+      const int r = rr(a, b);
+      const int q = cq[c];
+      const int k = k0 + lane, sl = lane + 1;
+      const bool act = q >= 0 && k < Mz;
+      const double uc = sU[r][sl], vc = sV[r][sl];
+      const double uw = sU[r - RJ][sl], ue = sU[r + RJ][sl], vs = sV[r - 1][sl], vn = sV[r + 1][sl];
+      const double Dx = cDx[c], Dy = cDy[c];
+      const double west = cwe[c][0], east = cwe[c][1], south = cwe[c][2], north = cwe[c][3];
+      const double u_x = Dx * (west * (uc - uw) + east * (ue - uc));
+      const double v_y = Dy * (south * (vc - vs) + north * (vn - vc));
+      double integrand = act ? u_x + v_y : 0.0;
+      double iprev = wave_shift_up1(integrand);
+      if (lane == 0) iprev = cI[c];
+      double dz = 0.0;
+      if (act && k > 0) dz = zs[k] - zs[k - 1];
+      const double t = (act && k > 0) ? 0.5 * (integrand + iprev) * dz : 0.0;  // PISMStressBalance.cc:280-282
+      const double S = cS[c] + wave_inclusive_scan(t);
+      double sig = 0.0;
+      if (HEAT && act && k <= cks[c]) {
+        const double us = sU[r - 1][sl], un = sU[r + 1][sl], vw = sV[r - RJ][sl], ve = sV[r + RJ][sl];
+        const double pressure = ec_pressure(P, cH[c] - zs[k]);
+        const double lnA = log_softness_sel<LAW, FASTLN>(P, Ec[tc], pressure);
+        const double hu_x = u_x, hu_y = Dy * (south * (uc - us) + north * (un - uc)),
+                     hv_x = Dx * (west * (vc - vw) + east * (ve - vc)), hv_y = v_y;
+        // one-sided at the base (PISMStressBalance.cc:464-476)
+        const double u_z = (k > 0) ? (sU[r][sl + 1] - sU[r][sl - 1]) * idz[k] : (sU[r][sl + 1] - uc) * idz[0];
+        const double v_z = (k > 0) ? (sV[r][sl + 1] - sV[r][sl - 1]) * idz[k] : (sV[r][sl + 1] - vc) * idz[0];
+        const double d2 = strain_D2(hu_x, hu_y, u_z, hv_x, hv_y, v_z);
+        sig = (d2 > 0.0) ? 2.0 * P.e_pow * exp_fast(P.hard_pow * lnA + P.heat_exp * log(d2)) : 0.0;
+      }
+      if (act) {
+        const long o = (long)q * Mz + k;
+        A.u[o] = uc;
+        A.v[o] = vc;
+        A.w[o] = cw0[c] - S;
+        if (HEAT) A.Sigma[o] = sig;
+      }
+      // carries of the wavefront's last lane (the last level of the chunk)
+      if (lane == 63) {
+        cS[c] = S;
+        cI[c] = integrand;
+      }
+    }
+    lds_barrier();
+    // slot KC (level k0+KC-1) becomes the next chunk's slot 0
+    if (tid < NR) {
+      sU[tid][0] = sU[tid][KC];
+      sV[tid][0] = sV[tid][KC];
+    }
+    lds_barrier();
+  }
+  // levels k0 .. Mz-1: constant u, v, integrand, Sigma = 0; w continues the trapezoid
+  if (k0 < Mz) {
+    const double ztop = zs[k0 - 1];
+#pragma unroll 1
+    for (int c = 0; c < NC; c++) {
+      const int q = cq[c];
+      if (q < 0) continue;
+      const int a = c / TJ, b = c - a * TJ;
+      const int r = rr(a, b);
+      const double ut = sU[r][0], vt = sV[r][0];  // level k0 - 1 (moved to slot 0)
+      const double it_ = cI[c], St = cS[c], w0 = cw0[c];
+      const long base = (long)q * Mz;
+      for (int k = k0 + tid; k < Mz; k += blockDim.x) {
+        A.u[base + k] = ut;
+        A.v[base + k] = vt;
+        A.w[base + k] = w0 - (St + it_ * (zs[k] - ztop));
+        if (HEAT) A.Sigma[base + k] = 0.0;
+      }
+    }
   }
 }
 
@@ -1346,9 +1656,13 @@ void launch_theta_thk(const DevParams &P, const double *H, const double *b, cons
 
 template <int LAW>
 static void launch_vel_law(const DevParams &P, const VelArgs &A, hipStream_t st) {
-  const StripMap sm = strip_map(P.Mx, P.My, WAVES_PER_BLOCK);
-  hipLaunchKernelGGL((k_velocity_uvw<LAW>), dim3(sm.per_xcd * N_XCD), dim3(256),
-                     (LAW >= 0 ? 2 : 1) * sizeof(double) * (size_t)P.Mz, st, P, A, sm);
+  const TileMap tm = tile_map(P.Mx, P.My);
+  const size_t lds = (LAW >= 0 ? 2 : 1) * sizeof(double) * (size_t)P.Mz;
+  if (LAW == LAW_GPBLD && P.ec == EC_DEFAULT) {
+    hipLaunchKernelGGL((k_velocity_tile<LAW, true>), dim3(tm.per_xcd * N_XCD), dim3(256), lds, st, P, A, tm);
+  } else {
+    hipLaunchKernelGGL((k_velocity_tile<LAW, false>), dim3(tm.per_xcd * N_XCD), dim3(256), lds, st, P, A, tm);
+  }
 }
 
 void launch_velocity_uvw(const DevParams &P, const VelArgs &A, hipStream_t st) {
