@@ -159,6 +159,10 @@ void make_dev_params(const sia_grid &g, const sia_params &p, DevParams &d) {
   d.inv_L = 1.0 / p.water_latent_heat_fusion;
   d.QR_cold = p.Paterson_Budd_Q_cold / p.ideal_gas_constant;
   d.QR_warm = p.Paterson_Budd_Q_warm / p.ideal_gas_constant;
+  {
+    const ExpCoef ec = exp_coef();
+    for (int i = 0; i < 11; i++) d.exp_c[i] = ec.c[i];
+  }
   d.alpha_float = 1 - p.ice_density / p.sea_water_density;
   d.sea_level = p.sea_level;
   d.icefree_thk = p.mask_icefree_thickness_standard;

@@ -43,6 +43,7 @@ struct DevParams {
   double sec_to_ka;
   double inv_ci, inv_L;      // 1 / c_i, 1 / L (the surrogate's per-level pass, flow_gpbld_fast)
   double QR_cold, QR_warm;   // Q / R of the Paterson-Budd cold and warm branches
+  double exp_c[11];          // 1/12!, ..., 1/2! for exp_fast (ExpCoef)
   // strain heating (PISMStressBalance.cc:384-388; flowlaws.cc:76, 249-253)
   double hard_pow;           // -1 / n
   double e_pow;              // e^(-1/n)
@@ -304,24 +305,34 @@ __device__ __forceinline__ double rcp_nr(double t) {
 // |n| < 2^11, and the remainder); e^r by its Taylor series to degree 12,
 // whose truncation is below 2.5e-16 relative for |r| <= 0.347; scaled by 2^n.
 // Within a few ulp of e^x.
-__device__ __forceinline__ double exp_fast(double x) {
+// The coefficients 1/k! are passed in wave-uniform (scalar) registers so that
+// each Horner step is one v_fma_f64 with a scalar operand (as literals they
+// would be copied into the accumulator before every step).
+struct ExpCoef {
+  double c[11];  // 1/12!, 1/11!, ..., 1/2!
+};
+
+__device__ __forceinline__ double exp_fast(double x, const ExpCoef &C) {
   const double n = rint(x * 1.4426950408889634074);
   double r = fma(-n, 6.93147180369123816490e-01, x);
   r = fma(-n, 1.90821492927058770002e-10, r);
-  double q = 1.0 / 479001600.0;                 // 1/12!
-  q = fma(q, r, 1.0 / 39916800.0);              // 1/11!
-  q = fma(q, r, 1.0 / 3628800.0);
-  q = fma(q, r, 1.0 / 362880.0);
-  q = fma(q, r, 1.0 / 40320.0);
-  q = fma(q, r, 1.0 / 5040.0);
-  q = fma(q, r, 1.0 / 720.0);
-  q = fma(q, r, 1.0 / 120.0);
-  q = fma(q, r, 1.0 / 24.0);
-  q = fma(q, r, 1.0 / 6.0);
-  q = fma(q, r, 0.5);
+  double q = C.c[0];
+#pragma unroll
+  for (int i = 1; i < 11; i++) q = fma(q, r, C.c[i]);
   q = fma(q, r, 1.0);
   q = fma(q, r, 1.0);
   return ldexp(q, (int)n);
+}
+
+__host__ __device__ inline ExpCoef exp_coef() {
+  ExpCoef C;
+  double f = 1.0;
+  for (int k = 2; k <= 12; k++) f *= (double)k;  // 12!
+  for (int i = 0; i < 11; i++) {
+    C.c[i] = 1.0 / f;   // 1/(12-i)!
+    f /= (double)(12 - i);
+  }
+  return C;
 }
 
 // GPBLD's flow (flowlaws.cc:209-221 through FlowLaw::flow, 98-101) with the
@@ -336,7 +347,8 @@ __device__ __forceinline__ double flow_gpbld_fast(const DevParams &P, double str
   const double E_s = P.ci * (Tm - P.T0);                   // enthalpy_cts_impl
   const double T_pa = ((E * P.inv_ci) + P.T0) - Tm + P.T_melt;  // temperature_impl, pressure_adjusted_temperature
   const bool cold_pb = T_pa < P.T_crit;                    // softness_parameter_paterson_budd
-  const double A_c = (cold_pb ? P.A_cold : P.A_warm) * exp_fast(-(cold_pb ? P.QR_cold : P.QR_warm) * rcp_nr(T_pa));
+  const double A_c = (cold_pb ? P.A_cold : P.A_warm) *
+                     exp_fast(-(cold_pb ? P.QR_cold : P.QR_warm) * rcp_nr(T_pa), *(const ExpCoef *)P.exp_c);
   double omega = (E - E_s) * P.inv_L;                      // water_fraction_impl
   omega = (P.wf_limit < omega) ? P.wf_limit : omega;
   const double A_t = P.A_T0 * (1.0 + P.wf_coeff * omega);

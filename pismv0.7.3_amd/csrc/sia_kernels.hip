@@ -1147,7 +1147,7 @@ __global__ __launch_bounds__(256) void k_velocity_tile(DevParams P, VelArgs A, T
         const double u_z = (k > 0) ? (sU[r][sl + 1] - sU[r][sl - 1]) * idz[k] : (sU[r][sl + 1] - uc) * idz[0];
         const double v_z = (k > 0) ? (sV[r][sl + 1] - sV[r][sl - 1]) * idz[k] : (sV[r][sl + 1] - vc) * idz[0];
         const double d2 = strain_D2(hu_x, hu_y, u_z, hv_x, hv_y, v_z);
-        sig = (d2 > 0.0) ? 2.0 * P.e_pow * exp_fast(P.hard_pow * lnA + P.heat_exp * log(d2)) : 0.0;
+        sig = (d2 > 0.0) ? 2.0 * P.e_pow * exp_fast(P.hard_pow * lnA + P.heat_exp * log(d2), *(const ExpCoef *)P.exp_c) : 0.0;
       }
       if (act) {
         const long o = (long)q * Mz + k;
