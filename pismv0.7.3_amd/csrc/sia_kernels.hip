@@ -494,21 +494,6 @@ __device__ __forceinline__ double column_block_impl(const DevParams &P, const Co
     return COLS_PER_WAVE;
   };
   const bool with_age = !FASTGP && P.use_age;
-  // enthalpy (and age) of the chunk (c, k0): level k0 + lane of both cells
-  auto load = [&](int c, int k0, double &ea, double &eb, double &ga, double &gb) {
-    if (c >= COLS_PER_WAVE) return;
-    const int ks = lane_value(c_ks, c);
-    const int k = k0 + lane;
-    if (k <= ks) {
-      const long qa = lane_value(c_qa, c), qb = lane_value(c_qb, c);
-      ea = A.E[qa * Mz + k];
-      eb = A.E[qb * Mz + k];
-      if (with_age) {
-        ga = A.age[qa * Mz + k];
-        gb = A.age[qb * Mz + k];
-      }
-    }
-  };
   // the chunk after (c, k0): the rest of column c, else the next column's first
   auto advance = [&](int &c, int &k0) {
     if (c >= COLS_PER_WAVE) return;
@@ -516,6 +501,26 @@ __device__ __forceinline__ double column_block_impl(const DevParams &P, const Co
     if (k0 > lane_value(c_ks, c)) {
       c = next_valid(c + 1);
       k0 = 0;
+    }
+  };
+  // enthalpy (and age) of the chunk (c, k0): level k0 + lane of both cells;
+  // lanes above ks load level ks (a valid address, the value is not used)
+  int lc = -1, l_ks = 0;
+  long l_qa = 0, l_qb = 0;
+  auto load = [&](int c, int k0, double &ea, double &eb, double &ga, double &gb) {
+    if (c >= COLS_PER_WAVE) return;
+    if (c != lc) {  // wave-uniform: the column's values are read once
+      lc = c;
+      l_ks = lane_value(c_ks, c);
+      l_qa = (long)lane_value(c_qa, c) * Mz;
+      l_qb = (long)lane_value(c_qb, c) * Mz;
+    }
+    const int k = min(k0 + lane, l_ks);
+    ea = A.E[l_qa + k];
+    eb = A.E[l_qb + k];
+    if (with_age) {
+      ga = A.age[l_qa + k];
+      gb = A.age[l_qb + k];
     }
   };
 
@@ -532,39 +537,44 @@ __device__ __forceinline__ double column_block_impl(const DevParams &P, const Co
   load(c, 0, ea, eb, ga, gb);
   load(c1, k1, e1a, e1b, g1a, g1b);
   double Dpart = 0.0, dprev_carry = 0.0, Icarry = 0.0;
+  // values of the column being evaluated, read once per column
+  int vc = -1, ks = 0;
+  double thk = 0.0, alpha = 0.0, c0 = 0.0, top = 0.0;
   while (c < COLS_PER_WAVE) {
-    const int ks = lane_value(c_ks, c);
-    const int cn = c1;
+    if (c != vc) {
+      vc = c;
+      ks = lane_value(c_ks, c);
+      thk = lane_value(c_thk, c);
+      alpha = lane_value(c_alpha, c);
+      c0 = lane_value(c_c0, c);
+      const double dzt = thk - zs[ks];  // SIAFD.cc:682-684
+      top = 0.5 * dzt * dzt;
+    }
     double na = 0.0, nb = 0.0, nga = 0.0, ngb = 0.0;
     load(c2, k2, na, nb, nga, ngb);
 
-    const double thk = lane_value(c_thk, c), alpha = lane_value(c_alpha, c), c0 = lane_value(c_c0, c);
     const int k = k0 + lane;
     const bool act = k <= ks;
-    double delta = 0.0, depth = 0.0, dz = 0.0;
-    if (act) {
-      depth = thk - zs[k];
-      const double pressure = ec_pressure(P, depth);
-      const double Ek = 0.5 * (ea + eb);
-      double fl;
-      if (FASTGP) {
-        fl = flow_gpbld_fast(P, alpha * pressure, Ek, pressure);
-      } else {
-        const double gs = with_age ? grain_size_vostok(P, 0.5 * (ga + gb)) : P.gs;
-        fl = flow<LAW>(P, alpha * pressure, Ek, pressure, gs);
-      }
-      delta = c0 * pressure * fl;
-      if (k > 0) dz = zs[k] - zs[k - 1];
+    const int kz = min(k, Mz - 1);
+    // depth >= 0 on the active lanes, where this is EnthalpyConverter::pressure
+    // (enthalpyConverter.cc:92-98: p_air + rho g 0 = p_air at depth 0)
+    const double depth = thk - zs[kz];
+    const double pressure = P.p_air + P.rho_i * P.g * depth;
+    const double Ek = 0.5 * (ea + eb);
+    double fl;
+    if (FASTGP) {
+      fl = flow_gpbld_fast(P, alpha * pressure, Ek, pressure);
+    } else {
+      const double gs = with_age ? grain_size_vostok(P, 0.5 * (ga + gb)) : P.gs;
+      fl = flow<LAW>(P, alpha * pressure, Ek, pressure, gs);
     }
+    const double delta = act ? c0 * pressure * fl : 0.0;
+    const double dz = (k > 0) ? zs[kz] - zs[kz - 1] : 0.0;
     double dprev = wave_shift_up1(delta);
     if (lane == 0) dprev = dprev_carry;
-    if (act && k > 0) {
-      Dpart += 0.5 * dz * ((depth + dz) * dprev + depth * delta);  // SIAFD.cc:677-680
-    }
-    if (k == ks) {
-      const double dzt = thk - zs[ks];  // SIAFD.cc:682-684
-      Dpart += 0.5 * dzt * dzt * delta;
-    }
+    const double trap = 0.5 * dz * ((depth + dz) * dprev + depth * delta);  // SIAFD.cc:677-680
+    if (act && k > 0) Dpart += trap;
+    if (k == ks) Dpart += top * delta;
     if (FULL) {
       const double t = (act && k > 0) ? 0.5 * dz * (dprev + delta) : 0.0;  // SIAFD.cc:863-867
       const double Ik = Icarry + wave_inclusive_scan(t);
@@ -572,7 +582,7 @@ __device__ __forceinline__ double column_block_impl(const DevParams &P, const Co
       Icarry = lane_value(Ik, 63);
     }
     dprev_carry = lane_value(delta, 63);
-    if (cn != c) {
+    if (c1 != c) {
       const double D = lane_value(wave_inclusive_scan(Dpart), 63);
       if (lane == c) myD = D;
       Dpart = 0.0;
