@@ -336,25 +336,25 @@ __host__ __device__ inline ExpCoef exp_coef() {
 }
 
 // GPBLD's flow (flowlaws.cc:209-221 through FlowLaw::flow, 98-101) with the
-// default enthalpy converter (enthalpyConverter.cc:92-234) and an integer
-// n - 1 in {1, 2, 3}, branch-free: the cold and the temperate softness are
-// both evaluated and the one E selects is kept. Against flow<LAW_GPBLD> the
-// divisions by c_i and L become products with their reciprocals, -Q / (R T_pa)
-// becomes -(Q / R) (1 / T_pa) and exp is exp_fast: the softness agrees to a
-// few ulp (a relative 1e-14 at most through the exponent's size).
+// default enthalpy converter (enthalpyConverter.cc:92-234) and n - 1 = NM1 in
+// {1, 2, 3}, branch-free: the cold and the temperate softness are both
+// evaluated and the one E selects is kept. Against flow<LAW_GPBLD> the
+// divisions by c_i and L become products with their reciprocals, and the cold
+// softness A exp(-Q / (R T_pa)) becomes exp(ln A - (Q / R) (1 / T_pa)) by
+// exp_fast: the softness agrees to a relative 1e-14 at most (the rounding of
+// an exponent of size 60).
+template <int NM1>
 __device__ __forceinline__ double flow_gpbld_fast(const DevParams &P, double stress, double E, double p) {
   const double Tm = P.T_melt - P.beta * p;                 // melting_temperature_impl
   const double E_s = P.ci * (Tm - P.T0);                   // enthalpy_cts_impl
   const double T_pa = ((E * P.inv_ci) + P.T0) - Tm + P.T_melt;  // temperature_impl, pressure_adjusted_temperature
-  const bool cold_pb = T_pa < P.T_crit;                    // softness_parameter_paterson_budd
-  const double A_c = (cold_pb ? P.A_cold : P.A_warm) *
-                     exp_fast(-(cold_pb ? P.QR_cold : P.QR_warm) * rcp_nr(T_pa), *(const ExpCoef *)P.exp_c);
-  double omega = (E - E_s) * P.inv_L;                      // water_fraction_impl
-  omega = (P.wf_limit < omega) ? P.wf_limit : omega;
+  const double y = rcp_nr(T_pa);
+  const double x_cold = P.ln_A_cold - P.QR_cold * y, x_warm = P.ln_A_warm - P.QR_warm * y;
+  const double A_c = exp_fast((T_pa < P.T_crit) ? x_cold : x_warm, *(const ExpCoef *)P.exp_c);  // softness_parameter_paterson_budd
+  const double omega = fmin((E - E_s) * P.inv_L, P.wf_limit);  // water_fraction_impl, capped
   const double A_t = P.A_T0 * (1.0 + P.wf_coeff * omega);
   const double A = (E < E_s) ? A_c : A_t;
-  const double s2 = stress * stress;
-  const double sp = (P.nm1_int == 1) ? stress : ((P.nm1_int == 2) ? s2 : s2 * stress);
+  const double sp = (NM1 == 1) ? stress : ((NM1 == 2) ? stress * stress : stress * stress * stress);
   return A * sp;
 }
 

@@ -424,7 +424,7 @@ __global__ void k_theta_thk(DevParams P, Fields2D F, const double *__restrict__ 
 // LDS, wmax: WAVES_PER_BLOCK doubles of LDS. With a list, the block's columns
 // are list entries bid * COLS_PER_BLOCK .. (n_list of them in all) instead of
 // staggered columns. Returns the block's D_max in thread 0.
-template <int LAW, bool FULL, int CPW, bool FASTGP>
+template <int LAW, bool FULL, int CPW, int FASTNM1>
 __device__ __forceinline__ double column_block_impl(const DevParams &P, const ColumnArgs &A, int bid, double *zs,
                                                     double *wmax, const int *list, int n_list) {
   constexpr int COLS_PER_WAVE = CPW;
@@ -493,6 +493,7 @@ __device__ __forceinline__ double column_block_impl(const DevParams &P, const Co
     }
     return COLS_PER_WAVE;
   };
+  constexpr bool FASTGP = FASTNM1 > 0;
   const bool with_age = !FASTGP && P.use_age;
   // the chunk after (c, k0): the rest of column c, else the next column's first
   auto advance = [&](int &c, int &k0) {
@@ -563,7 +564,7 @@ __device__ __forceinline__ double column_block_impl(const DevParams &P, const Co
     const double Ek = 0.5 * (ea + eb);
     double fl;
     if (FASTGP) {
-      fl = flow_gpbld_fast(P, alpha * pressure, Ek, pressure);
+      fl = flow_gpbld_fast<(FASTNM1 > 0 ? FASTNM1 : 1)>(P, alpha * pressure, Ek, pressure);
     } else {
       const double gs = with_age ? grain_size_vostok(P, 0.5 * (ga + gb)) : P.gs;
       fl = flow<LAW>(P, alpha * pressure, Ek, pressure, gs);
@@ -632,9 +633,11 @@ template <int LAW, bool FULL, int CPW>
 __device__ __forceinline__ double column_block(const DevParams &P, const ColumnArgs &A, int bid, double *zs,
                                                double *wmax, const int *list = nullptr, int n_list = 0) {
   if (LAW == LAW_GPBLD && use_gpbld_fast(P)) {
-    return column_block_impl<LAW, FULL, CPW, true>(P, A, bid, zs, wmax, list, n_list);
+    if (P.nm1_int == 1) return column_block_impl<LAW, FULL, CPW, 1>(P, A, bid, zs, wmax, list, n_list);
+    if (P.nm1_int == 2) return column_block_impl<LAW, FULL, CPW, 2>(P, A, bid, zs, wmax, list, n_list);
+    return column_block_impl<LAW, FULL, CPW, 3>(P, A, bid, zs, wmax, list, n_list);
   }
-  return column_block_impl<LAW, FULL, CPW, false>(P, A, bid, zs, wmax, list, n_list);
+  return column_block_impl<LAW, FULL, CPW, 0>(P, A, bid, zs, wmax, list, n_list);
 }
 
 constexpr int COLS_PER_WAVE = 16;
