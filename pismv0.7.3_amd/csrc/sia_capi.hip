@@ -450,7 +450,7 @@ SgColumnArgs sg_column_args(sia_handle *h, const sia_inputs *in) {
 
 SgListArgs sg_list_args(sia_handle *h) {
   return SgListArgs{h->sg_status, h->sg_block_count, h->sg_list, h->sg_nlist, h->sg_counts, h->flux, h->Dstag,
-                    h->sg_fb_list, h->sg_nfb};
+                    h->sg_fb_list, h->sg_nfb, 1};
 }
 
 // flux, D_stag and D_max of a fast update through the surrogate: classify and
@@ -1044,7 +1044,8 @@ int sia_surrogate_collect(sia_surrogate *sg, sia_handle *h, const sia_inputs *in
   if (rc) return rc;
   if ((rc = ensure_sg_buffers(h))) return rc;
   const SgColumnArgs C = sg_column_args(h, in);
-  const SgListArgs L = sg_list_args(h);
+  SgListArgs L = sg_list_args(h);
+  L.strips = 0;  // samples: every stride-th column in staggered index order
   launch_sg_list(h->dp, C, L, h->stream);
   SgFeatureArgs F{};
   F.list = h->sg_list;

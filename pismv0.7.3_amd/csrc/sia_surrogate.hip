@@ -92,7 +92,34 @@ __device__ __forceinline__ double half_sum(double v, int half) {
 // ---------------------------------------------------------------- classification
 constexpr int SG_CLS_BLOCK = 256;
 
-int sg_classify_blocks(const DevParams &P) { return (2 * P.Mx * P.My + SG_CLS_BLOCK - 1) / SG_CLS_BLOCK; }
+// Strip order: a strip of SG_STRIP map columns contributes 2 SG_STRIP entries
+// per row i, so one row of a strip fills about one 32-entry wavefront chunk of
+// the feature pass, and the four chunks of its 4-wavefront blocks are four
+// consecutive rows on one CU. Staggered column (i, j, 0) reads the enthalpy of
+// (i, j) and (i + 1, j): the block's rows share those columns through its L2,
+// which the staggered index order leaves to a wavefront a whole row of the
+// list away, on another XCD.
+constexpr int SG_STRIP = 16;
+
+__host__ __device__ inline int sg_strips(const DevParams &P) { return (P.My + SG_STRIP - 1) / SG_STRIP; }
+
+// enough blocks for either order
+int sg_classify_blocks(const DevParams &P) {
+  const long rows = 2L * P.Mx * P.My, strips = 2L * SG_STRIP * P.Mx * sg_strips(P);
+  return (int)((std::max(rows, strips) + SG_CLS_BLOCK - 1) / SG_CLS_BLOCK);
+}
+
+// staggered column of classification slot g in the list order, -1 for none
+__device__ __forceinline__ int sg_slot_column(const DevParams &P, long g, bool strips) {
+  const long nstag = 2L * P.Mx * P.My;
+  if (!strips) return g < nstag ? (int)g : -1;
+  const int o = (int)(g & 1), jj = (int)((g >> 1) & (SG_STRIP - 1));
+  const long r = g / (2 * SG_STRIP);
+  const int strip = (int)(r / P.Mx), i = (int)(r - (long)strip * P.Mx);
+  const int j = strip * SG_STRIP + jj;
+  if (strip >= sg_strips(P) || j >= P.My) return -1;
+  return 2 * (i * P.My + j) + o;
+}
 
 // status of staggered column s: SG_ZERO (thk == 0, SIAFD.cc:635-641, or zero
 // slope with thk <= Lz: D = 0), SG_FALLBACK (thk above Lz, whatever the slope:
@@ -100,10 +127,9 @@ int sg_classify_blocks(const DevParams &P) { return (2 * P.Mx * P.My + SG_CLS_BL
 // any thk != 0, SIAFD.cc:654; or thk below the second level), else SG_SURR.
 __device__ __forceinline__ void sg_classify_block(const DevParams &P, const SgColumnArgs &C, const SgListArgs &L,
                                                   int bid) {
-  const int s = bid * SG_CLS_BLOCK + threadIdx.x;
-  const int nstag = 2 * P.Mx * P.My;
+  const int s = sg_slot_column(P, (long)bid * SG_CLS_BLOCK + threadIdx.x, L.strips != 0);
   int st = -1;
-  if (s < nstag) {
+  if (s >= 0) {
     const int q = s >> 1, o = s & 1;
     const int i = q / P.My, j = q - i * P.My;
     const int q2 = (o == 0) ? wrapi(i + 1, P.Mx) * P.My + j : i * P.My + wrapi(j + 1, P.My);
@@ -196,9 +222,8 @@ __global__ __launch_bounds__(1024) void k_sg_scan(int nb, SgListArgs L) {
 // ordered compaction of the SG_SURR columns into list and of the classified
 // SG_FALLBACK columns into fb_list; wsum: 2 * SG_CLS_BLOCK / 64 ints of LDS
 __device__ __forceinline__ void sg_scatter_block(const DevParams &P, const SgListArgs &L, int bid, int *wsum) {
-  const int s = bid * SG_CLS_BLOCK + threadIdx.x;
-  const int nstag = 2 * P.Mx * P.My;
-  const unsigned char st = (s < nstag) ? L.status[s] : (unsigned char)SG_ZERO;
+  const int s = sg_slot_column(P, (long)bid * SG_CLS_BLOCK + threadIdx.x, L.strips != 0);
+  const unsigned char st = (s >= 0) ? L.status[s] : (unsigned char)SG_ZERO;
   const bool flag = st == SG_SURR, fflag = st == SG_FALLBACK;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   constexpr int NW = SG_CLS_BLOCK / 64;

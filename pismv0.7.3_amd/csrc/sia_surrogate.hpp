@@ -81,6 +81,9 @@ struct SgListArgs {
   double *flux, *Dstag;   // written for SG_ZERO columns
   int *fb_list;           // [2 Mx My] SG_FALLBACK columns: the classified ones in order, then the MLP's
   int *n_fb;              // device count
+  // list order: 0 staggered index order; 1 strips of SG_STRIP map columns
+  // along j, each walked down i (row i, then j, then the two faces)
+  int strips;
 };
 
 int sg_classify_blocks(const DevParams &P);
