@@ -33,6 +33,13 @@ __device__ __forceinline__ double dpp_f64(double v) {
 
 // value of lane l-1, 0 in lane 0 (wave_shr:1)
 __device__ __forceinline__ double wave_shift_up1(double v) { return dpp_f64<0x138, 0xf, true>(v); }
+// value of lane l-1, `first` in lane 0 (wave_shr:1; without bound_ctrl a lane
+// whose source lies outside the wave keeps the old value)
+__device__ __forceinline__ double wave_shift_up1_from(double v, double first) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(first), __double2loint(v), 0x138, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(first), __double2hiint(v), 0x138, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
 // value of lane l+1, 0 in lane 63 (wave_shl:1)
 __device__ __forceinline__ double wave_shift_down1(double v) { return dpp_f64<0x130, 0xf, true>(v); }
 
@@ -420,10 +427,15 @@ __global__ void k_theta_thk(DevParams P, Fields2D F, const double *__restrict__ 
 // takes flow_gpbld_fast, every other configuration the reference-form flow<LAW>
 // (the choice depends on the parameters only, so a column gets the same
 // arithmetic from every kernel that computes it).
-// block bid of the column grid with COLS_PER_WAVE = CPW; zs: Mz doubles of
-// LDS, wmax: WAVES_PER_BLOCK doubles of LDS. With a list, the block's columns
-// are list entries bid * COLS_PER_BLOCK .. (n_list of them in all) instead of
-// staggered columns. Returns the block's D_max in thread 0.
+// dynamic LDS of the column kernels: (z_k, dz_k) and z_k per level
+static inline size_t column_lds(int Mz) { return 3 * sizeof(double) * (size_t)Mz; }
+
+// block bid of the column grid with COLS_PER_WAVE = CPW; zs: column_lds(Mz)
+// bytes of LDS (16-byte aligned), wmax: WAVES_PER_BLOCK doubles of LDS. With a
+// list, the block's columns are list entries bid * COLS_PER_BLOCK .. (n_list
+// of them in all) instead of staggered columns. Returns the block's D_max in
+// thread 0.
+
 template <int LAW, bool FULL, int CPW, int FASTNM1>
 __device__ __forceinline__ double column_block_impl(const DevParams &P, const ColumnArgs &A, int bid, double *zs,
                                                     double *wmax, const int *list, int n_list) {
@@ -438,7 +450,15 @@ __device__ __forceinline__ double column_block_impl(const DevParams &P, const Co
     const bool mine = (int)threadIdx.x < COLS_PER_BLOCK && sb < nstag && A.only[sb] == ONLY_MARK;
     if (!__syncthreads_or(mine)) return 0.0;
   }
-  for (int k = threadIdx.x; k < Mz; k += blockDim.x) zs[k] = A.z[k];
+  // LDS (column_lds bytes): per level (z_k, z_k - z_{k-1}) (0 at k = 0), one
+  // 16-byte read a level; then z_k alone for kBelowHeight
+  double2 *zh = reinterpret_cast<double2 *>(zs);
+  double *zl = zs + 2 * Mz;
+  for (int k = threadIdx.x; k < Mz; k += blockDim.x) {
+    const double zk = A.z[k];
+    zl[k] = zk;
+    zh[k] = make_double2(zk, (k > 0) ? zk - A.z[k - 1] : 0.0);
+  }
   __syncthreads();
   const int s0 = (bid * WAVES_PER_BLOCK + wv) * COLS_PER_WAVE;
   // list mode: lane c < COLS_PER_WAVE takes list entry s0 + c (nstag: none)
@@ -463,7 +483,7 @@ __device__ __forceinline__ double column_block_impl(const DevParams &P, const Co
     if (c_thk == 0.0) {
       c_ks = -2;
     } else {
-      c_ks = kbelow(zs, Mz, P.Lz, c_thk);
+      c_ks = kbelow(zl, Mz, P.Lz, c_thk);
       if (c_ks < 0) {
         atomicOr(A.err, ERR_THK_LZ);
         c_ks = -1;
@@ -505,9 +525,14 @@ __device__ __forceinline__ double column_block_impl(const DevParams &P, const Co
     }
   };
   // enthalpy (and age) of the chunk (c, k0): level k0 + lane of both cells;
-  // lanes above ks load level ks (a valid address, the value is not used)
+  // lanes above ks load level ks (a valid address, the value is not used).
+  // The column bases are wave-uniform (scalar registers); a lane adds its
+  // 32-bit byte offset.
   int lc = -1, l_ks = 0;
   long l_qa = 0, l_qb = 0;
+  auto at = [](const double *base, long col, unsigned off) {
+    return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base + col) + off);
+  };
   auto load = [&](int c, int k0, double &ea, double &eb, double &ga, double &gb) {
     if (c >= COLS_PER_WAVE) return;
     if (c != lc) {  // wave-uniform: the column's values are read once
@@ -516,66 +541,68 @@ __device__ __forceinline__ double column_block_impl(const DevParams &P, const Co
       l_qa = (long)lane_value(c_qa, c) * Mz;
       l_qb = (long)lane_value(c_qb, c) * Mz;
     }
-    const int k = min(k0 + lane, l_ks);
-    ea = A.E[l_qa + k];
-    eb = A.E[l_qb + k];
+    const unsigned off = (unsigned)min(k0 + lane, l_ks) * 8u;
+    ea = at(A.E, l_qa, off);
+    eb = at(A.E, l_qb, off);
     if (with_age) {
-      ga = A.age[l_qa + k];
-      gb = A.age[l_qb + k];
+      ga = at(A.age, l_qa, off);
+      gb = at(A.age, l_qb, off);
     }
   };
 
   double myD = 0.0;  // lane c: D of column c before the edge override
-  // three chunks in flight: the one evaluated and the next two, whose loads
-  // are issued two chunks ahead of their use
+  // three chunks in flight: the one evaluated (B0) and the next two (B1, B2),
+  // whose loads are issued two chunks ahead of their use; the buffers move up
+  // after each chunk (unrolling the ring instead was measured slower)
   int c = next_valid(0), k0 = 0;
   int c1 = c, k1 = 0;
   advance(c1, k1);
   int c2 = c1, k2 = k1;
   advance(c2, k2);
-  double ea = 0.0, eb = 0.0, ga = 0.0, gb = 0.0;
-  double e1a = 0.0, e1b = 0.0, g1a = 0.0, g1b = 0.0;
-  load(c, 0, ea, eb, ga, gb);
-  load(c1, k1, e1a, e1b, g1a, g1b);
+  double B0[4] = {0.0, 0.0, 0.0, 0.0}, B1[4] = {0.0, 0.0, 0.0, 0.0}, B2[4] = {0.0, 0.0, 0.0, 0.0};
+  load(c, 0, B0[0], B0[1], B0[2], B0[3]);
+  load(c1, k1, B1[0], B1[1], B1[2], B1[3]);
   double Dpart = 0.0, dprev_carry = 0.0, Icarry = 0.0;
   // values of the column being evaluated, read once per column
   int vc = -1, ks = 0;
   double thk = 0.0, alpha = 0.0, c0 = 0.0, top = 0.0;
-  while (c < COLS_PER_WAVE) {
+  // evaluates chunk (c, k0) from Ecur, issues chunk (c2, k2) into Enext
+  auto step = [&](const double (&Ecur)[4], double (&Enext)[4]) -> bool {
+    if (c >= COLS_PER_WAVE) return false;
     if (c != vc) {
       vc = c;
       ks = lane_value(c_ks, c);
       thk = lane_value(c_thk, c);
       alpha = lane_value(c_alpha, c);
       c0 = lane_value(c_c0, c);
-      const double dzt = thk - zs[ks];  // SIAFD.cc:682-684
+      const double dzt = thk - zl[ks];  // SIAFD.cc:682-684
       top = 0.5 * dzt * dzt;
     }
-    double na = 0.0, nb = 0.0, nga = 0.0, ngb = 0.0;
-    load(c2, k2, na, nb, nga, ngb);
+    load(c2, k2, Enext[0], Enext[1], Enext[2], Enext[3]);
 
     const int k = k0 + lane;
     const bool act = k <= ks;
     const int kz = min(k, Mz - 1);
+    const double2 zk2 = zh[kz];  // z_kz, z_kz - z_{kz-1} (0 at kz = 0)
     // depth >= 0 on the active lanes, where this is EnthalpyConverter::pressure
     // (enthalpyConverter.cc:92-98: p_air + rho g 0 = p_air at depth 0)
-    const double depth = thk - zs[kz];
+    const double depth = thk - zk2.x;
     const double pressure = P.p_air + P.rho_i * P.g * depth;
-    const double Ek = 0.5 * (ea + eb);
+    const double Ek = 0.5 * (Ecur[0] + Ecur[1]);
     double fl;
     if (FASTGP) {
       fl = flow_gpbld_fast<(FASTNM1 > 0 ? FASTNM1 : 1)>(P, alpha * pressure, Ek, pressure);
     } else {
-      const double gs = with_age ? grain_size_vostok(P, 0.5 * (ga + gb)) : P.gs;
+      const double gs = with_age ? grain_size_vostok(P, 0.5 * (Ecur[2] + Ecur[3])) : P.gs;
       fl = flow<LAW>(P, alpha * pressure, Ek, pressure, gs);
     }
-    const double delta = act ? c0 * pressure * fl : 0.0;
-    const double dz = (k > 0) ? zs[kz] - zs[kz - 1] : 0.0;
-    double dprev = wave_shift_up1(delta);
-    if (lane == 0) dprev = dprev_carry;
+    // lanes above ks evaluate a value no sum takes: every use below is masked
+    // to the active lanes, and lane k > 0 reads lane k - 1, active when k is
+    const double delta = c0 * pressure * fl;
+    const double dz = zk2.y;
+    const double dprev = wave_shift_up1_from(delta, dprev_carry);
     const double trap = 0.5 * dz * ((depth + dz) * dprev + depth * delta);  // SIAFD.cc:677-680
     if (act && k > 0) Dpart += trap;
-    if (k == ks) Dpart += top * delta;
     if (FULL) {
       const double t = (act && k > 0) ? 0.5 * dz * (dprev + delta) : 0.0;  // SIAFD.cc:863-867
       const double Ik = Icarry + wave_inclusive_scan(t);
@@ -584,7 +611,9 @@ __device__ __forceinline__ double column_block_impl(const DevParams &P, const Co
     }
     dprev_carry = lane_value(delta, 63);
     if (c1 != c) {
-      const double D = lane_value(wave_inclusive_scan(Dpart), 63);
+      // the chunk holds level ks: D = the trapezoids + d_ks^2 / 2 delta_ks
+      // (SIAFD.cc:677-684)
+      const double D = lane_value(wave_inclusive_scan(Dpart), 63) + top * lane_value(delta, ks - k0);
       if (lane == c) myD = D;
       Dpart = 0.0;
       dprev_carry = 0.0;
@@ -595,14 +624,14 @@ __device__ __forceinline__ double column_block_impl(const DevParams &P, const Co
     c1 = c2;
     k1 = k2;
     advance(c2, k2);
-    ea = e1a;
-    eb = e1b;
-    ga = g1a;
-    gb = g1b;
-    e1a = na;
-    e1b = nb;
-    g1a = nga;
-    g1b = ngb;
+    return true;
+  };
+  while (step(B0, B2)) {
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      B0[t] = B1[t];
+      B1[t] = B2[t];
+    }
   }
 
   double dmax = 0.0;
@@ -649,7 +678,7 @@ constexpr int LIST_COLS_PER_BLOCK = WAVES_PER_BLOCK * LIST_COLS_PER_WAVE;
 
 template <int LAW, bool FULL>
 __global__ __launch_bounds__(256) void k_column(DevParams P, ColumnArgs A) {
-  extern __shared__ double zs[];  // Mz levels, staged once per block
+  extern __shared__ __align__(16) double zs[];  // column_lds(Mz) bytes
   __shared__ double wmax[WAVES_PER_BLOCK];
   const double m = column_block<LAW, FULL, COLS_PER_WAVE>(P, A, blockIdx.x, zs, wmax);
   if (threadIdx.x == 0) A.dmax_part[blockIdx.x] = m;
@@ -661,7 +690,7 @@ __global__ __launch_bounds__(256) void k_column(DevParams P, ColumnArgs A) {
 template <int LAW>
 __global__ __launch_bounds__(256) void k_column_list(DevParams P, ColumnArgs A, const int *__restrict__ list,
                                                      const int *__restrict__ n_list) {
-  extern __shared__ double zs[];
+  extern __shared__ __align__(16) double zs[];  // column_lds(Mz) bytes
   __shared__ double wmax[WAVES_PER_BLOCK];
   const int n = *n_list;
   double m = 0.0;
@@ -1578,7 +1607,7 @@ __global__ void k_mask_surface(DevParams P, const double *__restrict__ H_new, co
 template <int LAW>
 static void launch_column_law(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st, int extra) {
   const int nb = column_blocks(P);
-  const size_t lds = sizeof(double) * (size_t)P.Mz;
+  const size_t lds = column_lds(P.Mz);
   if (full) {
     hipLaunchKernelGGL((k_column<LAW, true>), dim3(nb), dim3(256), lds, st, P, A);
   } else {
@@ -1591,7 +1620,7 @@ template <int LAW>
 static void launch_column_list_law(const DevParams &P, const ColumnArgs &A, const int *list, const int *n_list,
                                    hipStream_t st, int extra) {
   const int nb = column_list_blocks(P);
-  hipLaunchKernelGGL((k_column_list<LAW>), dim3(nb), dim3(256), sizeof(double) * (size_t)P.Mz, st, P, A, list,
+  hipLaunchKernelGGL((k_column_list<LAW>), dim3(nb), dim3(256), column_lds(P.Mz), st, P, A, list,
                      n_list);
   hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, st, A.dmax_part, nb + extra, A.dmax_bits);
 }
@@ -1775,7 +1804,7 @@ __global__ __launch_bounds__(256) void k_ens_gradient(const EnsMember *__restric
 template <int LAW, bool FULL, int CPW>
 __global__ __launch_bounds__(256) void k_ens_column(const EnsMember *__restrict__ T, const int *__restrict__ start,
                                                     int nm) {
-  extern __shared__ double zs[];
+  extern __shared__ __align__(16) double zs[];  // column_lds(Mz) bytes
   __shared__ double wmax[WAVES_PER_BLOCK];
   const int m = ens_member(start, nm, blockIdx.x);
   const ColumnArgs A = global_view(T[m].A);
@@ -1824,7 +1853,7 @@ __global__ __launch_bounds__(256) void k_ens_mask(const EnsMember *__restrict__ 
 template <int LAW>
 __global__ __launch_bounds__(256) void k_ens_column_list(const EnsListMember *__restrict__ T,
                                                          const int *__restrict__ start, int nm) {
-  extern __shared__ double zs[];
+  extern __shared__ __align__(16) double zs[];  // column_lds(Mz) bytes
   __shared__ double wmax[WAVES_PER_BLOCK];
   const int m = ens_member(start, nm, blockIdx.x);
   const ColumnArgs A = global_view(T[m].A);
@@ -1841,7 +1870,7 @@ __global__ __launch_bounds__(256) void k_ens_column_list(const EnsListMember *__
 
 void launch_ens_column_list(const EnsListMember *T, const int *start, int nm, int nblocks, int max_mz, int law,
                             hipStream_t st) {
-  const size_t lds = sizeof(double) * (size_t)max_mz;
+  const size_t lds = column_lds(max_mz);
   switch (law) {
   case LAW_ISOGLEN: hipLaunchKernelGGL(k_ens_column_list<LAW_ISOGLEN>, dim3(nblocks), dim3(256), lds, st, T, start, nm); break;
   case LAW_PB: hipLaunchKernelGGL(k_ens_column_list<LAW_PB>, dim3(nblocks), dim3(256), lds, st, T, start, nm); break;
@@ -1861,7 +1890,7 @@ int ens_column_blocks(const DevParams &P, int cpw) {
 
 template <int LAW, int CPW>
 static void ens_column_law(const EnsMember *T, const int *start, int nm, int nblocks, int max_mz, hipStream_t st) {
-  hipLaunchKernelGGL((k_ens_column<LAW, false, CPW>), dim3(nblocks), dim3(256), sizeof(double) * (size_t)max_mz,
+  hipLaunchKernelGGL((k_ens_column<LAW, false, CPW>), dim3(nblocks), dim3(256), column_lds(max_mz),
                      st, T, start, nm);
 }
 
