@@ -98,7 +98,7 @@ BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA
 
 
 # kernels behind each profiling stage (for the rocprof traffic lookup)
-STAGE_KERNEL = {"SIA flux": "k_column", "SIA 3D hor. vel.": "k_velocity_uvw",
+STAGE_KERNEL = {"SIA flux": "k_column", "SIA 3D hor. vel.": "k_velocity_tile",
                 "surrogate features": "k_sg_features<false", "surrogate MLP": "k_sg_mlp"}
 
 

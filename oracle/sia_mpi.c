@@ -189,7 +189,9 @@ typedef struct {
   double *buf_s, *buf_r;
 } decomp;
 
-static int block_start(int M, int p, int r) { return (int)(((long)M * r) / p); }
+/* first index of block r of p: PISM's ownership ranges give block i the size
+   M / p + (M % p > i) (IceGrid.cc:438-446) */
+static int block_start(int M, int p, int r) { return r * (M / p) + (r < M % p ? r : M % p); }
 
 /* ncmax: the most values per point of a field that exchange() will move */
 static void decomp_init(decomp *d, int Mx, int My, int px, int py, int ncmax) {
@@ -197,19 +199,17 @@ static void decomp_init(decomp *d, int Mx, int My, int px, int py, int ncmax) {
   int n;
   MPI_Comm_size(MPI_COMM_WORLD, &n);
   if (px <= 0 || py <= 0) {
-    /* PETSc-like split: px / py close to Mx / My */
-    int best = 1;
-    double bestc = 1e300;
-    for (int a = 1; a <= n; a++) {
-      if (n % a) continue;
-      const double c = fabs(log(((double)Mx / a) / ((double)My / (n / a))));
-      if (c < bestc) {
-        bestc = c;
-        best = a;
-      }
+    /* PISM's default process grid (IceGrid.cc:395-433): Nx = round(sqrt(Mx n /
+       My)), lowered until it divides n; swapped when Mx > My and Nx < Ny */
+    px = (int)(0.5 + sqrt((double)Mx * (double)n / (double)My));
+    if (px < 1) px = 1;
+    while (px > 1 && n % px) px--;
+    py = n / px;
+    if (Mx > My && px < py) {
+      const int t = px;
+      px = py;
+      py = t;
     }
-    px = best;
-    py = n / best;
   }
   d->comm = MPI_COMM_WORLD;
   d->px = px;
