@@ -8,12 +8,13 @@
 //   k_gradient                       surface gradient, all 3 methods (SIAFD.cc:210-503)
 //   k_theta_thk                      get_theta + get_smoothed_thk (PISMBedSmoother.cc:286-420)
 //   k_column<LAW, FULL>              diffusive flux, D, D_max, and (FULL) the column
-//                                    integral I (SIAFD.cc:546-722, 819-879); one
-//                                    64-lane wavefront per staggered column, one lane
+//                                    integral I (SIAFD.cc:546-722, 819-879); a 64-lane
+//                                    wavefront walks 16 staggered columns, one lane
 //                                    per level, wave-tree reductions and scans
-//   k_velocity_uvw<LAW>              u, v (SIAFD.cc:918-958), w (PISMStressBalance.cc:209-286)
+//   k_velocity_tile<LAW, FASTLN>     u, v (SIAFD.cc:918-958), w (PISMStressBalance.cc:209-286)
 //                                    and (LAW >= 0) the volumetric strain heating
-//                                    (PISMStressBalance.cc:370-500) in one pass
+//                                    (PISMStressBalance.cc:370-500) in one pass over
+//                                    4x4 tiles of map columns
 //   k_mass_continuity, k_mask_surface, k_dt, accounting reductions
 //                                    iMgeometry.cc:56-149, 497-822; iMadaptive.cc:118-171
 #include "sia_kernels.hpp"
@@ -74,47 +75,10 @@ constexpr int WAVES_PER_BLOCK = 4;
 
 // ---------------------------------------------------------------- work ordering
 // MI355X dispatches workgroups round-robin over its 8 XCDs, each with its own
-// 4 MB L2. k_velocity_uvw reads the I columns of rows i-2 .. i+1, which at
-// full occupancy do not fit one L2 if each XCD walks whole rows. The grid is
-// cut into strips of STRIP columns (j); XCD x walks strips x, x+8, x+16, ...
-// down the rows (i), so the neighbours of a column were touched a few strip
-// widths earlier by the same XCD and are still in its L2, and the thick
-// interior of the ice sheet is shared evenly by the XCDs.
+// 4 MB L2; kernels whose neighbouring work items share data map blocks to
+// XCDs explicitly (blockIdx.x % N_XCD is the XCD).
 constexpr int N_XCD = 8;
-constexpr int STRIP = 32;
 
-struct StripMap {
-  int Mx, My, nstrips, per_xcd;  // per_xcd: work items reserved per XCD
-};
-
-__host__ __device__ inline StripMap strip_map(int Mx, int My, int per_item) {
-  StripMap m;
-  m.Mx = Mx;
-  m.My = My;
-  m.nstrips = (My + STRIP - 1) / STRIP;
-  const int strips_per_xcd = (m.nstrips + N_XCD - 1) / N_XCD;
-  const long cols = (long)strips_per_xcd * Mx * STRIP;
-  m.per_xcd = (int)((cols + per_item - 1) / per_item);
-  return m;
-}
-
-// column of local index Lp (0, 1, ...) of XCD x; false past the end
-__device__ __forceinline__ bool strip_column(const StripMap &m, int x, int Lp, int &i, int &j) {
-  const int span = m.Mx * STRIP;
-  const int ss = Lp / span;
-  const int st = x + N_XCD * ss;
-  if (st >= m.nstrips) return false;
-  const int r = Lp - ss * span;
-  const int width = min(STRIP, m.My - st * STRIP);
-  if (r >= m.Mx * width) return false;
-  i = r / width;
-  j = st * STRIP + (r - i * width);
-  return true;
-}
-
-// ---------------------------------------------------------------- resets
-// err_acc keeps the flags of every update since the last check: the flags of
-// the previous update move there before err is cleared for this one
 __global__ void k_reset(unsigned long long *dmax_bits, int *err, int *err_acc, unsigned long long *cfl_bits,
                         double max_dt) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
@@ -716,55 +680,6 @@ __global__ void k_reduce_max(const double *__restrict__ part, int n, unsigned lo
 }
 
 // ---------------------------------------------------------------- u, v, w
-// compute_3d_horizontal_velocity (SIAFD.cc:918-958) fused with
-// StressBalance::compute_vertical_velocity (PISMStressBalance.cc:209-286).
-// One wavefront per column (i,j), one lane per level. The wave evaluates u at
-// (i-1,j), (i,j), (i+1,j) and v at (i,j-1), (i,j), (i,j+1) with the
-// reference's formula from the I columns of the 16 staggered neighbours (so
-// each neighbour value is bitwise the one that neighbour writes), then w by a
-// trapezoid prefix scan over the levels. u, v, w are written once; u, v are
-// never read back from HBM.
-//
-// Lanes 0..15 put the 16 staggered neighbours into a per-wave LDS table
-// (I column, its kBelowHeight, h_x, h_y at that staggered point); each level
-// reads I[min(k, ks)] (I is constant above ks, SIAFD.cc:870-872). Above the
-// largest ks of the 16 every input is constant, so u, v and the w integrand
-// are those of level kmax.
-struct VelSlot {
-  const double *col;
-  int ks, pad;
-  double hx, hy;
-};
-
-// staggered index of neighbour slot t of column (i,j)
-//  u(i-1,j): e=I0(i-1,j) w=I0(i-2,j) n=I1(i-1,j) s=I1(i-1,j-1)
-//  u(i,j)  : e=I0(i,j)   w=I0(i-1,j) n=I1(i,j)   s=I1(i,j-1)
-//  u(i+1,j): e=I0(i+1,j) w=I0(i,j)   n=I1(i+1,j) s=I1(i+1,j-1)
-//  v(i,j-1): e=I0(i,j-1) w=I0(i-1,j-1) n=I1(i,j-1) s=I1(i,j-2)
-//  v(i,j+1): e=I0(i,j+1) w=I0(i-1,j+1) n=I1(i,j+1) s=I1(i,j)
-__device__ __forceinline__ int vel_slot(int t, int i, int j, int Mx, int My) {
-  const int im = wrapi(i - 1, Mx), ip = wrapi(i + 1, Mx), jm = wrapi(j - 1, My), jp = wrapi(j + 1, My);
-  const int im2 = wrapi(i - 2, Mx), jm2 = wrapi(j - 2, My);
-  switch (t) {
-  case 0: return 2 * (i * My + j);          // I0(i,j)
-  case 1: return 2 * (im * My + j);         // I0(i-1,j)
-  case 2: return 2 * (i * My + j) + 1;      // I1(i,j)
-  case 3: return 2 * (i * My + jm) + 1;     // I1(i,j-1)
-  case 4: return 2 * (im2 * My + j);        // I0(i-2,j)
-  case 5: return 2 * (im * My + j) + 1;     // I1(i-1,j)
-  case 6: return 2 * (im * My + jm) + 1;    // I1(i-1,j-1)
-  case 7: return 2 * (ip * My + j);         // I0(i+1,j)
-  case 8: return 2 * (ip * My + j) + 1;     // I1(i+1,j)
-  case 9: return 2 * (ip * My + jm) + 1;    // I1(i+1,j-1)
-  case 10: return 2 * (i * My + jm);        // I0(i,j-1)
-  case 11: return 2 * (im * My + jm);       // I0(i-1,j-1)
-  case 12: return 2 * (i * My + jm2) + 1;   // I1(i,j-2)
-  case 13: return 2 * (i * My + jp);        // I0(i,j+1)
-  case 14: return 2 * (im * My + jp);       // I0(i-1,j+1)
-  default: return 2 * (i * My + jp) + 1;    // I1(i,j+1)
-  }
-}
-
 // -0.25 * (I_e h_e + I_w h_w + I_n h_n + I_s h_s) + basal (SIAFD.cc:949-956)
 __device__ __forceinline__ double uv_formula(double Ie, double he, double Iw, double hw, double In, double hn,
                                              double Is, double hs, double basal) {
@@ -777,214 +692,20 @@ __device__ __forceinline__ double strain_D2(double u_x, double u_y, double u_z, 
                 0.5 * ((u_y + v_x) * (u_y + v_x) + u_z * u_z + v_z * v_z));
 }
 
-// LAW >= 0: the volumetric strain heating of the same column in the same pass
-// (StressBalance::compute_volumetric_strain_heating, PISMStressBalance.cc:370-500).
-// It needs u, v of the column and of its four neighbours at each level; u at
-// (i-1,j), (i+1,j) and v at (i,j-1), (i,j+1) are the w-divergence terms, and
-// u at (i,j-+1), v at (i-+1,j) come from the same 16 I columns with the other
-// gradient component, evaluated exactly as the neighbour's own wavefront
-// evaluates its u, v (same operands, same order), so they equal the stored
-// fields. u_z, v_z take the levels k-+1 from the neighbouring lanes; lane 63's
-// level k+1 is one more (wave-uniform) evaluation at k0 + 64, lane 0's k-1 is
-// carried from the previous chunk. Sigma = 2 e^(-1/n) B D^(1+1/n) is evaluated
-// as 2 e^(-1/n) exp(-ln A / n + (1/n + 1)/2 ln D^2), 0 above kBelowHeight(H).
-template <int LAW>
-__global__ __launch_bounds__(256) void k_velocity_uvw(DevParams P, VelArgs A, StripMap sm) {
-  constexpr bool HEAT = LAW >= 0;
-  extern __shared__ double zs[];  // z [Mz], then (HEAT) 1 / (z_{k+1} - z_{k-1}) [Mz] (k = 0: 1 / (z_1 - z_0))
-  __shared__ VelSlot tbl[WAVES_PER_BLOCK][16];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int Mx = P.Mx, My = P.My, Mz = P.Mz;
-  double *idz = zs + Mz;
-  for (int k = threadIdx.x; k < Mz; k += blockDim.x) {
-    zs[k] = A.z[k];
-    if (HEAT) {
-      idz[k] = (k == 0) ? 1.0 / (A.z[1] - A.z[0]) : ((k + 1 < Mz) ? 1.0 / (A.z[k + 1] - A.z[k - 1]) : 0.0);
-    }
-  }
-  __syncthreads();
-  // block b runs on XCD b % 8 as the (b / 8)-th block of that XCD
-  const int x = blockIdx.x % N_XCD;
-  const int Lp = __builtin_amdgcn_readfirstlane((blockIdx.x / N_XCD) * WAVES_PER_BLOCK + wv);
-  int i, j;
-  if (!strip_column(sm, x, Lp, i, j)) return;
-  int myks = 0;
-  if (lane < 16) {
-    const int sid = vel_slot(lane, i, j, Mx, My);
-    VelSlot e;
-    e.ks = A.ks[sid];
-    e.pad = 0;
-    e.col = ((sid & 1) ? A.I1 : A.I0) + (long)(sid >> 1) * Mz;
-    e.hx = A.hx[sid];
-    e.hy = A.hy[sid];
-    tbl[wv][lane] = e;
-    myks = e.ks;
-  }
-  // the table is private to this wavefront: a wavefront-scope fence orders
-  // its LDS writes before the reads below
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  int kmax = 0;
-#pragma unroll
-  for (int t = 0; t < 16; t++) kmax = max(kmax, lane_value(myks, t));
-  const int q = i * My + j;
-  const int im = wrapi(i - 1, Mx), ip = wrapi(i + 1, Mx), jm = wrapi(j - 1, My), jp = wrapi(j + 1, My);
-  const double ub_c = A.ub ? A.ub[q] : 0.0, ub_w = A.ub ? A.ub[im * My + j] : 0.0,
-               ub_e = A.ub ? A.ub[ip * My + j] : 0.0;
-  const double vb_c = A.vb ? A.vb[q] : 0.0, vb_s = A.vb ? A.vb[i * My + jm] : 0.0,
-               vb_n = A.vb ? A.vb[i * My + jp] : 0.0;
-  // margin-aware differences (PISMStressBalance.cc:223-260)
-  const double *mask = A.mask;
-  const int m = as_int(mask[q]);
-  const int me = as_int(mask[ip * My + j]), mw = as_int(mask[im * My + j]);
-  const int mn = as_int(mask[i * My + jp]), ms = as_int(mask[i * My + jm]);
-  double west = 1, east = 1, south = 1, north = 1;
-  if ((m_icy(m) && m_ice_free(me)) || (m_ice_free(m) && m_icy(me))) east = 0;
-  if ((m_icy(m) && m_ice_free(mw)) || (m_ice_free(m) && m_icy(mw))) west = 0;
-  const double D_x = (east + west > 0) ? 1.0 / (P.dx * (east + west)) : 0.0;
-  if ((m_icy(m) && m_ice_free(mn)) || (m_ice_free(m) && m_icy(mn))) north = 0;
-  if ((m_icy(m) && m_ice_free(ms)) || (m_ice_free(m) && m_icy(ms))) south = 0;
-  const double D_y = (north + south > 0) ? 1.0 / (P.dy * (north + south)) : 0.0;
-  const double w0 = A.bmr ? -A.bmr[q] : 0.0;
-  // strain heating: kBelowHeight(H) (throws above Lz, IceGrid.cc:381-389),
-  // the other neighbours' basal velocities, and the levels the formulas run
-  // to (ksH + 1 for u_z at ksH; above kmax every I column is clamped, so the
-  // formulas are constant there and equal the stored values)
-  int ksH = -1;
-  double Hij = 0.0, ub_s = 0.0, ub_n = 0.0, vb_w = 0.0, vb_e = 0.0;
-  if (HEAT) {
-    Hij = A.H[q];
-    ksH = kbelow(zs, Mz, P.Lz, Hij);
-    if (ksH < 0 && lane == 0) atomicOr(A.err, ERR_THK_LZ);
-    if (A.ub) {
-      ub_s = A.ub[i * My + jm];
-      ub_n = A.ub[i * My + jp];
-    }
-    if (A.vb) {
-      vb_w = A.vb[im * My + j];
-      vb_e = A.vb[ip * My + j];
-    }
-  }
-  const int kev = HEAT ? max(kmax, min(ksH + 1, Mz - 1)) : kmax;
-  const double *__restrict__ E_ij = HEAT ? A.E + (long)q * Mz : nullptr;
-  double *__restrict__ So = HEAT ? A.Sigma + (long)q * Mz : nullptr;
-  double carry_u = 0.0, carry_v = 0.0;  // u, v at the previous chunk's last level
-  double *__restrict__ uo = A.u + (long)q * Mz;
-  double *__restrict__ vo = A.v + (long)q * Mz;
-  double *__restrict__ wo = A.w + (long)q * Mz;
-  const VelSlot *T = tbl[wv];
-  double uc_t = 0.0, vc_t = 0.0, int_t = 0.0;  // values at kmax, constant above it
-  double carry_int = 0.0, carry_sum = 0.0;
-  for (int k0 = 0; k0 < Mz; k0 += 64) {
-    const int k = k0 + lane;
-    const bool act = k < Mz;
-    double uc, vc, integrand, sig = 0.0;
-    if (k0 <= kev) {
-      const int kk = act ? k : Mz - 1;
-      double Iv[16];
-      if (kmax == 0) {
-        // every neighbour is ice-free or thinner than the first level: I = I[0] = 0
-#pragma unroll
-        for (int t = 0; t < 16; t++) Iv[t] = 0.0;
-      } else {
-#pragma unroll
-        for (int t = 0; t < 16; t++) Iv[t] = glb(T[t].col)[min(kk, T[t].ks)];  // device memory: global loads
-      }
-      uc = uv_formula(Iv[0], T[0].hx, Iv[1], T[1].hx, Iv[2], T[2].hx, Iv[3], T[3].hx, ub_c);
-      vc = uv_formula(Iv[0], T[0].hy, Iv[1], T[1].hy, Iv[2], T[2].hy, Iv[3], T[3].hy, vb_c);
-      const double uw = uv_formula(Iv[1], T[1].hx, Iv[4], T[4].hx, Iv[5], T[5].hx, Iv[6], T[6].hx, ub_w);
-      const double ue = uv_formula(Iv[7], T[7].hx, Iv[0], T[0].hx, Iv[8], T[8].hx, Iv[9], T[9].hx, ub_e);
-      const double vs = uv_formula(Iv[10], T[10].hy, Iv[11], T[11].hy, Iv[3], T[3].hy, Iv[12], T[12].hy, vb_s);
-      const double vn = uv_formula(Iv[13], T[13].hy, Iv[14], T[14].hy, Iv[15], T[15].hy, Iv[2], T[2].hy, vb_n);
-      // PISMStressBalance.cc:276-278
-      const double u_x = D_x * (west * (uc - uw) + east * (ue - uc));
-      const double v_y = D_y * (south * (vc - vs) + north * (vn - vc));
-      integrand = u_x + v_y;
-      if (HEAT && k0 <= ksH) {
-        // u, v at level k0 + 64 for lane 63's k + 1 (the next chunk's lane 0)
-        double un_c = 0.0, vn_c = 0.0;
-        if (k0 + 63 <= ksH) {
-          const int kn = min(k0 + 64, Mz - 1);
-          double J[4];
-#pragma unroll
-          for (int t = 0; t < 4; t++) J[t] = (kmax == 0) ? 0.0 : glb(T[t].col)[min(kn, T[t].ks)];
-          un_c = uv_formula(J[0], T[0].hx, J[1], T[1].hx, J[2], T[2].hx, J[3], T[3].hx, ub_c);
-          vn_c = uv_formula(J[0], T[0].hy, J[1], T[1].hy, J[2], T[2].hy, J[3], T[3].hy, vb_c);
-        }
-        double u_up = wave_shift_down1(uc), v_up = wave_shift_down1(vc);
-        double u_dn = wave_shift_up1(uc), v_dn = wave_shift_up1(vc);
-        if (lane == 63) {
-          u_up = un_c;
-          v_up = vn_c;
-        }
-        if (lane == 0) {
-          u_dn = carry_u;
-          v_dn = carry_v;
-        }
-        if (k <= ksH) {
-          // the neighbours' u, v as their own wavefronts evaluate them
-          const double us = uv_formula(Iv[10], T[10].hx, Iv[11], T[11].hx, Iv[3], T[3].hx, Iv[12], T[12].hx, ub_s);
-          const double un = uv_formula(Iv[13], T[13].hx, Iv[14], T[14].hx, Iv[15], T[15].hx, Iv[2], T[2].hx, ub_n);
-          const double vw = uv_formula(Iv[1], T[1].hy, Iv[4], T[4].hy, Iv[5], T[5].hy, Iv[6], T[6].hy, vb_w);
-          const double ve = uv_formula(Iv[7], T[7].hy, Iv[0], T[0].hy, Iv[8], T[8].hy, Iv[9], T[9].hy, vb_e);
-          const double pressure = ec_pressure(P, Hij - zs[k]);
-          const double lnA = log_softness<LAW>(P, E_ij[k], pressure);
-          const double hu_x = D_x * (west * (uc - uw) + east * (ue - uc)),
-                       hu_y = D_y * (south * (uc - us) + north * (un - uc)),
-                       hv_x = D_x * (west * (vc - vw) + east * (ve - vc)),
-                       hv_y = D_y * (south * (vc - vs) + north * (vn - vc));
-          // one-sided at the base (PISMStressBalance.cc:464-476)
-          const double u_z = (k > 0) ? (u_up - u_dn) * idz[k] : (u_up - uc) * idz[0];
-          const double v_z = (k > 0) ? (v_up - v_dn) * idz[k] : (v_up - vc) * idz[0];
-          const double d2 = strain_D2(hu_x, hu_y, u_z, hv_x, hv_y, v_z);
-          sig = (d2 > 0.0) ? 2.0 * P.e_pow * exp(P.hard_pow * lnA + P.heat_exp * log(d2)) : 0.0;
-        }
-      }
-      if (HEAT) {
-        carry_u = lane_value(uc, 63);
-        carry_v = lane_value(vc, 63);
-      }
-      if (kev < k0 + 64) {
-        uc_t = lane_value(uc, kev - k0);
-        vc_t = lane_value(vc, kev - k0);
-        int_t = lane_value(integrand, kev - k0);
-      }
-    } else {
-      uc = uc_t;
-      vc = vc_t;
-      integrand = int_t;
-    }
-    double dz = 0.0;
-    if (act) {
-      // u, v, w stream out once per update: non-temporal stores keep them
-      // from displacing the I columns the neighbouring wavefronts read from L2
-      __builtin_nontemporal_store(uc, uo + k);
-      __builtin_nontemporal_store(vc, vo + k);
-      if (HEAT) __builtin_nontemporal_store(sig, So + k);
-      if (k > 0) dz = zs[k] - zs[k - 1];
-    } else {
-      integrand = 0.0;
-    }
-    double iprev = wave_shift_up1(integrand);
-    if (lane == 0) iprev = carry_int;
-    const double t = (act && k > 0) ? 0.5 * (integrand + iprev) * dz : 0.0;  // PISMStressBalance.cc:280-282
-    const double S = carry_sum + wave_inclusive_scan(t);
-    if (act) __builtin_nontemporal_store(w0 - S, wo + k);
-    carry_sum = lane_value(S, 63);
-    carry_int = lane_value(integrand, 63);
-  }
-}
-
 // ---------------------------------------------------------------- u, v, w by tiles
-// The same outputs as k_velocity_uvw, computed by tiles of TI x TJ map columns:
+// compute_3d_horizontal_velocity (SIAFD.cc:918-958) fused with
+// StressBalance::compute_vertical_velocity (PISMStressBalance.cc:209-286) and,
+// for LAW >= 0, the volumetric strain heating (PISMStressBalance.cc:370-500),
+// computed by tiles of TI x TJ map columns:
 // per chunk of KC levels, u and v of the tile and of a ring of one column
 // around it go to LDS once (each from the four I columns of its faces, with
 // the reference's formula, so a ring value is bitwise the one its own tile
 // stores); the w integrand, w (a prefix scan over the wavefront's lanes, one
 // level each) and the strain heating read them there.
 // A chunk holds the levels k0-1 .. k0+KC of u and v (u_z at k0 and k0+KC-1);
-// its last two slots become the next chunk's first two. Above the tile's
+// its slot KC (level k0+KC-1) becomes the next chunk's slot 0. Sigma =
+// 2 e^(-1/n) B D^(1+1/n) is evaluated as 2 e^(-1/n) exp(-ln A / n + (1/n + 1)/2
+// ln D^2) with ln A from the flow law, 0 above kBelowHeight(H). Above the tile's
 // largest kBelowHeight (of its faces and of H) u, v and the integrand are
 // constant and the strain heating is 0, so those levels are written directly,
 // w by the trapezoid of a constant integrand (S + int (z_k - z_top)). Each
