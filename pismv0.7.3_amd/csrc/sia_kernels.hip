@@ -165,42 +165,42 @@ __device__ __forceinline__ long idx2(const DevParams &P, int i, int j) {
 }
 
 // haseloff first pass (SIAFD.cc:402-438): x-derivative on the i-offset grid and its weight
-__device__ __forceinline__ void haseloff_x(const DevParams &P, const Fields2D &F, int i, int j,
-                                           double &hx0, double &wi) {
-  const long a = idx2(P, i, j), c = idx2(P, i + 1, j);
-  const int m0 = as_int(F.mask[a]), m1 = as_int(F.mask[c]);
-  const double h0 = F.h[a], h1 = F.h[c];
+template <class Acc>
+__device__ __forceinline__ void haseloff_x(const DevParams &P, const Acc &F, int i, int j, double &hx0,
+                                           double &wi) {
+  const int m0 = F.mask(i, j), m1 = F.mask(i + 1, j);
+  const double h0 = F.h(i, j), h1 = F.h(i + 1, j);
   if ((m_floating_ice(m0) && m_ice_free_ocean(m1)) || (m_ice_free_ocean(m0) && m_floating_ice(m1))) {
     hx0 = 0.0; wi = 0.0;
-  } else if ((m_icy(m0) && m_ice_free(m1) && F.b[c] > h0) || (m_ice_free(m0) && m_icy(m1) && F.b[a] > h1)) {
+  } else if ((m_icy(m0) && m_ice_free(m1) && F.b(i + 1, j) > h0) || (m_ice_free(m0) && m_icy(m1) && F.b(i, j) > h1)) {
     hx0 = 0.0; wi = 0.0;
   } else {
     hx0 = (h1 - h0) / P.dx; wi = 1.0;
   }
 }
-__device__ __forceinline__ void haseloff_y(const DevParams &P, const Fields2D &F, int i, int j,
-                                           double &hy1, double &wj) {
-  const long a = idx2(P, i, j), c = idx2(P, i, j + 1);
-  const int m0 = as_int(F.mask[a]), m1 = as_int(F.mask[c]);
-  const double h0 = F.h[a], h1 = F.h[c];
+template <class Acc>
+__device__ __forceinline__ void haseloff_y(const DevParams &P, const Acc &F, int i, int j, double &hy1,
+                                           double &wj) {
+  const int m0 = F.mask(i, j), m1 = F.mask(i, j + 1);
+  const double h0 = F.h(i, j), h1 = F.h(i, j + 1);
   if ((m_floating_ice(m0) && m_ice_free_ocean(m1)) || (m_ice_free_ocean(m0) && m_floating_ice(m1))) {
     hy1 = 0.0; wj = 0.0;
-  } else if ((m_icy(m0) && m_ice_free(m1) && F.b[c] > h0) || (m_ice_free(m0) && m_icy(m1) && F.b[a] > h1)) {
+  } else if ((m_icy(m0) && m_ice_free(m1) && F.b(i, j + 1) > h0) || (m_ice_free(m0) && m_icy(m1) && F.b(i, j) > h1)) {
     hy1 = 0.0; wj = 0.0;
   } else {
     hy1 = (h1 - h0) / P.dy; wj = 1.0;
   }
 }
 
-// surface gradient at map point q on both staggered grids
-__device__ __forceinline__ void gradient_point(const DevParams &P, const Fields2D &F, double *__restrict__ hx,
-                                               double *__restrict__ hy, long q) {
-  int i, j;
-  map_ij(q, P.My, i, j);
+// surface gradient at map point q = (i, j) on both staggered grids; F reads
+// H, b, h and the mask at (i', j') for |i' - i|, |j' - j| <= 1
+template <class Acc>
+__device__ __forceinline__ void gradient_at(const DevParams &P, const Acc &F, double *__restrict__ hx,
+                                            double *__restrict__ hy, long q, int i, int j) {
   const double dx = P.dx, dy = P.dy;
   double hx0, hy0, hx1, hy1;
   if (P.grad == 1) {  // mahaffy, SIAFD.cc:305-316
-#define HH(a, c) F.h[idx2(P, (a), (c))]
+#define HH(a, c) F.h((a), (c))
     hx0 = (HH(i + 1, j) - HH(i, j)) / dx;
     hy0 = (+HH(i + 1, j + 1) + HH(i, j + 1) - HH(i + 1, j - 1) - HH(i, j - 1)) / (4.0 * dy);
     hy1 = (HH(i, j + 1) - HH(i, j)) / dy;
@@ -209,8 +209,8 @@ __device__ __forceinline__ void gradient_point(const DevParams &P, const Fields2
   } else if (P.grad == 2) {  // eta, SIAFD.cc:210-284
     const double n = P.n, etapow = (2.0 * n + 2.0) / n, invpow = 1.0 / etapow,
                  dinvpow = (-n - 2.0) / (2.0 * n + 2.0);
-#define ETA(a, c) pow(F.H[idx2(P, (a), (c))], etapow)
-#define BB(a, c) F.b[idx2(P, (a), (c))]
+#define ETA(a, c) pow(F.H((a), (c)), etapow)
+#define BB(a, c) F.b((a), (c))
     {
       const double e00 = ETA(i, j), e10 = ETA(i + 1, j);
       const double mean_eta = 0.5 * (e10 + e00);
@@ -243,7 +243,7 @@ __device__ __forceinline__ void gradient_point(const DevParams &P, const Fields2
     double wi_c, wj_c;
     haseloff_x(P, F, i, j, hx0, wi_c);
     haseloff_y(P, F, i, j, hy1, wj_c);
-    const bool icy = m_icy(as_int(F.mask[q]));
+    const bool icy = m_icy(F.mask(i, j));
     // x-derivative, j-offset (SIAFD.cc:444-470)
     {
       double a1, w1, a2, w2, a3, w3;  // (i-1,j), (i-1,j+1), (i,j+1)
@@ -285,10 +285,58 @@ __device__ __forceinline__ void gradient_point(const DevParams &P, const Fields2
   hy[2 * q + 1] = hy1;
 }
 
-__global__ void k_gradient(DevParams P, Fields2D F, double *__restrict__ hx, double *__restrict__ hy) {
-  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= (long)P.Mx * P.My) return;
-  gradient_point(P, F, hx, hy, q);
+// field reads through global memory, with the periodic index wraps
+struct GlobalFields {
+  const DevParams &P;
+  const Fields2D &F;
+  __device__ double H(int i, int j) const { return F.H[idx2(P, i, j)]; }
+  __device__ double b(int i, int j) const { return F.b[idx2(P, i, j)]; }
+  __device__ double h(int i, int j) const { return F.h[idx2(P, i, j)]; }
+  __device__ int mask(int i, int j) const { return as_int(F.mask[idx2(P, i, j)]); }
+};
+
+__device__ __forceinline__ void gradient_point(const DevParams &P, const Fields2D &F, double *__restrict__ hx,
+                                               double *__restrict__ hy, long q) {
+  int i, j;
+  map_ij(q, P.My, i, j);
+  gradient_at(P, GlobalFields{P, F}, hx, hy, q, i, j);
+}
+
+// k_gradient: a block stages a tile of GT_I x GT_J map points and its ring of
+// one point (periodic wraps) in LDS, then each thread evaluates one point from
+// there with the same arithmetic as gradient_point
+constexpr int GT_I = 8, GT_J = 32, GT_SJ = GT_J + 2, GT_N = (GT_I + 2) * GT_SJ;
+
+struct TileFields {
+  const double *sH, *sb, *sh;
+  const int *sm;
+  int i0, j0;  // global index of the tile's first point; the ring starts at i0 - 1, j0 - 1
+  __device__ int at(int i, int j) const { return (i - i0 + 1) * GT_SJ + (j - j0 + 1); }
+  __device__ double H(int i, int j) const { return sH[at(i, j)]; }
+  __device__ double b(int i, int j) const { return sb[at(i, j)]; }
+  __device__ double h(int i, int j) const { return sh[at(i, j)]; }
+  __device__ int mask(int i, int j) const { return sm[at(i, j)]; }
+};
+
+__global__ __launch_bounds__(GT_I * GT_J) void k_gradient(DevParams P, Fields2D F, double *__restrict__ hx,
+                                                          double *__restrict__ hy) {
+  __shared__ double sH[GT_N], sb[GT_N], sh[GT_N];
+  __shared__ int sm[GT_N];
+  const int tj = (P.My + GT_J - 1) / GT_J;
+  const int i0 = (blockIdx.x / tj) * GT_I, j0 = (blockIdx.x % tj) * GT_J;
+  const bool eta = P.grad == 2, has_b = P.grad != 1;
+  for (int e = threadIdx.x; e < GT_N; e += blockDim.x) {
+    const int a = e / GT_SJ, c = e - a * GT_SJ;
+    const long g = idx2(P, i0 - 1 + a, j0 - 1 + c);
+    sh[e] = F.h[g];
+    sm[e] = as_int(F.mask[g]);
+    if (has_b) sb[e] = F.b[g];
+    if (eta) sH[e] = F.H[g];
+  }
+  __syncthreads();
+  const int i = i0 + (int)threadIdx.x / GT_J, j = j0 + (int)threadIdx.x % GT_J;
+  if (i >= P.Mx || j >= P.My) return;
+  gradient_at(P, TileFields{sH, sb, sh, sm, i0, j0}, hx, hy, (long)i * P.My + j, i, j);
 }
 
 // SIAFD_Regional::compute_surface_gradient's override (regional.cc:45-109): where
@@ -1412,9 +1460,9 @@ void launch_bed_smoother(const DevParams &P, double s2, double s3, double s4, co
 
 void launch_gradient(const DevParams &P, const double *H, const double *b, const double *h,
                      const double *mask, double *hx, double *hy, hipStream_t st) {
-  const long n = (long)P.Mx * P.My;
   Fields2D F{H, b, h, mask};
-  hipLaunchKernelGGL(k_gradient, dim3(blocks_for(n, 256)), dim3(256), 0, st, P, F, hx, hy);
+  const long tiles = (long)((P.Mx + GT_I - 1) / GT_I) * ((P.My + GT_J - 1) / GT_J);
+  hipLaunchKernelGGL(k_gradient, dim3((unsigned)tiles), dim3(GT_I * GT_J), 0, st, P, F, hx, hy);
 }
 
 void launch_regional_gradient(const DevParams &P, const double *nmm, const double *hst, double *hx, double *hy,
