@@ -2,7 +2,8 @@
 C ABI:
 
 * config 1: the 61x61x31 dome (isothermal Glen, A = 1e-16 Pa-3 a-1): one full
-  update (D, flux, u, v, w, strain heating to the bars of test_gpu_parity.py),
+  update (D, flux, u, v, w, strain heating to the bars of test_gpu_parity.py;
+  the device flux also against the Vialov closed form M0 r / 2 to 1 %),
   then 100 steps with the diffusivity/CFL-limited dt computed on the device;
   every step's dt and H against the oracle;
 * config 2: a 96 x 96 window of the 1501x2801x401 polythermal sheet across
@@ -86,8 +87,21 @@ def step_both(S, oracle, w, nsteps, full):
 def test_config1_dome_full_update_and_100_steps(S, oracle):
     say("[gpu] config 1: dome 61x61x31, full update + 100 steps against the oracle")
     w = W.dome()
-    errs = check_against_oracle(run_gpu(w), run_oracle(oracle, w), w.grid, label="config 1")
+    g = run_gpu(w)
+    errs = check_against_oracle(g, run_oracle(oracle, w), w.grid, label="config 1")
     say("  full update, max rel errors:", {k: "%.1e" % v for k, v in errs.items()})
+    # the closed form on the device's own flux: the Vialov profile's steady
+    # flux |Q(r)| = M0 r / 2 (exactTestsABCDE.c:28-53) along both axes to 1 %,
+    # as the discrete SIA reproduces it away from the dome and the margin
+    M0 = W.vialov_smb(w.params)
+    x = T.horizontal_coordinates(750e3, 61)
+    fl = g["flux"]
+    fl = (fl.cpu().numpy() if hasattr(fl, "cpu") else np.asarray(fl)).reshape(61, 61, 2)
+    for i in range(34, 56):
+        xs = x[i] + 12.5e3
+        assert fl[i, 30, 0] == pytest.approx(M0 * xs / 2, rel=1e-2)
+        assert fl[30, i, 1] == pytest.approx(M0 * xs / 2, rel=1e-2)
+        assert fl[60 - i - 1, 30, 0] == pytest.approx(-M0 * xs / 2, rel=1e-2)
     step_both(S, oracle, w, 100, full=True)
 
 
