@@ -29,6 +29,7 @@ struct sia_handle {
   std::string last_error;
 
   double *d_z = nullptr;
+  double *d_idz = nullptr;  // 1 / (z_{k+1} - z_{k-1}) per level (VelArgs::idz)
   // bed smoother state (BedSmoother members, PISMBedSmoother.hh:79-124)
   double *topgs = nullptr, *maxtl = nullptr, *C2 = nullptr, *C3 = nullptr, *C4 = nullptr;
   int64_t bed_counter = -1;  // SIAFD::init sets it to -1 (SIAFD.cc:95-97)
@@ -162,6 +163,8 @@ void make_dev_params(const sia_grid &g, const sia_params &p, DevParams &d) {
   {
     const ExpCoef ec = exp_coef();
     for (int i = 0; i < 11; i++) d.exp_c[i] = ec.c[i];
+    const LogCoef lc = log_coef();
+    for (int i = 0; i < 10; i++) d.log_c[i] = lc.c[i];
   }
   d.alpha_float = 1 - p.ice_density / p.sea_water_density;
   d.sea_level = p.sea_level;
@@ -399,6 +402,7 @@ int update_impl(sia_handle *h, const sia_inputs *in, int fast, bool allow_sync) 
     Stage stage(h, SIA_EV_3D_HOR_VEL);
     VelArgs V{};
     V.z = h->d_z;
+    V.idz = h->d_idz;
     V.mask = in->mask;
     V.I0 = h->I0;
     V.I1 = h->I1;
@@ -654,7 +658,8 @@ int sia_create(const sia_grid *grid, const sia_params *params, sia_handle **out)
   make_dev_params(h->grid, h->params, h->dp);
   const size_t n2 = (size_t)grid->Mx * grid->My;
   int rc;
-  if ((rc = dalloc(h, &h->d_z, (size_t)grid->Mz)) || (rc = dalloc(h, &h->topgs, n2)) ||
+  if ((rc = dalloc(h, &h->d_z, (size_t)grid->Mz)) || (rc = dalloc(h, &h->d_idz, (size_t)grid->Mz)) ||
+      (rc = dalloc(h, &h->topgs, n2)) ||
       (rc = dalloc(h, &h->maxtl, n2)) || (rc = dalloc(h, &h->C2, n2)) || (rc = dalloc(h, &h->C3, n2)) ||
       (rc = dalloc(h, &h->C4, n2)) || (rc = dalloc(h, &h->hx, 2 * n2)) || (rc = dalloc(h, &h->hy, 2 * n2)) ||
       (rc = dalloc(h, &h->theta, n2)) || (rc = dalloc(h, &h->thks, n2)) || (rc = dalloc(h, &h->flux, 2 * n2)) ||
@@ -667,7 +672,14 @@ int sia_create(const sia_grid *grid, const sia_params *params, sia_handle **out)
     sia_destroy(h);
     return rc;
   }
+  // the strain heating's centred vertical differences (PISMStressBalance.cc:464-476)
+  std::vector<double> idz(grid->Mz);
+  for (int k = 0; k < grid->Mz; k++) {
+    const double *z = h->z.data();
+    idz[k] = (k == 0) ? 1.0 / (z[1] - z[0]) : ((k + 1 < grid->Mz) ? 1.0 / (z[k + 1] - z[k - 1]) : 0.0);
+  }
   if (hipMemcpy(h->d_z, h->z.data(), sizeof(double) * grid->Mz, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->d_idz, idz.data(), sizeof(double) * grid->Mz, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemset(h->err, 0, sizeof(int)) != hipSuccess || hipMemset(h->err_acc, 0, sizeof(int)) != hipSuccess ||
       hipDeviceSynchronize() != hipSuccess) {
     sia_destroy(h);
@@ -682,7 +694,7 @@ int sia_destroy(sia_handle *h) {
   if (h->sg) h->sg->bound--;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
   else (void)hipDeviceSynchronize();
-  void *bufs[] = {h->d_z, h->topgs, h->maxtl, h->C2, h->C3, h->C4, h->hx, h->hy, h->theta, h->thks,
+  void *bufs[] = {h->d_z, h->d_idz, h->topgs, h->maxtl, h->C2, h->C3, h->C4, h->hx, h->hy, h->theta, h->thks,
                   h->flux, h->Dstag, h->I0, h->I1, h->u, h->v, h->w, h->Sigma, h->ks, h->dmax_bits, h->cfl_bits, h->err, h->err_acc,
                   h->dt_dev, h->acc_dev, h->partials, h->H_tmp, h->dmax_part, h->sl_hx, h->sl_hy,
                   h->sl_x, h->sl_y, h->sg_status,

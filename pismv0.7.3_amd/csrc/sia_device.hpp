@@ -44,6 +44,7 @@ struct DevParams {
   double inv_ci, inv_L;      // 1 / c_i, 1 / L (the surrogate's per-level pass, flow_gpbld_fast)
   double QR_cold, QR_warm;   // Q / R of the Paterson-Budd cold and warm branches
   double exp_c[11];          // 1/12!, ..., 1/2! for exp_fast (ExpCoef)
+  double log_c[10];          // 2/19, 2/17, ..., 2/3, 2 for log_fast (LogCoef)
   // strain heating (PISMStressBalance.cc:384-388; flowlaws.cc:76, 249-253)
   double hard_pow;           // -1 / n
   double e_pow;              // e^(-1/n)
@@ -335,6 +336,39 @@ __host__ __device__ inline ExpCoef exp_coef() {
   return C;
 }
 
+// ln x for finite x > 0 (the strain heating's ln D^2, the temperate softness's
+// ln(1 + c omega)): x = 2^e m with m in [sqrt(1/2), sqrt(2)), so that
+// ln x = e ln 2 + 2 atanh(s), s = (m - 1) / (m + 1), |s| <= 0.1716; the
+// series 2 atanh(s) = sum_k 2 s^(2k+1) / (2k+1) to k = 9 leaves a truncation
+// below 3e-17 relative; e ln 2 in two parts as in exp_fast. Within a few ulp of
+// ln x. The coefficients 2 / (2k + 1) come in scalar registers, as exp_fast's.
+struct LogCoef {
+  double c[10];  // 2/19, 2/17, ..., 2/3, 2
+};
+
+__device__ __forceinline__ double log_fast(double x, const LogCoef &C) {
+  int e;
+  double m = frexp(x, &e);  // [0.5, 1)
+  if (m < 0.70710678118654752440) {
+    m += m;
+    e -= 1;
+  }
+  const double f = m - 1.0;  // exact
+  const double s = f * rcp_nr(2.0 + f);
+  const double s2 = s * s;
+  double q = C.c[0];
+#pragma unroll
+  for (int i = 1; i < 10; i++) q = fma(q, s2, C.c[i]);
+  const double de = (double)e;
+  return fma(de, 6.93147180369123816490e-01, fma(de, 1.90821492927058770002e-10, s * q));
+}
+
+__host__ __device__ inline LogCoef log_coef() {
+  LogCoef C;
+  for (int i = 0; i < 10; i++) C.c[i] = 2.0 / (double)(2 * (9 - i) + 1);
+  return C;
+}
+
 // GPBLD's flow (flowlaws.cc:209-221 through FlowLaw::flow, 98-101) with the
 // default enthalpy converter (enthalpyConverter.cc:92-234) and n - 1 = NM1 in
 // {1, 2, 3}, branch-free: the cold and the temperate softness are both
@@ -370,7 +404,7 @@ __device__ __forceinline__ double log_softness_gpbld_fast(const DevParams &P, do
   }
   double omega = (E - E_s) * P.inv_L;
   if (P.wf_limit < omega) omega = P.wf_limit;
-  return P.ln_A_T0 + log(1.0 + P.wf_coeff * omega);
+  return P.ln_A_T0 + log_fast(1.0 + P.wf_coeff * omega, *(const LogCoef *)P.log_c);
 }
 
 // SIAFD::grainSizeVostok (SIAFD.cc:980-1018)

@@ -770,13 +770,30 @@ __device__ __forceinline__ void lds_barrier() {
 
 namespace vtile {
 constexpr int TI = 4, TJ = 4, KC = 64;
-constexpr int RJ = TJ + 2, NR = (TI + 2) * RJ;                       // u, v columns: tile + ring
+// u, v columns: the tile and the ring around it without its corners, which
+// no difference reads: row a = -1 (TJ), rows 0 .. TI-1 (TJ + 2 each), row TI (TJ)
+constexpr int RJ = TJ + 2, NR = 2 * TJ + TI * RJ;
 constexpr int FJ0 = TJ + 2, NF0 = (TI + 3) * FJ0;                    // I0 faces (a, b): a in [-2, TI], b in [-1, TJ]
 constexpr int FJ1 = TJ + 3, NF1 = (TI + 2) * FJ1;                    // I1 faces (a, b): a in [-1, TI], b in [-2, TJ]
 constexpr int NF = NF0 + NF1, NC = TI * TJ, NS = KC + 2;
 __device__ __forceinline__ int f0(int a, int b) { return (a + 2) * FJ0 + (b + 1); }
 __device__ __forceinline__ int f1(int a, int b) { return NF0 + (a + 1) * FJ1 + (b + 2); }
-__device__ __forceinline__ int rr(int a, int b) { return (a + 1) * RJ + (b + 1); }
+__device__ __forceinline__ int rr(int a, int b) {
+  return (a < 0) ? b : ((a < TI) ? TJ + a * RJ + (b + 1) : TJ + TI * RJ + b);
+}
+// (a, b) of ring slot r
+__device__ __forceinline__ void rr_inv(int r, int &a, int &b) {
+  if (r < TJ) {
+    a = -1;
+    b = r;
+  } else if (r < TJ + TI * RJ) {
+    a = (r - TJ) / RJ;
+    b = (r - TJ) - a * RJ - 1;
+  } else {
+    a = TI;
+    b = r - TJ - TI * RJ;
+  }
+}
 __device__ __forceinline__ int wrapm(int i, int M) { const int r = i % M; return r < 0 ? r + M : r; }
 }  // namespace vtile
 
@@ -817,28 +834,25 @@ __device__ __forceinline__ double log_softness_sel(const DevParams &P, double E,
 }
 
 template <int LAW, bool FASTLN>
-__global__ __launch_bounds__(256) void k_velocity_tile(DevParams P, VelArgs A, TileMap tm) {
+__global__ __launch_bounds__(256, 4) void k_velocity_tile(DevParams P, VelArgs A, TileMap tm) {
   using namespace vtile;
   constexpr bool HEAT = LAW >= 0;
-  extern __shared__ double zs[];  // z [Mz], then (HEAT) 1 / (z_{k+1} - z_{k-1}) [Mz] (k = 0: 1 / (z_1 - z_0))
-  __shared__ const double *fcol[NF];
+  // 37.3 KB of LDS, so that 4 blocks (16 wavefronts) fit on a CU; the level
+  // coordinates come from global memory (z and A.idz, a few KB, cache-resident)
+  __shared__ int fcq[NF];  // map column of the face's I column
   __shared__ int fks[NF];
   __shared__ double fhx[NF], fhy[NF];
   __shared__ double rub[NR], rvb[NR];
   __shared__ double sU[NR][NS], sV[NR][NS];
-  __shared__ double cDx[NC], cDy[NC], cwe[NC][4], cw0[NC], cH[NC], cS[NC], cI[NC];
-  __shared__ int cks[NC], cq[NC];
+  __shared__ double cDx[NC], cDy[NC], cw0[NC], cH[NC], cS[NC], cI[NC];
+  __shared__ int cks[NC], cq[NC], cfac[NC];  // cfac: bit 0 west, 1 east, 2 south, 3 north
   __shared__ int s_kend;
   const int tid = threadIdx.x;
   const int Mx = P.Mx, My = P.My, Mz = P.Mz;
-  double *idz = zs + Mz;
+  const double *zg = A.z;
   int ti, tj;
   if (!tile_of_block(tm, blockIdx.x, ti, tj)) return;
   const int i0 = ti * TI, j0 = tj * TJ;
-  for (int k = tid; k < Mz; k += blockDim.x) {
-    zs[k] = A.z[k];
-    if (HEAT) idz[k] = (k == 0) ? 1.0 / (A.z[1] - A.z[0]) : ((k + 1 < Mz) ? 1.0 / (A.z[k + 1] - A.z[k - 1]) : 0.0);
-  }
   if (tid == 0) s_kend = 0;
   __syncthreads();
   int kend = 0;
@@ -849,7 +863,7 @@ __global__ __launch_bounds__(256) void k_velocity_tile(DevParams P, VelArgs A, T
     else { a = (f - NF0) / FJ1 - 1; b = (f - NF0) % FJ1 - 2; o = 1; }
     const int gi = wrapm(i0 + a, Mx), gj = wrapm(j0 + b, My);
     const int sid = 2 * (gi * My + gj) + o;
-    fcol[f] = (o ? A.I1 : A.I0) + (long)(gi * My + gj) * Mz;
+    fcq[f] = gi * My + gj;
     const int k = A.ks[sid];
     fks[f] = k;
     fhx[f] = A.hx[sid];
@@ -858,7 +872,8 @@ __global__ __launch_bounds__(256) void k_velocity_tile(DevParams P, VelArgs A, T
   }
   // ring columns: basal velocity
   for (int r = tid; r < NR; r += blockDim.x) {
-    const int a = r / RJ - 1, b = r % RJ - 1;
+    int a, b;
+    rr_inv(r, a, b);
     const long q = (long)wrapm(i0 + a, Mx) * My + wrapm(j0 + b, My);
     rub[r] = A.ub ? A.ub[q] : 0.0;
     rvb[r] = A.vb ? A.vb[q] : 0.0;
@@ -882,15 +897,12 @@ __global__ __launch_bounds__(256) void k_velocity_tile(DevParams P, VelArgs A, T
       if ((m_icy(m) && m_ice_free(ms)) || (m_ice_free(m) && m_icy(ms))) south = 0;
       cDx[tid] = (east + west > 0) ? 1.0 / (P.dx * (east + west)) : 0.0;
       cDy[tid] = (north + south > 0) ? 1.0 / (P.dy * (north + south)) : 0.0;
-      cwe[tid][0] = west;
-      cwe[tid][1] = east;
-      cwe[tid][2] = south;
-      cwe[tid][3] = north;
+      cfac[tid] = (west != 0 ? 1 : 0) | (east != 0 ? 2 : 0) | (south != 0 ? 4 : 0) | (north != 0 ? 8 : 0);
       cw0[tid] = A.bmr ? -A.bmr[q] : 0.0;
       if (HEAT) {
         const double Hq = A.H[q];
         cH[tid] = Hq;
-        ksH = kbelow(zs, Mz, P.Lz, Hq);  // throws above Lz (IceGrid.cc:381-389)
+        ksH = kbelow(zg, Mz, P.Lz, Hq);  // throws above Lz (IceGrid.cc:381-389)
         if (ksH < 0) atomicOr(A.err, ERR_THK_LZ);
         kend = max(kend, ksH);
       }
@@ -920,6 +932,11 @@ __global__ __launch_bounds__(256) void k_velocity_tile(DevParams P, VelArgs A, T
         Ec[t] = (HEAT && cq[c] >= 0 && k <= cks[c]) ? glb(A.E)[(long)cq[c] * Mz + k] : 0.0;
       }
     }
+    // level coordinates of this chunk's lanes: z_k, z_{k-1} and the centred
+    // difference's 1 / (z_{k+1} - z_{k-1}) (1 / (z_1 - z_0) at k = 0)
+    const int kl = min(k0 + (tid & 63), Mz - 1);
+    const double z_k = zg[kl], z_km1 = zg[max(kl - 1, 0)];
+    const double idz_k = HEAT ? A.idz[kl] : 0.0;
     // phase B: u, v at levels k0 .. k0+KC-1 (slots 1 .. KC), one wavefront
     // per ring column, one lane per level; then level k0+KC (slot KC+1) of
     // every ring column. Slot 0 (level k0-1) came from the last chunk.
@@ -927,14 +944,17 @@ __global__ __launch_bounds__(256) void k_velocity_tile(DevParams P, VelArgs A, T
       const int lane = tid & 63;
       auto uv_at = [&](int r, int sl) {
         const int kk = min(k0 - 1 + sl, Mz - 1);
-        const int a = r / RJ - 1, b = r - (a + 1) * RJ - 1;
+        int a, b;
+        rr_inv(r, a, b);
         const int fe = f0(a, b), fw = f0(a - 1, b), fn = f1(a, b), fs = f1(a, b - 1);
-        const double Ie = glb(fcol[fe])[min(kk, fks[fe])], Iw = glb(fcol[fw])[min(kk, fks[fw])];
-        const double In = glb(fcol[fn])[min(kk, fks[fn])], Is = glb(fcol[fs])[min(kk, fks[fs])];
+        const double Ie = A.I0[(long)fcq[fe] * Mz + min(kk, fks[fe])];
+        const double Iw = A.I0[(long)fcq[fw] * Mz + min(kk, fks[fw])];
+        const double In = A.I1[(long)fcq[fn] * Mz + min(kk, fks[fn])];
+        const double Is = A.I1[(long)fcq[fs] * Mz + min(kk, fks[fs])];
         sU[r][sl] = uv_formula(Ie, fhx[fe], Iw, fhx[fw], In, fhx[fn], Is, fhx[fs], rub[r]);
         sV[r][sl] = uv_formula(Ie, fhy[fe], Iw, fhy[fw], In, fhy[fn], Is, fhy[fs], rvb[r]);
       };
-#pragma unroll 3
+#pragma unroll 4
       for (int r = tid >> 6; r < NR; r += WAVES_PER_BLOCK) uv_at(r, lane + 1);
       if (tid < NR) uv_at(tid, KC + 1);
     }
@@ -951,30 +971,33 @@ __global__ __launch_bounds__(256) void k_velocity_tile(DevParams P, VelArgs A, T
       const int k = k0 + lane, sl = lane + 1;
       const bool act = q >= 0 && k < Mz;
       const double uc = sU[r][sl], vc = sV[r][sl];
-      const double uw = sU[r - RJ][sl], ue = sU[r + RJ][sl], vs = sV[r - 1][sl], vn = sV[r + 1][sl];
+      const int rw = rr(a - 1, b), re = rr(a + 1, b), rs = rr(a, b - 1), rn = rr(a, b + 1);
+      const double uw = sU[rw][sl], ue = sU[re][sl], vs = sV[rs][sl], vn = sV[rn][sl];
       const double Dx = cDx[c], Dy = cDy[c];
-      const double west = cwe[c][0], east = cwe[c][1], south = cwe[c][2], north = cwe[c][3];
+      const int fac = cfac[c];
+      const double west = (fac & 1) ? 1.0 : 0.0, east = (fac & 2) ? 1.0 : 0.0;
+      const double south = (fac & 4) ? 1.0 : 0.0, north = (fac & 8) ? 1.0 : 0.0;
       const double u_x = Dx * (west * (uc - uw) + east * (ue - uc));
       const double v_y = Dy * (south * (vc - vs) + north * (vn - vc));
       double integrand = act ? u_x + v_y : 0.0;
       double iprev = wave_shift_up1(integrand);
       if (lane == 0) iprev = cI[c];
       double dz = 0.0;
-      if (act && k > 0) dz = zs[k] - zs[k - 1];
+      if (act && k > 0) dz = z_k - z_km1;
       const double t = (act && k > 0) ? 0.5 * (integrand + iprev) * dz : 0.0;  // PISMStressBalance.cc:280-282
       const double S = cS[c] + wave_inclusive_scan(t);
       double sig = 0.0;
       if (HEAT && act && k <= cks[c]) {
-        const double us = sU[r - 1][sl], un = sU[r + 1][sl], vw = sV[r - RJ][sl], ve = sV[r + RJ][sl];
-        const double pressure = ec_pressure(P, cH[c] - zs[k]);
+        const double us = sU[rs][sl], un = sU[rn][sl], vw = sV[rw][sl], ve = sV[re][sl];
+        const double pressure = ec_pressure(P, cH[c] - z_k);
         const double lnA = log_softness_sel<LAW, FASTLN>(P, Ec[tc], pressure);
         const double hu_x = u_x, hu_y = Dy * (south * (uc - us) + north * (un - uc)),
                      hv_x = Dx * (west * (vc - vw) + east * (ve - vc)), hv_y = v_y;
         // one-sided at the base (PISMStressBalance.cc:464-476)
-        const double u_z = (k > 0) ? (sU[r][sl + 1] - sU[r][sl - 1]) * idz[k] : (sU[r][sl + 1] - uc) * idz[0];
-        const double v_z = (k > 0) ? (sV[r][sl + 1] - sV[r][sl - 1]) * idz[k] : (sV[r][sl + 1] - vc) * idz[0];
+        const double u_z = (k > 0) ? (sU[r][sl + 1] - sU[r][sl - 1]) * idz_k : (sU[r][sl + 1] - uc) * idz_k;
+        const double v_z = (k > 0) ? (sV[r][sl + 1] - sV[r][sl - 1]) * idz_k : (sV[r][sl + 1] - vc) * idz_k;
         const double d2 = strain_D2(hu_x, hu_y, u_z, hv_x, hv_y, v_z);
-        sig = (d2 > 0.0) ? 2.0 * P.e_pow * exp_fast(P.hard_pow * lnA + P.heat_exp * log(d2), *(const ExpCoef *)P.exp_c) : 0.0;
+        sig = (d2 > 0.0) ? 2.0 * P.e_pow * exp_fast(P.hard_pow * lnA + P.heat_exp * log_fast(d2, *(const LogCoef *)P.log_c), *(const ExpCoef *)P.exp_c) : 0.0;
       }
       if (act) {
         const long o = (long)q * Mz + k;
@@ -999,7 +1022,7 @@ __global__ __launch_bounds__(256) void k_velocity_tile(DevParams P, VelArgs A, T
   }
   // levels k0 .. Mz-1: constant u, v, integrand, Sigma = 0; w continues the trapezoid
   if (k0 < Mz) {
-    const double ztop = zs[k0 - 1];
+    const double ztop = zg[k0 - 1];
 #pragma unroll 1
     for (int c = 0; c < NC; c++) {
       const int q = cq[c];
@@ -1012,7 +1035,7 @@ __global__ __launch_bounds__(256) void k_velocity_tile(DevParams P, VelArgs A, T
       for (int k = k0 + tid; k < Mz; k += blockDim.x) {
         A.u[base + k] = ut;
         A.v[base + k] = vt;
-        A.w[base + k] = w0 - (St + it_ * (zs[k] - ztop));
+        A.w[base + k] = w0 - (St + it_ * (zg[k] - ztop));
         if (HEAT) A.Sigma[base + k] = 0.0;
       }
     }
@@ -1484,11 +1507,10 @@ void launch_theta_thk(const DevParams &P, const double *H, const double *b, cons
 template <int LAW>
 static void launch_vel_law(const DevParams &P, const VelArgs &A, hipStream_t st) {
   const TileMap tm = tile_map(P.Mx, P.My);
-  const size_t lds = (LAW >= 0 ? 2 : 1) * sizeof(double) * (size_t)P.Mz;
   if (LAW == LAW_GPBLD && P.ec == EC_DEFAULT) {
-    hipLaunchKernelGGL((k_velocity_tile<LAW, true>), dim3(tm.per_xcd * N_XCD), dim3(256), lds, st, P, A, tm);
+    hipLaunchKernelGGL((k_velocity_tile<LAW, true>), dim3(tm.per_xcd * N_XCD), dim3(256), 0, st, P, A, tm);
   } else {
-    hipLaunchKernelGGL((k_velocity_tile<LAW, false>), dim3(tm.per_xcd * N_XCD), dim3(256), lds, st, P, A, tm);
+    hipLaunchKernelGGL((k_velocity_tile<LAW, false>), dim3(tm.per_xcd * N_XCD), dim3(256), 0, st, P, A, tm);
   }
 }
 

@@ -38,6 +38,9 @@ struct VelArgs {
   const double *H, *E;
   double *Sigma;
   int *err;
+  // [Mz] 1 / (z_{k+1} - z_{k-1}), 1 / (z_1 - z_0) at k = 0, 0 at k = Mz - 1 (the
+  // centred vertical differences of the strain heating)
+  const double *idz;
 };
 
 // SIA_Sliding (SIA_Sliding.cc:106-267): mode 0 pressure-melting-activated
