@@ -919,6 +919,72 @@ __global__ __launch_bounds__(256, 4) void k_velocity_tile(DevParams P, VelArgs A
   kend = min(s_kend, Mz - 1);
 
   int k0 = 0;
+  // u and v of ring slot r at level k0 - 1 + sl (slot sl) from the I of its four faces
+  auto uv_at = [&](int r, int sl) {
+    const int kk = min(k0 - 1 + sl, Mz - 1);
+    int a, b;
+    rr_inv(r, a, b);
+    const int fe = f0(a, b), fw = f0(a - 1, b), fn = f1(a, b), fs = f1(a, b - 1);
+    const double Ie = A.I0[(long)fcq[fe] * Mz + min(kk, fks[fe])];
+    const double Iw = A.I0[(long)fcq[fw] * Mz + min(kk, fks[fw])];
+    const double In = A.I1[(long)fcq[fn] * Mz + min(kk, fks[fn])];
+    const double Is = A.I1[(long)fcq[fs] * Mz + min(kk, fks[fs])];
+    sU[r][sl] = uv_formula(Ie, fhx[fe], Iw, fhx[fw], In, fhx[fn], Is, fhx[fs], rub[r]);
+    sV[r][sl] = uv_formula(Ie, fhy[fe], Iw, fhy[fw], In, fhy[fn], Is, fhy[fs], rvb[r]);
+  };
+  if (kend == 0) {
+    // Every face of the tile has kBelowHeight 0, so its I is 0 at every level,
+    // and every column's H lies below the second level: u and v of every column
+    // are their base values at every level, the integrand is constant (w is
+    // its trapezoid, w_0 - int z_k) and the strain heating is 0 above the
+    // base. Each tile row (TJ map columns, consecutive in memory) is written
+    // as one contiguous run.
+    if (tid < NR) uv_at(tid, 1);
+    lds_barrier();
+    if (tid < NC && cq[tid] >= 0) {
+      const int c = tid, a = c / TJ, b = c - a * TJ;
+      const int r = rr(a, b), rw = rr(a - 1, b), re = rr(a + 1, b), rs = rr(a, b - 1), rn = rr(a, b + 1);
+      const double uc = sU[r][1], vc = sV[r][1];
+      const double Dx = cDx[c], Dy = cDy[c];
+      const int fac = cfac[c];
+      const double west = (fac & 1) ? 1.0 : 0.0, east = (fac & 2) ? 1.0 : 0.0;
+      const double south = (fac & 4) ? 1.0 : 0.0, north = (fac & 8) ? 1.0 : 0.0;
+      const double u_x = Dx * (west * (uc - sU[rw][1]) + east * (sU[re][1] - uc));
+      const double v_y = Dy * (south * (vc - sV[rs][1]) + north * (sV[rn][1] - vc));
+      double sig = 0.0;
+      if (HEAT && cks[c] >= 0) {
+        const double pressure = ec_pressure(P, cH[c] - zg[0]);
+        const double lnA = log_softness_sel<LAW, FASTLN>(P, glb(A.E)[(long)cq[c] * Mz], pressure);
+        const double hu_y = Dy * (south * (uc - sU[rs][1]) + north * (sU[rn][1] - uc));
+        const double hv_x = Dx * (west * (vc - sV[rw][1]) + east * (sV[re][1] - vc));
+        const double d2 = strain_D2(u_x, hu_y, 0.0, hv_x, v_y, 0.0);  // u, v constant in z: u_z = v_z = 0
+        sig = (d2 > 0.0) ? 2.0 * P.e_pow * exp_fast(P.hard_pow * lnA + P.heat_exp * log_fast(d2, *(const LogCoef *)P.log_c), *(const ExpCoef *)P.exp_c) : 0.0;
+      }
+      cI[c] = u_x + v_y;
+      cS[c] = sig;  // the strain heating at the base
+    }
+    lds_barrier();
+    const float inv_Mz = 1.0f / (float)Mz;
+#pragma unroll 1
+    for (int a = 0; a < TI; a++) {
+      if (i0 + a >= Mx) break;
+      const int n = min(TJ, My - j0) * Mz;
+      const long base = ((long)(i0 + a) * My + j0) * Mz;
+      // lanes on 128-byte lines: a wavefront's store covers 4 whole lines
+      const int sh = (int)(base & 15);
+      for (int e = tid - sh; e < n; e += blockDim.x) {
+        if (e < 0) continue;
+        int b = (int)((float)e * inv_Mz), k = e - b * Mz;  // e = b Mz + k
+        if (k < 0) { b--; k += Mz; } else if (k >= Mz) { b++; k -= Mz; }
+        const int c = a * TJ + b, r = rr(a, b);
+        A.u[base + e] = sU[r][1];
+        A.v[base + e] = sV[r][1];
+        A.w[base + e] = cw0[c] - cI[c] * zg[k];
+        if (HEAT) A.Sigma[base + e] = (k == 0) ? cS[c] : 0.0;
+      }
+    }
+    return;
+  }
   for (; k0 <= kend; k0 += KC) {
     // enthalpy of this chunk's levels of the wavefront's tile columns (phase
     // C), loaded now so that phase B hides its latency
@@ -942,18 +1008,6 @@ __global__ __launch_bounds__(256, 4) void k_velocity_tile(DevParams P, VelArgs A
     // every ring column. Slot 0 (level k0-1) came from the last chunk.
     {
       const int lane = tid & 63;
-      auto uv_at = [&](int r, int sl) {
-        const int kk = min(k0 - 1 + sl, Mz - 1);
-        int a, b;
-        rr_inv(r, a, b);
-        const int fe = f0(a, b), fw = f0(a - 1, b), fn = f1(a, b), fs = f1(a, b - 1);
-        const double Ie = A.I0[(long)fcq[fe] * Mz + min(kk, fks[fe])];
-        const double Iw = A.I0[(long)fcq[fw] * Mz + min(kk, fks[fw])];
-        const double In = A.I1[(long)fcq[fn] * Mz + min(kk, fks[fn])];
-        const double Is = A.I1[(long)fcq[fs] * Mz + min(kk, fks[fs])];
-        sU[r][sl] = uv_formula(Ie, fhx[fe], Iw, fhx[fw], In, fhx[fn], Is, fhx[fs], rub[r]);
-        sV[r][sl] = uv_formula(Ie, fhy[fe], Iw, fhy[fw], In, fhy[fn], Is, fhy[fs], rvb[r]);
-      };
 #pragma unroll 4
       for (int r = tid >> 6; r < NR; r += WAVES_PER_BLOCK) uv_at(r, lane + 1);
       if (tid < NR) uv_at(tid, KC + 1);
@@ -1032,7 +1086,10 @@ __global__ __launch_bounds__(256, 4) void k_velocity_tile(DevParams P, VelArgs A
       const double ut = sU[r][0], vt = sV[r][0];  // level k0 - 1 (moved to slot 0)
       const double it_ = cI[c], St = cS[c], w0 = cw0[c];
       const long base = (long)q * Mz;
-      for (int k = k0 + tid; k < Mz; k += blockDim.x) {
+      // lanes on 128-byte lines: a wavefront's store covers 4 whole lines
+      const int sh = (int)((base + k0) & 15);
+      for (int k = k0 + tid - sh; k < Mz; k += blockDim.x) {
+        if (k < k0) continue;
         A.u[base + k] = ut;
         A.v[base + k] = vt;
         A.w[base + k] = w0 - (St + it_ * (zg[k] - ztop));
