@@ -346,7 +346,8 @@ int update_impl(sia_handle *h, const sia_inputs *in, int fast, bool allow_sync) 
   const size_t n2 = (size_t)h->grid.Mx * h->grid.My, n3 = n2 * (size_t)h->grid.Mz;
   int rc;
   if (full) {
-    if ((rc = dalloc(h, &h->I0, n3)) || (rc = dalloc(h, &h->I1, n3)) || (rc = dalloc(h, &h->u, n3)) ||
+    const size_t nI = n2 * (size_t)i_stride(h->grid.Mz);
+    if ((rc = dalloc(h, &h->I0, nI)) || (rc = dalloc(h, &h->I1, nI)) || (rc = dalloc(h, &h->u, n3)) ||
         (rc = dalloc(h, &h->v, n3)) || (rc = dalloc(h, &h->w, n3)) || (rc = dalloc(h, &h->Sigma, n3)) ||
         (rc = dalloc(h, &h->ks, 2 * n2))) {
       return rc;

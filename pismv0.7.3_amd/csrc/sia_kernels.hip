@@ -512,7 +512,7 @@ __device__ __forceinline__ double column_block_impl(const DevParams &P, const Co
 
   auto Icol_of = [&](int c) -> double * {
     const int s_c = list ? lane_value(s, c) : s0 + c;
-    return ((s_c & 1) ? A.I1 : A.I0) + (long)(s_c >> 1) * Mz;
+    return ((s_c & 1) ? A.I1 : A.I0) + (long)(s_c >> 1) * i_stride(Mz);
   };
   // first column >= c with a level to evaluate; the columns passed over get
   // I = 0 at every level (readers use I[ks] with ks stored as 0)
@@ -925,10 +925,11 @@ __global__ __launch_bounds__(256, 4) void k_velocity_tile(DevParams P, VelArgs A
     int a, b;
     rr_inv(r, a, b);
     const int fe = f0(a, b), fw = f0(a - 1, b), fn = f1(a, b), fs = f1(a, b - 1);
-    const double Ie = A.I0[(long)fcq[fe] * Mz + min(kk, fks[fe])];
-    const double Iw = A.I0[(long)fcq[fw] * Mz + min(kk, fks[fw])];
-    const double In = A.I1[(long)fcq[fn] * Mz + min(kk, fks[fn])];
-    const double Is = A.I1[(long)fcq[fs] * Mz + min(kk, fks[fs])];
+    const long Is_ = i_stride(Mz);
+    const double Ie = A.I0[(long)fcq[fe] * Is_ + min(kk, fks[fe])];
+    const double Iw = A.I0[(long)fcq[fw] * Is_ + min(kk, fks[fw])];
+    const double In = A.I1[(long)fcq[fn] * Is_ + min(kk, fks[fn])];
+    const double Is = A.I1[(long)fcq[fs] * Is_ + min(kk, fks[fs])];
     sU[r][sl] = uv_formula(Ie, fhx[fe], Iw, fhx[fw], In, fhx[fn], Is, fhx[fs], rub[r]);
     sV[r][sl] = uv_formula(Ie, fhy[fe], Iw, fhy[fw], In, fhy[fn], Is, fhy[fs], rvb[r]);
   };

@@ -29,6 +29,11 @@ struct ColumnArgs {
 // value of ColumnArgs::only that selects a staggered column (the surrogate's fallback status)
 constexpr unsigned char ONLY_MARK = 2;
 
+// levels allocated per staggered column of I (the column kernel's output on
+// full updates, read by the velocity tiles): Mz rounded up to 16, so that every
+// column starts on a 128-byte line and a wavefront's 64 levels cover 4 lines
+__host__ __device__ inline int i_stride(int Mz) { return (Mz + 15) & ~15; }
+
 struct VelArgs {
   const double *z, *mask, *I0, *I1, *hx, *hy, *ub, *vb, *bmr;
   const int *ks;
