@@ -837,8 +837,12 @@ template <int LAW, bool FASTLN>
 __global__ __launch_bounds__(256, 4) void k_velocity_tile(DevParams P, VelArgs A, TileMap tm) {
   using namespace vtile;
   constexpr bool HEAT = LAW >= 0;
-  // 37.3 KB of LDS, so that 4 blocks (16 wavefronts) fit on a CU; the level
-  // coordinates come from global memory (z and A.idz, a few KB, cache-resident)
+  // 37.3 KB of static LDS and z (8 Mz bytes): up to Mz = 459, 4 blocks (16
+  // wavefronts) fit on a CU. z is read from LDS so that the store loops (the
+  // fill, the rows of tiles below the second level) issue no vector-memory
+  // load: one would wait for every store before it on the in-order
+  // vector-memory counter.
+  extern __shared__ double zs[];
   __shared__ int fcq[NF];  // map column of the face's I column
   __shared__ int fks[NF];
   __shared__ double fhx[NF], fhy[NF];
@@ -849,10 +853,11 @@ __global__ __launch_bounds__(256, 4) void k_velocity_tile(DevParams P, VelArgs A
   __shared__ int s_kend;
   const int tid = threadIdx.x;
   const int Mx = P.Mx, My = P.My, Mz = P.Mz;
-  const double *zg = A.z;
+  const double *zg = zs;
   int ti, tj;
   if (!tile_of_block(tm, blockIdx.x, ti, tj)) return;
   const int i0 = ti * TI, j0 = tj * TJ;
+  for (int k = tid; k < Mz; k += blockDim.x) zs[k] = A.z[k];
   if (tid == 0) s_kend = 0;
   __syncthreads();
   int kend = 0;
@@ -1565,10 +1570,11 @@ void launch_theta_thk(const DevParams &P, const double *H, const double *b, cons
 template <int LAW>
 static void launch_vel_law(const DevParams &P, const VelArgs &A, hipStream_t st) {
   const TileMap tm = tile_map(P.Mx, P.My);
+  const size_t lds = sizeof(double) * (size_t)P.Mz;
   if (LAW == LAW_GPBLD && P.ec == EC_DEFAULT) {
-    hipLaunchKernelGGL((k_velocity_tile<LAW, true>), dim3(tm.per_xcd * N_XCD), dim3(256), 0, st, P, A, tm);
+    hipLaunchKernelGGL((k_velocity_tile<LAW, true>), dim3(tm.per_xcd * N_XCD), dim3(256), lds, st, P, A, tm);
   } else {
-    hipLaunchKernelGGL((k_velocity_tile<LAW, false>), dim3(tm.per_xcd * N_XCD), dim3(256), 0, st, P, A, tm);
+    hipLaunchKernelGGL((k_velocity_tile<LAW, false>), dim3(tm.per_xcd * N_XCD), dim3(256), lds, st, P, A, tm);
   }
 }
 
