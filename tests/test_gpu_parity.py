@@ -161,6 +161,33 @@ def test_glen_exponent_4_and_sliding_and_bmr(S, oracle):
     check_against_oracle(g, o, w.grid, ub=ub, vb=vb, bmr=bmr, label="n4")
 
 
+def test_margin_tiles_with_sliding_and_bmr(S, oracle):
+    """A 50x51 window across the ice margin (not a multiple of the velocity
+    pass's 4x4 tiles in either direction) with a sliding velocity and a basal
+    melt rate. The tiles whose faces and columns all lie below the second level
+    are written as rows (u = u_b and v = v_b at every level, w linear in z, the
+    strain heating at the base only); the others take the level chunks."""
+    say("[gpu] margin tiles with sliding and basal melt")
+    w = W.greenland_like(140, 190, 1350, 1401, Mz=101, Lz=4000.0)
+    Mx, My = w.grid.Mx, w.grid.My
+    H = w.fields["H"].reshape(Mx, My)
+    tiles_dry = [H[i:i + 4, j:j + 4].max() == 0.0 for i in range(0, Mx - 8, 4) for j in range(0, My - 8, 4)]
+    assert any(tiles_dry) and not all(tiles_dry)
+    n2 = Mx * My
+    rng = np.random.default_rng(11)
+    ub = rng.uniform(-1e-6, 1e-6, n2)
+    vb = rng.uniform(-1e-6, 1e-6, n2)
+    bmr = rng.uniform(0, 3e-9, n2)
+    g = run_gpu(w, ub=ub, vb=vb, bmr=bmr)
+    o = run_oracle(oracle, w, ub=ub, vb=vb, bmr=bmr)
+    errs = check_against_oracle(g, o, w.grid, ub=ub, vb=vb, bmr=bmr, label="margin")
+    say("  max rel errors:", {k: "%.1e" % v for k, v in errs.items()})
+    # ice-free columns slide at u_b, v_b at every level (bitwise: I = 0 there)
+    dry = (H.ravel() == 0.0)
+    u = g["u"].reshape(n2, -1)
+    assert np.any(u[dry] != 0.0)
+
+
 def test_grain_size_from_age(S, oracle):
     """Goldsby-Kohlstedt with grain size from age (SIAFD.cc:575-577, 666-669)."""
     say("[gpu] GK with age-dependent grain size")
