@@ -514,14 +514,15 @@ __device__ __forceinline__ double column_block_impl(const DevParams &P, const Co
     const int s_c = list ? lane_value(s, c) : s0 + c;
     return ((s_c & 1) ? A.I1 : A.I0) + (long)(s_c >> 1) * i_stride(Mz);
   };
-  // first column >= c with a level to evaluate; the columns passed over get
-  // I = 0 at every level (readers use I[ks] with ks stored as 0)
+  // first column >= c with a level to evaluate. I is 0 at level 0 of every
+  // column (the integral from the base), and the columns passed over have ks
+  // stored as 0: readers take I = 0 at index 0 and never read it, so it is
+  // not written.
   auto next_valid = [&](int c) -> int {
     for (; c < COLS_PER_WAVE; c++) {
       const int kk = lane_value(c_ks, c);
       if (kk == -3) return COLS_PER_WAVE;
       if (kk >= 0) return c;
-      if (FULL && kk != -4 && lane == 0) Icol_of(c)[0] = 0.0;
     }
     return COLS_PER_WAVE;
   };
@@ -618,7 +619,7 @@ __device__ __forceinline__ double column_block_impl(const DevParams &P, const Co
     if (FULL) {
       const double t = (act && k > 0) ? 0.5 * dz * (dprev + delta) : 0.0;  // SIAFD.cc:863-867
       const double Ik = Icarry + wave_inclusive_scan(t);
-      if (act) Icol_of(c)[k] = Ik;
+      if (act && k > 0) Icol_of(c)[k] = Ik;
       Icarry = lane_value(Ik, 63);
     }
     dprev_carry = lane_value(delta, 63);
@@ -930,11 +931,13 @@ __global__ __launch_bounds__(256, 4) void k_velocity_tile(DevParams P, VelArgs A
     int a, b;
     rr_inv(r, a, b);
     const int fe = f0(a, b), fw = f0(a - 1, b), fn = f1(a, b), fs = f1(a, b - 1);
+    // I is 0 at level 0 (and never stored there)
     const long Is_ = i_stride(Mz);
-    const double Ie = A.I0[(long)fcq[fe] * Is_ + min(kk, fks[fe])];
-    const double Iw = A.I0[(long)fcq[fw] * Is_ + min(kk, fks[fw])];
-    const double In = A.I1[(long)fcq[fn] * Is_ + min(kk, fks[fn])];
-    const double Is = A.I1[(long)fcq[fs] * Is_ + min(kk, fks[fs])];
+    const int le = min(kk, fks[fe]), lw = min(kk, fks[fw]), ln = min(kk, fks[fn]), ls = min(kk, fks[fs]);
+    const double Ie = le > 0 ? A.I0[(long)fcq[fe] * Is_ + le] : 0.0;
+    const double Iw = lw > 0 ? A.I0[(long)fcq[fw] * Is_ + lw] : 0.0;
+    const double In = ln > 0 ? A.I1[(long)fcq[fn] * Is_ + ln] : 0.0;
+    const double Is = ls > 0 ? A.I1[(long)fcq[fs] * Is_ + ls] : 0.0;
     sU[r][sl] = uv_formula(Ie, fhx[fe], Iw, fhx[fw], In, fhx[fn], Is, fhx[fs], rub[r]);
     sV[r][sl] = uv_formula(Ie, fhy[fe], Iw, fhy[fw], In, fhy[fn], Is, fhy[fs], rvb[r]);
   };
