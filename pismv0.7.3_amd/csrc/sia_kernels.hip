@@ -682,7 +682,8 @@ __device__ __forceinline__ double column_block(const DevParams &P, const ColumnA
   return column_block_impl<LAW, FULL, CPW, 0>(P, A, bid, zs, wmax, list, n_list);
 }
 
-constexpr int COLS_PER_WAVE = 16;
+// 32 measured fastest on the 1501x2801x401 sheet (8, 16, 48 and 64 slower)
+constexpr int COLS_PER_WAVE = 32;
 constexpr int COLS_PER_BLOCK = WAVES_PER_BLOCK * COLS_PER_WAVE;
 // the FP64 pass over a list (the surrogate's fallback columns, a small share of
 // deep columns) takes 4 columns per wavefront, so that more wavefronts share it
