@@ -778,6 +778,7 @@ constexpr int RJ = TJ + 2, NR = 2 * TJ + TI * RJ;
 constexpr int FJ0 = TJ + 2, NF0 = (TI + 3) * FJ0;                    // I0 faces (a, b): a in [-2, TI], b in [-1, TJ]
 constexpr int FJ1 = TJ + 3, NF1 = (TI + 2) * FJ1;                    // I1 faces (a, b): a in [-1, TI], b in [-2, TJ]
 constexpr int NF = NF0 + NF1, NC = TI * TJ, NS = KC + 2;
+constexpr int VW = NC / 4;  // wavefronts of a tile block: 4 tile columns each in phase C
 __device__ __forceinline__ int f0(int a, int b) { return (a + 2) * FJ0 + (b + 1); }
 __device__ __forceinline__ int f1(int a, int b) { return NF0 + (a + 1) * FJ1 + (b + 2); }
 __device__ __forceinline__ int rr(int a, int b) {
@@ -836,7 +837,7 @@ __device__ __forceinline__ double log_softness_sel(const DevParams &P, double E,
 }
 
 template <int LAW, bool FASTLN>
-__global__ __launch_bounds__(256, 4) void k_velocity_tile(DevParams P, VelArgs A, TileMap tm) {
+__global__ __launch_bounds__(64 * vtile::VW, 4) void k_velocity_tile(DevParams P, VelArgs A, TileMap tm) {
   using namespace vtile;
   constexpr bool HEAT = LAW >= 0;
   // 37.3 KB of static LDS and z (8 Mz bytes): up to Mz = 459, 4 blocks (16
@@ -998,12 +999,12 @@ __global__ __launch_bounds__(256, 4) void k_velocity_tile(DevParams P, VelArgs A
   for (; k0 <= kend; k0 += KC) {
     // enthalpy of this chunk's levels of the wavefront's tile columns (phase
     // C), loaded now so that phase B hides its latency
-    double Ec[NC / WAVES_PER_BLOCK];
+    double Ec[NC / VW];
     {
       const int lane = tid & 63;
 #pragma unroll
-      for (int t = 0; t < NC / WAVES_PER_BLOCK; t++) {
-        const int c = (tid >> 6) + t * WAVES_PER_BLOCK;
+      for (int t = 0; t < NC / VW; t++) {
+        const int c = (tid >> 6) + t * VW;
         const int k = k0 + lane;
         Ec[t] = (HEAT && cq[c] >= 0 && k <= cks[c]) ? glb(A.E)[(long)cq[c] * Mz + k] : 0.0;
       }
@@ -1019,15 +1020,15 @@ __global__ __launch_bounds__(256, 4) void k_velocity_tile(DevParams P, VelArgs A
     {
       const int lane = tid & 63;
 #pragma unroll 4
-      for (int r = tid >> 6; r < NR; r += WAVES_PER_BLOCK) uv_at(r, lane + 1);
+      for (int r = tid >> 6; r < NR; r += VW) uv_at(r, lane + 1);
       if (tid < NR) uv_at(tid, KC + 1);
     }
     lds_barrier();
     // phase C: integrand, w and the strain heating of levels k0 .. k0+KC-1;
     // one wavefront per tile column, one lane per level
 #pragma unroll
-    for (int tc = 0; tc < NC / WAVES_PER_BLOCK; tc++) {
-      const int c = (tid >> 6) + tc * WAVES_PER_BLOCK;
+    for (int tc = 0; tc < NC / VW; tc++) {
+      const int c = (tid >> 6) + tc * VW;
       const int lane = tid & 63;
       const int a = c / TJ, b = c - a * TJ;
       const int r = rr(a, b);
@@ -1576,9 +1577,11 @@ static void launch_vel_law(const DevParams &P, const VelArgs &A, hipStream_t st)
   const TileMap tm = tile_map(P.Mx, P.My);
   const size_t lds = sizeof(double) * (size_t)P.Mz;
   if (LAW == LAW_GPBLD && P.ec == EC_DEFAULT) {
-    hipLaunchKernelGGL((k_velocity_tile<LAW, true>), dim3(tm.per_xcd * N_XCD), dim3(256), lds, st, P, A, tm);
+    hipLaunchKernelGGL((k_velocity_tile<LAW, true>), dim3(tm.per_xcd * N_XCD), dim3(64 * vtile::VW), lds, st, P, A,
+                       tm);
   } else {
-    hipLaunchKernelGGL((k_velocity_tile<LAW, false>), dim3(tm.per_xcd * N_XCD), dim3(256), lds, st, P, A, tm);
+    hipLaunchKernelGGL((k_velocity_tile<LAW, false>), dim3(tm.per_xcd * N_XCD), dim3(64 * vtile::VW), lds, st, P, A,
+                       tm);
   }
 }
 
