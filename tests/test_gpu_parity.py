@@ -188,6 +188,18 @@ def test_margin_tiles_with_sliding_and_bmr(S, oracle):
     assert np.any(u[dry] != 0.0)
 
 
+def test_full_update_601_levels(S, oracle):
+    """A full update with 601 levels: the velocity pass keeps z in LDS (8 Mz
+    bytes beside its 37 KB of tile data), so above 459 levels it runs 3 tile
+    blocks per CU instead of 4; the results must not depend on that."""
+    say("[gpu] full update with 601 levels")
+    w = W.greenland_like(140, 164, 1350, 1375, Mz=601, Lz=4000.0)
+    g = run_gpu(w)
+    o = run_oracle(oracle, w)
+    errs = check_against_oracle(g, o, w.grid, label="Mz601")
+    say("  max rel errors:", {k: "%.1e" % v for k, v in errs.items()})
+
+
 def test_grain_size_from_age(S, oracle):
     """Goldsby-Kohlstedt with grain size from age (SIAFD.cc:575-577, 666-669)."""
     say("[gpu] GK with age-dependent grain size")
