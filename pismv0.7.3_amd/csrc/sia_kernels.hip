@@ -748,20 +748,23 @@ __device__ __forceinline__ double strain_D2(double u_x, double u_y, double u_z, 
 // for LAW >= 0, the volumetric strain heating (PISMStressBalance.cc:370-500),
 // computed by tiles of TI x TJ map columns:
 // per chunk of KC levels, u and v of the tile and of a ring of one column
-// around it go to LDS once (each from the four I columns of its faces, with
-// the reference's formula, so a ring value is bitwise the one its own tile
-// stores); the w integrand, w (a prefix scan over the wavefront's lanes, one
-// level each) and the strain heating read them there.
+// around it (without the ring's corners, which no difference reads) go to LDS
+// once (each from the four I columns of its faces, with the reference's
+// formula, so a ring value is bitwise the one its own tile stores); the w
+// integrand, w (a prefix scan over the wavefront's lanes, one level each) and
+// the strain heating read them there.
 // A chunk holds the levels k0-1 .. k0+KC of u and v (u_z at k0 and k0+KC-1);
 // its slot KC (level k0+KC-1) becomes the next chunk's slot 0. Sigma =
 // 2 e^(-1/n) B D^(1+1/n) is evaluated as 2 e^(-1/n) exp(-ln A / n + (1/n + 1)/2
 // ln D^2) with ln A from the flow law, 0 above kBelowHeight(H). Above the tile's
 // largest kBelowHeight (of its faces and of H) u, v and the integrand are
 // constant and the strain heating is 0, so those levels are written directly,
-// w by the trapezoid of a constant integrand (S + int (z_k - z_top)). Each
-// column's KC levels leave as runs of 512 bytes (shorter runs of this layout,
-// whose columns are not aligned to cache lines, write HBM at a fraction of
-// the rate).
+// w by the trapezoid of a constant integrand (S + int (z_k - z_top)), on
+// 128-byte lines; a tile whose faces and columns all lie below the second
+// level has no chunk at all and is written as contiguous rows. Each column's
+// KC levels of a chunk leave as runs of 512 bytes (shorter runs of this
+// layout, whose columns are not aligned to cache lines, write HBM at a
+// fraction of the rate).
 // workgroup barrier that orders LDS only: the global stores in flight are
 // not waited for (__syncthreads would wait for them at every chunk)
 __device__ __forceinline__ void lds_barrier() {
