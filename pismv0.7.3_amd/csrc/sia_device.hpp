@@ -383,11 +383,20 @@ __device__ __forceinline__ double flow_gpbld_fast(const DevParams &P, double str
   const double E_s = P.ci * (Tm - P.T0);                   // enthalpy_cts_impl
   const double T_pa = ((E * P.inv_ci) + P.T0) - Tm + P.T_melt;  // temperature_impl, pressure_adjusted_temperature
   const double y = rcp_nr(T_pa);
-  const double x_cold = P.ln_A_cold - P.QR_cold * y, x_warm = P.ln_A_warm - P.QR_warm * y;
-  const double A_c = exp_fast((T_pa < P.T_crit) ? x_cold : x_warm, *(const ExpCoef *)P.exp_c);  // softness_parameter_paterson_budd
-  const double omega = fmin((E - E_s) * P.inv_L, P.wf_limit);  // water_fraction_impl, capped
-  const double A_t = P.A_T0 * (1.0 + P.wf_coeff * omega);
-  const double A = (E < E_s) ? A_c : A_t;
+  // the warm Paterson-Budd branch and the temperate softness are evaluated only
+  // when some lane of the wavefront takes them (wave-uniform branches); each
+  // lane's value is the one it selects either way
+  double x = P.ln_A_cold - P.QR_cold * y;
+  if (__ballot(!(T_pa < P.T_crit))) {
+    const double x_warm = P.ln_A_warm - P.QR_warm * y;
+    x = (T_pa < P.T_crit) ? x : x_warm;
+  }
+  double A = exp_fast(x, *(const ExpCoef *)P.exp_c);  // softness_parameter_paterson_budd
+  if (__ballot(!(E < E_s))) {
+    const double omega = fmin((E - E_s) * P.inv_L, P.wf_limit);  // water_fraction_impl, capped
+    const double A_t = P.A_T0 * (1.0 + P.wf_coeff * omega);
+    A = (E < E_s) ? A : A_t;
+  }
   const double sp = (NM1 == 1) ? stress : ((NM1 == 2) ? stress * stress : stress * stress * stress);
   return A * sp;
 }
