@@ -99,7 +99,9 @@ enum sia_enthalpy_converter_kind {
   SIA_EC_KIRCHHOFF = 2           /* KirchhoffEnthalpyConverter (enthalpyConverter.cc:422-435) */
 };
 
-/* ---- grid: IceGrid's Mx, My, Mz, dx, dy, z[] (IceGrid.hh:201-295) ------- */
+/* ---- grid: IceGrid's Mx, My, Mz, dx, dy, z[] (IceGrid.hh:201-295) -------
+   Mx, My >= 3, 2 Mx My < 2^31, 2 <= Mz <= 4096; sia_create returns
+   SIA_ERR_ARGUMENT otherwise, before any device call. */
 typedef struct sia_grid {
   int32_t Mx, My, Mz;
   int32_t reserved;

@@ -66,6 +66,33 @@ def test_params_default_matches_config_defaults(lib):
     assert ctypes.sizeof(T.SiaParams) == 4 * 8 + 8 * len(T._PARAM_DOUBLE_FIELDS)
 
 
+def test_create_rejects_invalid_grids_before_any_device_call(lib):
+    """sia_create checks the grid and the parameter selections before its
+    first HIP call and returns a status: at least 3 columns each way, at least
+    2 and at most 4096 levels, strictly increasing levels, known flow laws."""
+    create = lib.sia_create
+    create.argtypes = [ctypes.POINTER(T.SiaGrid), ctypes.POINTER(T.SiaParams), ctypes.POINTER(ctypes.c_void_p)]
+    create.restype = ctypes.c_int
+    h = ctypes.c_void_p()
+
+    def status(Mx, My, z, params=None):
+        g = T.make_grid(Mx, My, 1000.0, 1000.0, z)
+        p = params if params is not None else T.make_params()
+        return create(ctypes.byref(g), ctypes.byref(p), ctypes.byref(h))
+
+    z = T.vertical_levels(4000.0, 11)
+    assert status(5, 5, T.vertical_levels(4000.0, 4097)) == T.SIA_ERR_ARGUMENT
+    assert status(2, 5, z) == T.SIA_ERR_ARGUMENT
+    assert status(5, 2, z) == T.SIA_ERR_ARGUMENT
+    assert status(5, 5, z[:1]) == T.SIA_ERR_ARGUMENT
+    flat = z.copy()
+    flat[5] = flat[4]
+    assert status(5, 5, flat) == T.SIA_ERR_ARGUMENT
+    assert status(5, 5, z, T.make_params(sia_flow_law=99)) == T.SIA_ERR_CONFIG
+    assert status(5, 5, z, T.make_params(surface_gradient_method=7)) == T.SIA_ERR_CONFIG
+    assert h.value is None
+
+
 def test_library_is_built_for_gfx950(lib):
     data = open(LIB, "rb").read()
     assert b"amdgcn-amd-amdhsa--gfx950" in data

@@ -191,13 +191,34 @@ def test_margin_tiles_with_sliding_and_bmr(S, oracle):
 def test_full_update_601_levels(S, oracle):
     """A full update with 601 levels: the velocity pass keeps z in LDS (8 Mz
     bytes beside its 37 KB of tile data), so above 459 levels it runs 3 tile
-    blocks per CU instead of 4; the results must not depend on that."""
-    say("[gpu] full update with 601 levels")
-    w = W.greenland_like(140, 164, 1350, 1375, Mz=601, Lz=4000.0)
+    blocks per CU instead of 4; the results must not depend on that. The
+    window crosses the ice margin (70 % icy, up to 963 m: 3 chunks of 64
+    levels), so both the chunk loop and the ice-free rows run."""
+    say("[gpu] full update with 601 levels, margin window 24x25")
+    w = W.greenland_like(170, 194, 1385, 1410, Mz=601, Lz=4000.0)
+    assert 0.5 < (w.fields["H"] > 0).mean() < 0.9 and w.fields["H"].max() > 900.0
     g = run_gpu(w)
     o = run_oracle(oracle, w)
     errs = check_against_oracle(g, o, w.grid, label="Mz601")
     say("  max rel errors:", {k: "%.1e" % v for k, v in errs.items()})
+
+
+def test_full_and_fast_update_at_the_level_limit(S, oracle):
+    """4096 levels (about 1 m apart), the most sia_create accepts: the column
+    pass stages 24 bytes a level in LDS (96 KB) and the velocity tiles 8 bytes
+    a level beside their 37 KB, so each runs fewer blocks per CU, and a thick
+    column takes up to 50 chunks of 64 levels. A full and a fast update against
+    the oracle on a margin window (69 % icy, up to 817 m) and an interior one
+    (3142-3165 m: about 3240 levels of ice, 51 chunks)."""
+    for (i0, j0) in ((172, 1395), (740, 1380)):
+        say("[gpu] full and fast update with 4096 levels, window 16x16 at (%d, %d)" % (i0, j0))
+        w = W.greenland_like(i0, i0 + 16, j0, j0 + 16, Mz=4096, Lz=4000.0)
+        assert w.fields["H"].max() > 800.0
+        for full in (True, False):
+            g = run_gpu(w, full=full)
+            o = run_oracle(oracle, w, full=full)
+            errs = check_against_oracle(g, o, w.grid, full=full, label="Mz4096 (%d, %d) full=%d" % (i0, j0, full))
+            say("  full=%d max rel errors:" % full, {k: "%.1e" % v for k, v in errs.items()})
 
 
 def test_grain_size_from_age(S, oracle):
