@@ -104,8 +104,10 @@ def vialov_smb(params, H0=3600.0, L=750e3, n=3.0):
 
 
 # ------------------------------------------------------------------ greenland-like
+# z_spacing: "equal" (the benchmark sheet) or "quadratic" (PISM's default,
+# grid_ice_vertical_spacing, cdl:1289, with grid_lambda = 4, cdl:1334)
 GL_DEFAULT = dict(Mx=1501, My=2801, dx=1000.0, Mz=401, Lz=4000.0, seed=20261203,
-                  n_bumps=200, Hmax=3200.0)
+                  n_bumps=200, Hmax=3200.0, z_spacing="equal")
 
 
 def gl_params(**overrides):
@@ -168,7 +170,7 @@ def greenland_like(i0=0, i1=None, j0=0, j1=None, **kw):
     f_t = 0.1 * uniform01(seed, S_TEMPERATE, col)
     omega = 0.01 * uniform01(seed, S_OMEGA, col)
     T_s = 238.15 + 10.0 * uniform01(seed, S_TSURF, col)
-    z = T.vertical_levels(Lz, Mz)
+    z = T.vertical_levels(Lz, Mz, cfg["z_spacing"])
     E = enthalpy_columns(P, H, f_t, omega, T_s, z)
     grid = T.make_grid(i1 - i0, j1 - j0, dx, dx, z)
     fields = dict(H=H.ravel(), b=b.ravel(), h=h.ravel(), mask=mask.ravel(), E=E.ravel())

@@ -221,6 +221,24 @@ def test_full_and_fast_update_at_the_level_limit(S, oracle):
             say("  full=%d max rel errors:" % full, {k: "%.1e" % v for k, v in errs.items()})
 
 
+@pytest.mark.parametrize("case", [((170, 210, 1385, 1429), 101), ((700, 740, 1180, 1220), 401)])
+def test_quadratic_levels(S, oracle, case):
+    """PISM's default vertical grid, quadratic with lambda = 4 (cdl:1289,
+    1334; IceGrid.cc:362-369): levels 4x finer at the base than the mean, so
+    dz changes at every level (the trapezoids, kBelowHeight, the centred
+    vertical differences of the strain heating). A margin window at 101
+    levels and an interior one at 401, full and fast updates."""
+    win, Mz = case
+    say("[gpu] quadratic levels, window %s, Mz %d" % (win, Mz))
+    w = W.greenland_like(*win, Mz=Mz, Lz=4000.0, z_spacing="quadratic")
+    assert w.fields["H"].max() > 1000.0
+    for full in (True, False):
+        g = run_gpu(w, full=full)
+        o = run_oracle(oracle, w, full=full)
+        errs = check_against_oracle(g, o, w.grid, full=full, label="quadratic %s full=%d" % (win, full))
+        say("  full=%d max rel errors:" % full, {k: "%.1e" % v for k, v in errs.items()})
+
+
 def test_grain_size_from_age(S, oracle):
     """Goldsby-Kohlstedt with grain size from age (SIAFD.cc:575-577, 666-669)."""
     say("[gpu] GK with age-dependent grain size")

@@ -135,12 +135,31 @@ def test_inference_matches_oracle(S, trained, win):
     say("[gpu] fused inference vs oracle forward, window %s" % (win,))
     sg, _ = trained
     w, dev = window_fields(S, win)
+    inference_vs_oracle(S, sg, w, dev, margin_window=(win == WINDOW_MARGIN))
+
+
+def test_inference_matches_oracle_on_quadratic_levels(S, trained):
+    """PISM's default vertical grid (quadratic with lambda = 4, cdl:1289,
+    1334): 401 levels from 2.5 m apart at the base to 17.5 m at the top, so up
+    to 7 levels share one of the feature pass's 15.6 m level buckets and the
+    cell bounds take the step-by-step walk past a bucket's first level. The
+    network's inputs do not depend on the level spacing (32 cells in
+    z / thk), so the model trained on equal levels applies."""
+    say("[gpu] fused inference vs oracle forward, quadratic levels")
+    sg, _ = trained
+    w, dev = window_fields(S, WINDOW_TEST, z_spacing="quadratic")
+    z = np.array([w.grid.z[k] for k in range(w.grid.Mz)])
+    assert z[1] - z[0] < 3.0 and z[-1] - z[-2] > 15.0
+    inference_vs_oracle(S, sg, w, dev)
+
+
+def inference_vs_oracle(S, sg, w, dev, margin_window=False):
     ref = plain_fast(S, w, dev)
     sur = surrogate_fast(S, sg, w, dev)
     norm, wts = parse_norm(sg.save())
     st = SO.staggered_inputs(w.grid, w.fields, ref["thks"], ref["theta"], ref["hx"], ref["hy"])
     sel = np.nonzero(eligible(w, st))[0]
-    if win == WINDOW_MARGIN:
+    if margin_window:
         assert len(sel) % 2 == 1 and len(sel) % 32 != 0, len(sel)
     z = np.array([w.grid.z[k] for k in range(w.grid.Mz)])
     c = SO.columns(w.params, z, st["thk"][sel], st["alpha"][sel], st["theta"][sel], st["E"][sel], targets=False)
