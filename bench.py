@@ -35,14 +35,19 @@ sizes, one kernel launch per stage for all members, 50 steps captured in a
 HIP graph) and reports it under "ensemble", beside the same members stepped
 with per-member launches.
 
---combined runs the combined workload instead and prints its own line: the
-surrogate inside the 64-member ensemble (the sheet + 63 ensemble members),
-5000 steps in HIP graphs of 100, the surrogate's flux error checked in FP64
-every 100 steps, beside an all-plain FP64 run of the same seeds.
+It also runs the combined workload (the sheet + 63 ensemble members in one
+batched ensemble, the surrogate bound to the sheet) for 200 steps with the
+surrogate trained above, beside an all-plain FP64 run of the same seeds, and
+reports it under "combined" (--no-combined skips it).
+
+--combined runs the combined workload alone at full length and prints its own
+line: 5000 steps in HIP graphs of 100, one surrogate trained per Glen
+exponent, the surrogate's flux error checked in FP64 every 100 steps, beside
+an all-plain FP64 run of the same seeds.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--path surrogate|plain]
-                       [--mode full|fast] [--no-cpu-baseline] [--no-ensemble] [--copy-calibration GIB]
-                       [--combined]
+                       [--mode full|fast] [--no-cpu-baseline] [--no-ensemble] [--no-combined]
+                       [--copy-calibration GIB] [--combined]
 """
 import argparse
 import json
@@ -421,7 +426,7 @@ def leg_error(leg):
             "p99_99_rel_flux_err_all_columns": max(c["p99_99_all"] for c in leg["checks"]
                                                    if c["p99_99_all"] is not None),
             "against": "an FP64 update of the same state of each bound member, every %d steps (all columns every "
-                       "%d steps)" % (COMBINED_CHECK, 10 * COMBINED_CHECK),
+                       "%d steps and at the last check)" % (COMBINED_CHECK, 10 * COMBINED_CHECK),
             "bound_columns_last_check": {"zero": int(cnt[0]), "surrogate": int(cnt[1]), "fp64_fallback": int(cnt[2])}}
 
 
@@ -489,6 +494,34 @@ def run_combined(S, torch, args):
     print(json.dumps(out), flush=True)
 
 
+def run_combined_short(S, torch, sg, steps):
+    """The combined workload for a few graphs of COMBINED_CHECK steps with the
+    benchmark's sheet surrogate bound to the sheet (the depth policy of the
+    full combined run: no member of 101-201 levels is bound), beside an
+    all-plain FP64 run of the same seeds."""
+    log("combined workload, %d steps: the surrogate on the sheet, FP64 on the 63 members" % steps)
+    pol = run_combined_leg(S, torch, {3.0: sg}, steps, COMBINED_CHECK, np.random.default_rng(20261203),
+                           min_mz=SURROGATE_MIN_MZ)
+    log("combined workload, %d steps: all-plain FP64" % steps)
+    pln = run_combined_leg(S, torch, None, steps, COMBINED_CHECK)
+    cols = pol["cols"]
+    return {
+        "metric": "SIA column-updates/s, combined workload: surrogate inside the 64-member ensemble",
+        "value": round(cols / (pol["ms_per_step"] / 1000.0), 1), "unit": "column-updates/s",
+        "steps": steps, "ms_per_step": round(pol["ms_per_step"], 4),
+        "workload": "the 1501x2801x401 sheet + the first 63 members of the ensemble workload (64 members, "
+                    "%d columns); one step = fast SIA update + dt + mass continuity of every member; HIP graphs "
+                    "of %d steps; the surrogate of this run bound to the sheet (members with Mz >= %d), FP64 on "
+                    "the others" % (cols, COMBINED_CHECK, SURROGATE_MIN_MZ),
+        "tile_cols_per_wave": pol["tile"],
+        "speedup_vs_all_plain": round(pln["ms_per_step"] / pol["ms_per_step"], 3),
+        "all_plain_fp64": {"value": round(cols / (pln["ms_per_step"] / 1000.0), 1),
+                           "ms_per_step": round(pln["ms_per_step"], 4), "tile_cols_per_wave": pln["tile"]},
+        "surrogate_error": leg_error(pol),
+        "final_state_vs_all_plain": final_vs(pol, pln),
+    }
+
+
 def error_stats(a, b):
     """Relative error of a against b over the entries where b != 0."""
     nz = b != 0
@@ -518,6 +551,8 @@ def main():
     ap.add_argument("--train-batch", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the 64-member ensemble measurement")
+    ap.add_argument("--no-combined", action="store_true",
+                    help="skip the 200-step combined-workload measurement of the default run")
     ap.add_argument("--combined", action="store_true",
                     help="run the combined workload instead (surrogate inside the 64-member ensemble, 5000 steps)")
     ap.add_argument("--combined-steps", type=int, default=COMBINED_STEPS,
@@ -625,11 +660,14 @@ def main():
                                      "unit": "TFLOP/s", "frac": round(tf / BF16_PEAK_TFLOPS, 4),
                                      "kernel": "surrogate MLP", "avg_launch_ms": round(mlp_ms, 4)}
 
+    if args.path == "surrogate" and not args.no_combined:
+        out_extra["combined"] = run_combined_short(S, torch, sg, 2 * COMBINED_CHECK)
+        log("  %s" % out_extra["combined"])
+    if args.path == "surrogate":
+        sur.close()
+        sg.close()  # frees the training samples
     if not args.no_ensemble:
         log("64-member ensemble (one launch per stage, HIP graph of 50 steps)")
-        if args.path == "surrogate":
-            sur.close()
-            sg.close()  # frees the training samples
         out_extra["ensemble"] = run_ensemble(S, torch)
         log("  %s" % out_extra["ensemble"])
 
