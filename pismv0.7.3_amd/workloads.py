@@ -105,9 +105,10 @@ def vialov_smb(params, H0=3600.0, L=750e3, n=3.0):
 
 # ------------------------------------------------------------------ greenland-like
 # z_spacing: "equal" (the benchmark sheet) or "quadratic" (PISM's default,
-# grid_ice_vertical_spacing, cdl:1289, with grid_lambda = 4, cdl:1334)
+# grid_ice_vertical_spacing, cdl:1289, with grid_lambda = 4, cdl:1334);
+# dy: None for dy = dx (the benchmark sheet)
 GL_DEFAULT = dict(Mx=1501, My=2801, dx=1000.0, Mz=401, Lz=4000.0, seed=20261203,
-                  n_bumps=200, Hmax=3200.0, z_spacing="equal")
+                  n_bumps=200, Hmax=3200.0, z_spacing="equal", dy=None)
 
 
 def gl_params(**overrides):
@@ -138,14 +139,15 @@ def greenland_like(i0=0, i1=None, j0=0, j1=None, **kw):
     cfg.update({k: v for k, v in kw.items() if k in cfg})
     poverrides = {k: v for k, v in kw.items() if k not in cfg}
     Mx, My, dx, Mz, Lz, seed = cfg["Mx"], cfg["My"], cfg["dx"], cfg["Mz"], cfg["Lz"], cfg["seed"]
+    dy = dx if cfg["dy"] is None else cfg["dy"]
     i1 = Mx if i1 is None else i1
     j1 = My if j1 is None else j1
     P = gl_params(**poverrides)
-    Lx, Ly = 0.5 * (Mx - 1) * dx, 0.5 * (My - 1) * dx
+    Lx, Ly = 0.5 * (Mx - 1) * dx, 0.5 * (My - 1) * dy
     ii = np.arange(i0, i1)
     jj = np.arange(j0, j1)
     x = -Lx + ii * dx
-    y = -Ly + jj * dx
+    y = -Ly + jj * dy
     X, Y = np.meshgrid(x, y, indexing="ij")
     # bed
     a, w, cx, cy = gl_bumps(seed, cfg["n_bumps"], Lx, Ly)
@@ -172,7 +174,7 @@ def greenland_like(i0=0, i1=None, j0=0, j1=None, **kw):
     T_s = 238.15 + 10.0 * uniform01(seed, S_TSURF, col)
     z = T.vertical_levels(Lz, Mz, cfg["z_spacing"])
     E = enthalpy_columns(P, H, f_t, omega, T_s, z)
-    grid = T.make_grid(i1 - i0, j1 - j0, dx, dx, z)
+    grid = T.make_grid(i1 - i0, j1 - j0, dx, dy, z)
     fields = dict(H=H.ravel(), b=b.ravel(), h=h.ravel(), mask=mask.ravel(), E=E.ravel())
     return Workload("greenland_like", grid, P, fields, (Mx, My, Mz), (i0, i1, j0, j1))
 

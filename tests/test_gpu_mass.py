@@ -31,16 +31,33 @@ def S():
 def test_mass_continuity_step_matches_oracle(S, oracle):
     say("[gpu] dome: 10 steps of update + dt + mass continuity vs oracle")
     w = W.dome(Mx=41, My=41, Mz=31)
+    M0 = W.vialov_smb(w.params)
+    steps_vs_oracle(S, oracle, w, M0 * w.params.ice_density, 10)
+
+
+def test_mass_continuity_on_rectangular_cells(S, oracle):
+    """dx = 1 km, dy = 2.5 km on a margin window (a 1501x1121 sheet of the
+    same extent): the flux divergence, the diffusivity dt limit
+    (iMadaptive.cc:157-171) and the interface fluxes each take their own
+    spacing, which a grid with dx = dy cannot tell apart."""
+    say("[gpu] rectangular cells: 6 steps of update + dt + mass continuity vs oracle")
+    w = W.greenland_like(160, 200, 570, 606, My=1121, Mz=101, dx=1000.0, dy=2500.0)
+    assert w.grid.dx != w.grid.dy
+    steps_vs_oracle(S, oracle, w, 0.3 / T.SECONDS_PER_YEAR * w.params.ice_density, 6)
+
+
+def steps_vs_oracle(S, oracle, w, smb_value, nsteps):
+    """nsteps of update (full, fast alternately) + dt + mass continuity on the
+    GPU and in the oracle from the same state, with a uniform SMB (kg m-2 s-1)."""
     st = {k: v.copy() for k, v in w.fields.items()}
     E = st.pop("E")
     dev = {k: S.to_device(v) for k, v in st.items()}
     dE = S.to_device(E)
-    M0 = W.vialov_smb(w.params)
-    smb = np.full(w.grid.Mx * w.grid.My, M0 * w.params.ice_density)  # kg m-2 s-1
+    smb = np.full(w.grid.Mx * w.grid.My, smb_value)
     dsmb = S.to_device(smb)
     sia = S.SIAFD(w.grid, w.params)
     geom = S.geometry(dev["H"], dev["h"], dev["mask"], dev["b"], smb=dsmb)
-    for step in range(10):
+    for step in range(nsteps):
         sia.update(dev["H"], dev["b"], dev["h"], dev["mask"], dE, fast=(step % 2 == 1))
         dt_g = sia.max_timestep()
         acc_g = sia.mass_continuity_step(geom, dt_g)

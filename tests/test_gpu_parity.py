@@ -239,6 +239,22 @@ def test_quadratic_levels(S, oracle, case):
         say("  full=%d max rel errors:" % full, {k: "%.1e" % v for k, v in errs.items()})
 
 
+def test_rectangular_cells(S, oracle):
+    """dx = 1 km, dy = 2.5 km (a 40x36 margin window of a 1501x1121 sheet of
+    the same extent): every difference along x and along y takes its own
+    spacing (the surface gradient and its haseloff weights, u_x and v_y of the
+    vertical velocity and of the strain heating), which a grid with dx = dy
+    cannot tell apart. Full and fast updates."""
+    say("[gpu] rectangular cells dx = 1 km, dy = 2.5 km")
+    w = W.greenland_like(160, 200, 570, 606, My=1121, Mz=101, dx=1000.0, dy=2500.0)
+    assert w.grid.dx != w.grid.dy and 0.3 < (w.fields["H"] > 0).mean() < 0.9
+    for full in (True, False):
+        g = run_gpu(w, full=full)
+        o = run_oracle(oracle, w, full=full)
+        errs = check_against_oracle(g, o, w.grid, full=full, label="dx != dy full=%d" % full)
+        say("  full=%d max rel errors:" % full, {k: "%.1e" % v for k, v in errs.items()})
+
+
 def test_grain_size_from_age(S, oracle):
     """Goldsby-Kohlstedt with grain size from age (SIAFD.cc:575-577, 666-669)."""
     say("[gpu] GK with age-dependent grain size")
