@@ -161,6 +161,19 @@ def test_glen_exponent_4_and_sliding_and_bmr(S, oracle):
     check_against_oracle(g, o, w.grid, ub=ub, vb=vb, bmr=bmr, label="n4")
 
 
+@pytest.mark.parametrize("n", [2.0, 3.5])
+def test_other_glen_exponents(S, oracle, n):
+    """n = 2 (GPBLD's integer-power column body with n - 1 = 1) and n = 3.5 (a
+    non-integer exponent: pow in the flow law, the generic column body, and
+    non-integer exponents in the strain heating); full updates."""
+    say("[gpu] Glen exponent", n)
+    w = W.greenland_like(740, 772, 1380, 1420, Mz=101, Lz=4000.0, sia_Glen_exponent=n)
+    g = run_gpu(w)
+    o = run_oracle(oracle, w)
+    errs = check_against_oracle(g, o, w.grid, label="n=%g" % n)
+    say("  max rel errors:", {k: "%.1e" % v for k, v in errs.items()})
+
+
 def test_margin_tiles_with_sliding_and_bmr(S, oracle):
     """A 50x51 window across the ice margin (not a multiple of the velocity
     pass's 4x4 tiles in either direction) with a sliding velocity and a basal
