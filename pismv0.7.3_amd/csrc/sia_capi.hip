@@ -669,7 +669,8 @@ int sia_create(const sia_grid *grid, const sia_params *params, sia_handle **out)
       (rc = dalloc(h, &h->dt_dev, 1)) ||
       (rc = dalloc(h, &h->acc_dev, 8)) || (rc = dalloc(h, &h->H_tmp, n2)) ||
       (rc = dalloc(h, &h->partials, (size_t)mass_partial_blocks(h->dp) * 8)) ||
-      (rc = dalloc(h, &h->dmax_part, (size_t)(column_blocks(h->dp) + sg_mlp_blocks())))) {
+      (rc = dalloc(h, &h->dmax_part,
+                   (size_t)(std::max(column_blocks(h->dp), column_strip_blocks(h->dp)) + sg_mlp_blocks())))) {
     sia_destroy(h);
     return rc;
   }

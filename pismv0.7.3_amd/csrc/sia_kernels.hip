@@ -450,7 +450,7 @@ static inline size_t column_lds(int Mz) { return 3 * sizeof(double) * (size_t)Mz
 
 template <int LAW, bool FULL, int CPW, int FASTNM1>
 __device__ __forceinline__ double column_block_impl(const DevParams &P, const ColumnArgs &A, int bid, double *zs,
-                                                    double *wmax, const int *list, int n_list) {
+                                                    double *wmax, const int *list, int n_list, bool strips) {
   constexpr int COLS_PER_WAVE = CPW;
   constexpr int COLS_PER_BLOCK = WAVES_PER_BLOCK * COLS_PER_WAVE;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -473,8 +473,25 @@ __device__ __forceinline__ double column_block_impl(const DevParams &P, const Co
   }
   __syncthreads();
   const int s0 = (bid * WAVES_PER_BLOCK + wv) * COLS_PER_WAVE;
-  // list mode: lane c < COLS_PER_WAVE takes list entry s0 + c (nstag: none)
-  const int s = list ? ((lane < COLS_PER_WAVE && s0 + lane < n_list) ? list[s0 + lane] : nstag) : s0 + lane;
+  // list mode: lane c < COLS_PER_WAVE takes list entry s0 + c (nstag: none).
+  // strips (32 columns a wavefront): the wavefront takes the 32 staggered
+  // columns of one row of a strip of 16 map columns along j; strip k goes to
+  // XCD k % 8, and each XCD walks its strips row by row, so the rows i and
+  // i + 1 that both read the enthalpy of column (i + 1, j) run close in time
+  // on one L2 (in staggered order they are a row of the grid apart, mostly on
+  // different XCDs). Otherwise consecutive staggered columns.
+  int s;
+  if (list) {
+    s = (lane < COLS_PER_WAVE && s0 + lane < n_list) ? list[s0 + lane] : nstag;
+  } else if (COLS_PER_WAVE == 32 && strips) {
+    const long seg = (long)(bid >> 3) * WAVES_PER_BLOCK + wv;  // row segment of this XCD
+    const int sl = (int)(seg / Mx), row = (int)(seg - (long)sl * Mx);
+    const int strip = sl * N_XCD + (bid & (N_XCD - 1));
+    const int j = strip * 16 + (lane >> 1);
+    s = (lane < COLS_PER_WAVE && j < My) ? 2 * (row * My + j) + (lane & 1) : nstag;
+  } else {
+    s = s0 + lane;
+  }
 
   // lane c < COLS_PER_WAVE: column s0 + c. c_ks: -4 not selected (A.only),
   // -3 no column, -2 zero thickness (delta = 0, SIAFD.cc:635-641), -1
@@ -511,7 +528,7 @@ __device__ __forceinline__ double column_block_impl(const DevParams &P, const Co
   }
 
   auto Icol_of = [&](int c) -> double * {
-    const int s_c = list ? lane_value(s, c) : s0 + c;
+    const int s_c = (list || strips) ? lane_value(s, c) : s0 + c;
     return ((s_c & 1) ? A.I1 : A.I0) + (long)(s_c >> 1) * i_stride(Mz);
   };
   // first column >= c with a level to evaluate. I is 0 at level 0 of every
@@ -673,13 +690,14 @@ __device__ __forceinline__ bool use_gpbld_fast(const DevParams &P) {
 
 template <int LAW, bool FULL, int CPW>
 __device__ __forceinline__ double column_block(const DevParams &P, const ColumnArgs &A, int bid, double *zs,
-                                               double *wmax, const int *list = nullptr, int n_list = 0) {
+                                               double *wmax, const int *list = nullptr, int n_list = 0,
+                                               bool strips = false) {
   if (LAW == LAW_GPBLD && use_gpbld_fast(P)) {
-    if (P.nm1_int == 1) return column_block_impl<LAW, FULL, CPW, 1>(P, A, bid, zs, wmax, list, n_list);
-    if (P.nm1_int == 2) return column_block_impl<LAW, FULL, CPW, 2>(P, A, bid, zs, wmax, list, n_list);
-    return column_block_impl<LAW, FULL, CPW, 3>(P, A, bid, zs, wmax, list, n_list);
+    if (P.nm1_int == 1) return column_block_impl<LAW, FULL, CPW, 1>(P, A, bid, zs, wmax, list, n_list, strips);
+    if (P.nm1_int == 2) return column_block_impl<LAW, FULL, CPW, 2>(P, A, bid, zs, wmax, list, n_list, strips);
+    return column_block_impl<LAW, FULL, CPW, 3>(P, A, bid, zs, wmax, list, n_list, strips);
   }
-  return column_block_impl<LAW, FULL, CPW, 0>(P, A, bid, zs, wmax, list, n_list);
+  return column_block_impl<LAW, FULL, CPW, 0>(P, A, bid, zs, wmax, list, n_list, strips);
 }
 
 // 32 measured fastest on the 1501x2801x401 sheet (8, 16, 48 and 64 slower)
@@ -691,10 +709,10 @@ constexpr int LIST_COLS_PER_WAVE = 4;
 constexpr int LIST_COLS_PER_BLOCK = WAVES_PER_BLOCK * LIST_COLS_PER_WAVE;
 
 template <int LAW, bool FULL>
-__global__ __launch_bounds__(256) void k_column(DevParams P, ColumnArgs A) {
+__global__ __launch_bounds__(256) void k_column(DevParams P, ColumnArgs A, int strips) {
   extern __shared__ __align__(16) double zs[];  // column_lds(Mz) bytes
   __shared__ double wmax[WAVES_PER_BLOCK];
-  const double m = column_block<LAW, FULL, COLS_PER_WAVE>(P, A, blockIdx.x, zs, wmax);
+  const double m = column_block<LAW, FULL, COLS_PER_WAVE>(P, A, blockIdx.x, zs, wmax, nullptr, 0, strips != 0);
   if (threadIdx.x == 0) A.dmax_part[blockIdx.x] = m;
 }
 
@@ -1484,13 +1502,15 @@ __global__ void k_mask_surface(DevParams P, const double *__restrict__ H_new, co
 
 // ---------------------------------------------------------------- launchers
 template <int LAW>
-static void launch_column_law(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st, int extra) {
-  const int nb = column_blocks(P);
+static void launch_column_law(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st, int extra,
+                              bool allow_strips) {
+  const bool strips = allow_strips && use_column_strips(P, A);
+  const int nb = strips ? column_strip_blocks(P) : column_blocks(P);
   const size_t lds = column_lds(P.Mz);
   if (full) {
-    hipLaunchKernelGGL((k_column<LAW, true>), dim3(nb), dim3(256), lds, st, P, A);
+    hipLaunchKernelGGL((k_column<LAW, true>), dim3(nb), dim3(256), lds, st, P, A, (int)strips);
   } else {
-    hipLaunchKernelGGL((k_column<LAW, false>), dim3(nb), dim3(256), lds, st, P, A);
+    hipLaunchKernelGGL((k_column<LAW, false>), dim3(nb), dim3(256), lds, st, P, A, (int)strips);
   }
   hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, st, A.dmax_part, nb + extra, A.dmax_bits);
 }
@@ -1517,21 +1537,33 @@ void launch_column_list(const DevParams &P, const ColumnArgs &A, const int *list
   }
 }
 
-void launch_column(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st, int extra) {
+void launch_column(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st, int extra, bool allow_strips) {
   switch (P.law) {
-  case LAW_ISOGLEN: launch_column_law<LAW_ISOGLEN>(P, A, full, st, extra); break;
-  case LAW_PB: launch_column_law<LAW_PB>(P, A, full, st, extra); break;
-  case LAW_GPBLD: launch_column_law<LAW_GPBLD>(P, A, full, st, extra); break;
-  case LAW_HOOKE: launch_column_law<LAW_HOOKE>(P, A, full, st, extra); break;
-  case LAW_ARR: launch_column_law<LAW_ARR>(P, A, full, st, extra); break;
-  case LAW_ARRWARM: launch_column_law<LAW_ARRWARM>(P, A, full, st, extra); break;
-  default: launch_column_law<LAW_GK>(P, A, full, st, extra); break;
+  case LAW_ISOGLEN: launch_column_law<LAW_ISOGLEN>(P, A, full, st, extra, allow_strips); break;
+  case LAW_PB: launch_column_law<LAW_PB>(P, A, full, st, extra, allow_strips); break;
+  case LAW_GPBLD: launch_column_law<LAW_GPBLD>(P, A, full, st, extra, allow_strips); break;
+  case LAW_HOOKE: launch_column_law<LAW_HOOKE>(P, A, full, st, extra, allow_strips); break;
+  case LAW_ARR: launch_column_law<LAW_ARR>(P, A, full, st, extra, allow_strips); break;
+  case LAW_ARRWARM: launch_column_law<LAW_ARRWARM>(P, A, full, st, extra, allow_strips); break;
+  default: launch_column_law<LAW_GK>(P, A, full, st, extra, allow_strips); break;
   }
 }
 
 static inline unsigned blocks_for(long n, int per_block) { return (unsigned)((n + per_block - 1) / per_block); }
 
 int column_blocks(const DevParams &P) { return (int)blocks_for(2L * P.Mx * P.My, COLS_PER_BLOCK); }
+
+// strips of 16 map columns (one row of a strip a wavefront), ceil(strips / 8)
+// of them on each of the 8 XCDs; blocks of a strip slot past the last strip
+// exit at once
+int column_strip_blocks(const DevParams &P) {
+  const long per_xcd_strips = ((P.My + 15) / 16 + N_XCD - 1) / N_XCD;
+  return (int)(N_XCD * blocks_for(per_xcd_strips * P.Mx, WAVES_PER_BLOCK));
+}
+
+// strips pay where a row holds several of them; the selection mode
+// (ColumnArgs::only) keeps the staggered order
+bool use_column_strips(const DevParams &P, const ColumnArgs &A) { return A.only == nullptr && P.My >= 64; }
 
 int column_list_blocks(const DevParams &P) { return column_blocks(P) < 1024 ? column_blocks(P) : 1024; }
 

@@ -83,10 +83,16 @@ void launch_theta_thk(const DevParams &P, const double *H, const double *b, cons
                       const double *mask, const double *topgs, const double *maxtl, const double *C2,
                       const double *C3, const double *C4, double *theta, double *thks, int *err,
                       hipStream_t st);
+// blocks (= per-block D_max partials) of the column pass in staggered order,
+// and in XCD-interleaved strips (used where use_column_strips holds and the
+// caller allows it); A.dmax_part must hold the larger of the two
 int column_blocks(const DevParams &P);
+int column_strip_blocks(const DevParams &P);
+bool use_column_strips(const DevParams &P, const ColumnArgs &A);
 // extra_partials: further per-block D_max partials that other kernels wrote
-// after A.dmax_part[column_blocks(P) - 1]; all are reduced into A.dmax_bits
-void launch_column(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st, int extra_partials = 0);
+// after the column pass's own; all are reduced into A.dmax_bits
+void launch_column(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st, int extra_partials = 0,
+                   bool allow_strips = true);
 // fast-update FP64 pass over the n_list (device count) staggered columns of
 // list: column_list_blocks(P) <= column_blocks(P) per-block D_max partials,
 // then extra_partials more, reduced into A.dmax_bits

@@ -126,6 +126,6 @@ extern "C" int sia_flux_update_noalloc(const sia_grid *grid, const sia_params *p
   A.dmax_bits = dmax_bits;
   A.dmax_part = dmax_part;
   A.err = err;
-  launch_column(P, A, false, st);
+  launch_column(P, A, false, st, 0, false);  // staggered order: its partials fit N_PART
   return hipGetLastError() == hipSuccess ? SIA_OK : SIA_ERR_HIP;
 }
