@@ -1554,7 +1554,9 @@ int sia_ensemble_create(sia_handle *const *members, const sia_geometry *geometry
   }
   for (int i = 0; i < n; i++) {
     e->part_off[i] = part_total;
-    part_total += (size_t)(members[i]->sg ? 0 : ens_column_blocks(members[i]->dp, 8)) + (size_t)e->mlp_part[i] +
+    int ncol_max = 0;
+    for (int c = 0; c < 3; c++) ncol_max = std::max(ncol_max, ens_column_blocks(members[i]->dp, ENS_CPW[c]));
+    part_total += (size_t)(members[i]->sg ? 0 : ncol_max) + (size_t)e->mlp_part[i] +
                   (size_t)e->lst_part[i];
   }
   if (pt_total >= 2147483647LL || col_total[0] >= 2147483647LL) {

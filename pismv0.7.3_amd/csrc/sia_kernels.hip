@@ -1561,9 +1561,8 @@ int column_strip_blocks(const DevParams &P) {
   return (int)(N_XCD * blocks_for(per_xcd_strips * P.Mx, WAVES_PER_BLOCK));
 }
 
-// strips pay where a row holds several of them; the selection mode
-// (ColumnArgs::only) keeps the staggered order
-bool use_column_strips(const DevParams &P, const ColumnArgs &A) { return A.only == nullptr && P.My >= 64; }
+// the selection mode (ColumnArgs::only) keeps the staggered order
+bool use_column_strips(const DevParams &P, const ColumnArgs &A) { return A.only == nullptr && column_strips_ok(P); }
 
 int column_list_blocks(const DevParams &P) { return column_blocks(P) < 1024 ? column_blocks(P) : 1024; }
 
@@ -1721,7 +1720,8 @@ __global__ __launch_bounds__(256) void k_ens_column(const EnsMember *__restrict_
   __shared__ double wmax[WAVES_PER_BLOCK];
   const int m = ens_member(start, nm, blockIdx.x);
   const ColumnArgs A = global_view(T[m].A);
-  const double mb = column_block<LAW, FULL, CPW>(T[m].P, A, blockIdx.x - start[m], zs, wmax);
+  const bool strips = CPW == COLS_PER_WAVE && column_strips_ok(T[m].P);  // as ens_column_blocks
+  const double mb = column_block<LAW, FULL, CPW>(T[m].P, A, blockIdx.x - start[m], zs, wmax, nullptr, 0, strips);
   if (threadIdx.x == 0) A.dmax_part[blockIdx.x - start[m]] = mb;
 }
 
@@ -1797,8 +1797,15 @@ void launch_ens_column_list(const EnsListMember *T, const int *start, int nm, in
 
 int ens_point_blocks(const DevParams &P) { return (int)(((long)P.Mx * P.My + 255) / 256); }
 
+// a member's column blocks: in strips (as the plain pass) at 32 columns a
+// wavefront where column_strips_ok holds, else in staggered order; a multiple
+// of the XCD count, so that every member's block range starts on XCD 0 and
+// its strips land on the XCDs the plain pass puts them on
 int ens_column_blocks(const DevParams &P, int cpw) {
-  return (int)((2L * P.Mx * P.My + WAVES_PER_BLOCK * cpw - 1) / (WAVES_PER_BLOCK * cpw));
+  const long nb = (cpw == COLS_PER_WAVE && column_strips_ok(P))
+                      ? column_strip_blocks(P)
+                      : (2L * P.Mx * P.My + WAVES_PER_BLOCK * cpw - 1) / (WAVES_PER_BLOCK * cpw);
+  return (int)((nb + N_XCD - 1) / N_XCD * N_XCD);
 }
 
 template <int LAW, int CPW>

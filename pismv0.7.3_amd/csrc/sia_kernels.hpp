@@ -21,7 +21,7 @@ struct ColumnArgs {
   double *I0, *I1;           // [Mx][My][Mz] (full updates; levels 0..ks stored)
   int *ks;                   // [Mx][My][2] kBelowHeight of each staggered column (full updates)
   unsigned long long *dmax_bits;
-  double *dmax_part;         // [column_blocks(P)] per-block maxima of D
+  double *dmax_part;         // per-block maxima of D [max(column_blocks(P), column_strip_blocks(P))]
   int *err;
   const unsigned char *only; // nullptr, or per staggered column: compute it only if only[s] == ONLY_MARK
 };
@@ -88,6 +88,8 @@ void launch_theta_thk(const DevParams &P, const double *H, const double *b, cons
 // caller allows it); A.dmax_part must hold the larger of the two
 int column_blocks(const DevParams &P);
 int column_strip_blocks(const DevParams &P);
+// strips pay where a row of the grid holds several of them
+__host__ __device__ inline bool column_strips_ok(const DevParams &P) { return P.My >= 64; }
 bool use_column_strips(const DevParams &P, const ColumnArgs &A);
 // extra_partials: further per-block D_max partials that other kernels wrote
 // after the column pass's own; all are reduced into A.dmax_bits
