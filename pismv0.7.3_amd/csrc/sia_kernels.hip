@@ -474,20 +474,20 @@ __device__ __forceinline__ double column_block_impl(const DevParams &P, const Co
   __syncthreads();
   const int s0 = (bid * WAVES_PER_BLOCK + wv) * COLS_PER_WAVE;
   // list mode: lane c < COLS_PER_WAVE takes list entry s0 + c (nstag: none).
-  // strips (32 columns a wavefront): the wavefront takes the 32 staggered
-  // columns of one row of a strip of 16 map columns along j; strip k goes to
-  // XCD k % 8, and each XCD walks its strips row by row, so the rows i and
-  // i + 1 that both read the enthalpy of column (i + 1, j) run close in time
-  // on one L2 (in staggered order they are a row of the grid apart, mostly on
-  // different XCDs). Otherwise consecutive staggered columns.
+  // strips (32 or 64 columns a wavefront): the wavefront takes the staggered
+  // columns of one row of a strip of COLS_PER_WAVE / 2 map columns along j;
+  // strip k goes to XCD k % 8, and each XCD walks its strips row by row, so
+  // the rows i and i + 1 that both read the enthalpy of column (i + 1, j) run
+  // close in time on one L2 (in staggered order they are a row of the grid
+  // apart, mostly on different XCDs). Otherwise consecutive staggered columns.
   int s;
   if (list) {
     s = (lane < COLS_PER_WAVE && s0 + lane < n_list) ? list[s0 + lane] : nstag;
-  } else if (COLS_PER_WAVE == 32 && strips) {
+  } else if ((COLS_PER_WAVE == 32 || COLS_PER_WAVE == 64) && strips) {
     const long seg = (long)(bid >> 3) * WAVES_PER_BLOCK + wv;  // row segment of this XCD
     const int sl = (int)(seg / Mx), row = (int)(seg - (long)sl * Mx);
     const int strip = sl * N_XCD + (bid & (N_XCD - 1));
-    const int j = strip * 16 + (lane >> 1);
+    const int j = strip * (COLS_PER_WAVE / 2) + (lane >> 1);
     s = (lane < COLS_PER_WAVE && j < My) ? 2 * (row * My + j) + (lane & 1) : nstag;
   } else {
     s = s0 + lane;
@@ -700,19 +700,30 @@ __device__ __forceinline__ double column_block(const DevParams &P, const ColumnA
   return column_block_impl<LAW, FULL, CPW, 0>(P, A, bid, zs, wmax, list, n_list, strips);
 }
 
-// 32 measured fastest on the 1501x2801x401 sheet (8, 16, 48 and 64 slower)
-constexpr int COLS_PER_WAVE = 32;
-constexpr int COLS_PER_BLOCK = WAVES_PER_BLOCK * COLS_PER_WAVE;
+// columns a wavefront of the plain pass. On the 1501x2801x401 sheet 64 (in
+// strips of 32 map columns) is faster than 32 (in staggered order the two
+// measured the same; 8, 16 and 48 slower); on grids of 801x801 and below,
+// stepped one at a time, 64 is slower: half as many wavefronts leave more of
+// the chip idle in the last round. 64 is taken where the grid gives each of
+// the chip's 4096 wavefront slots (256 CUs x 4 SIMDs x 4 waves) at least 16
+// wavefronts.
+constexpr int COLS_PER_WAVE_WIDE = 64, COLS_PER_WAVE_NARROW = 32;
+static int column_cpw(const DevParams &P) {
+  return 2L * P.Mx * P.My >= 16L * 4096 * COLS_PER_WAVE_WIDE ? COLS_PER_WAVE_WIDE : COLS_PER_WAVE_NARROW;
+}
+// the widest of the ensemble's tile widths (8, 16, 32 columns a wavefront),
+// which takes strips
+constexpr int ENS_STRIP_CPW = 32;
 // the FP64 pass over a list (the surrogate's fallback columns, a small share of
 // deep columns) takes 4 columns per wavefront, so that more wavefronts share it
 constexpr int LIST_COLS_PER_WAVE = 4;
 constexpr int LIST_COLS_PER_BLOCK = WAVES_PER_BLOCK * LIST_COLS_PER_WAVE;
 
-template <int LAW, bool FULL>
+template <int LAW, bool FULL, int CPW>
 __global__ __launch_bounds__(256) void k_column(DevParams P, ColumnArgs A, int strips) {
   extern __shared__ __align__(16) double zs[];  // column_lds(Mz) bytes
   __shared__ double wmax[WAVES_PER_BLOCK];
-  const double m = column_block<LAW, FULL, COLS_PER_WAVE>(P, A, blockIdx.x, zs, wmax, nullptr, 0, strips != 0);
+  const double m = column_block<LAW, FULL, CPW>(P, A, blockIdx.x, zs, wmax, nullptr, 0, strips != 0);
   if (threadIdx.x == 0) A.dmax_part[blockIdx.x] = m;
 }
 
@@ -1507,10 +1518,18 @@ static void launch_column_law(const DevParams &P, const ColumnArgs &A, bool full
   const bool strips = allow_strips && use_column_strips(P, A);
   const int nb = strips ? column_strip_blocks(P) : column_blocks(P);
   const size_t lds = column_lds(P.Mz);
-  if (full) {
-    hipLaunchKernelGGL((k_column<LAW, true>), dim3(nb), dim3(256), lds, st, P, A, (int)strips);
+  const bool wide = column_cpw(P) == COLS_PER_WAVE_WIDE;
+  if (full && wide) {
+    hipLaunchKernelGGL((k_column<LAW, true, COLS_PER_WAVE_WIDE>), dim3(nb), dim3(256), lds, st, P, A, (int)strips);
+  } else if (full) {
+    hipLaunchKernelGGL((k_column<LAW, true, COLS_PER_WAVE_NARROW>), dim3(nb), dim3(256), lds, st, P, A,
+                       (int)strips);
+  } else if (wide) {
+    hipLaunchKernelGGL((k_column<LAW, false, COLS_PER_WAVE_WIDE>), dim3(nb), dim3(256), lds, st, P, A,
+                       (int)strips);
   } else {
-    hipLaunchKernelGGL((k_column<LAW, false>), dim3(nb), dim3(256), lds, st, P, A, (int)strips);
+    hipLaunchKernelGGL((k_column<LAW, false, COLS_PER_WAVE_NARROW>), dim3(nb), dim3(256), lds, st, P, A,
+                       (int)strips);
   }
   hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(1024), 0, st, A.dmax_part, nb + extra, A.dmax_bits);
 }
@@ -1551,15 +1570,19 @@ void launch_column(const DevParams &P, const ColumnArgs &A, bool full, hipStream
 
 static inline unsigned blocks_for(long n, int per_block) { return (unsigned)((n + per_block - 1) / per_block); }
 
-int column_blocks(const DevParams &P) { return (int)blocks_for(2L * P.Mx * P.My, COLS_PER_BLOCK); }
+int column_blocks(const DevParams &P) {
+  return (int)blocks_for(2L * P.Mx * P.My, WAVES_PER_BLOCK * column_cpw(P));
+}
 
-// strips of 16 map columns (one row of a strip a wavefront), ceil(strips / 8)
-// of them on each of the 8 XCDs; blocks of a strip slot past the last strip
-// exit at once
-int column_strip_blocks(const DevParams &P) {
-  const long per_xcd_strips = ((P.My + 15) / 16 + N_XCD - 1) / N_XCD;
+// strips of cpw / 2 map columns (one row of a strip a wavefront of cpw
+// columns), ceil(strips / 8) of them on each of the 8 XCDs; blocks of a strip
+// slot past the last strip exit at once
+static int strip_blocks(const DevParams &P, int cpw) {
+  const int w = cpw / 2;
+  const long per_xcd_strips = ((P.My + w - 1) / w + N_XCD - 1) / N_XCD;
   return (int)(N_XCD * blocks_for(per_xcd_strips * P.Mx, WAVES_PER_BLOCK));
 }
+int column_strip_blocks(const DevParams &P) { return strip_blocks(P, column_cpw(P)); }
 
 // the selection mode (ColumnArgs::only) keeps the staggered order
 bool use_column_strips(const DevParams &P, const ColumnArgs &A) { return A.only == nullptr && column_strips_ok(P); }
@@ -1720,7 +1743,7 @@ __global__ __launch_bounds__(256) void k_ens_column(const EnsMember *__restrict_
   __shared__ double wmax[WAVES_PER_BLOCK];
   const int m = ens_member(start, nm, blockIdx.x);
   const ColumnArgs A = global_view(T[m].A);
-  const bool strips = CPW == COLS_PER_WAVE && column_strips_ok(T[m].P);  // as ens_column_blocks
+  const bool strips = CPW == ENS_STRIP_CPW && column_strips_ok(T[m].P);  // as ens_column_blocks
   const double mb = column_block<LAW, FULL, CPW>(T[m].P, A, blockIdx.x - start[m], zs, wmax, nullptr, 0, strips);
   if (threadIdx.x == 0) A.dmax_part[blockIdx.x - start[m]] = mb;
 }
@@ -1797,13 +1820,13 @@ void launch_ens_column_list(const EnsListMember *T, const int *start, int nm, in
 
 int ens_point_blocks(const DevParams &P) { return (int)(((long)P.Mx * P.My + 255) / 256); }
 
-// a member's column blocks: in strips (as the plain pass) at 32 columns a
-// wavefront where column_strips_ok holds, else in staggered order; a multiple
-// of the XCD count, so that every member's block range starts on XCD 0 and
-// its strips land on the XCDs the plain pass puts them on
+// a member's column blocks: in strips (16 map columns, as the plain pass at
+// 32 columns a wavefront) at ENS_STRIP_CPW columns a wavefront where
+// column_strips_ok holds, else in staggered order; a multiple of the XCD
+// count, so that every member's block range starts on XCD 0
 int ens_column_blocks(const DevParams &P, int cpw) {
-  const long nb = (cpw == COLS_PER_WAVE && column_strips_ok(P))
-                      ? column_strip_blocks(P)
+  const long nb = (cpw == ENS_STRIP_CPW && column_strips_ok(P))
+                      ? strip_blocks(P, ENS_STRIP_CPW)
                       : (2L * P.Mx * P.My + WAVES_PER_BLOCK * cpw - 1) / (WAVES_PER_BLOCK * cpw);
   return (int)((nb + N_XCD - 1) / N_XCD * N_XCD);
 }

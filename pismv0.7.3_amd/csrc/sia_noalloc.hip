@@ -22,7 +22,7 @@ namespace {
 //   h_x, h_y, D_stag [2 n2 each] | D_max partials [n2 / 32] | 3 words
 // (CFL bound, accumulated flags, flags when the caller passes none).
 constexpr long MAXC = SIA_NOALLOC_MAX_COLUMNS;
-constexpr long N_PART = MAXC / 32;  // at least the column blocks (2 n2 staggered columns, 128 per block)
+constexpr long N_PART = MAXC / 32;  // at least the column blocks (2 n2 staggered columns, 128 or more per block)
 constexpr long SCRATCH_DOUBLES = MAX_MZ + 7 * MAXC + 3 * 2 * MAXC + N_PART + 8;
 
 }  // namespace
