@@ -234,6 +234,22 @@ def test_full_and_fast_update_at_the_level_limit(S, oracle):
             say("  full=%d max rel errors:" % full, {k: "%.1e" % v for k, v in errs.items()})
 
 
+def test_wide_column_wavefronts(S, oracle):
+    """Grids of at least 2^21 map columns run the column pass at 64 staggered
+    columns a wavefront, in XCD-interleaved strips of 32 map columns (smaller
+    grids, as in the other tests: 32 columns in strips of 16). A full and a
+    fast update of a 1460x1460x21 window (73 % icy, up to 3200 m) against the
+    oracle."""
+    say("[gpu] full and fast update of a 1460x1460x21 window (64-column wavefronts)")
+    w = W.greenland_like(20, 1480, 670, 2130, Mz=21, Lz=4000.0)
+    assert w.grid.Mx * w.grid.My >= 2 ** 21 and w.fields["H"].max() > 3000.0
+    for full in (True, False):
+        g = run_gpu(w, full=full)
+        o = run_oracle(oracle, w, full=full)
+        errs = check_against_oracle(g, o, w.grid, full=full, label="1460x1460 full=%d" % full)
+        say("  full=%d max rel errors:" % full, {k: "%.1e" % v for k, v in errs.items()})
+
+
 @pytest.mark.parametrize("case", [((170, 210, 1385, 1429), 101), ((700, 740, 1180, 1220), 401)])
 def test_quadratic_levels(S, oracle, case):
     """PISM's default vertical grid, quadratic with lambda = 4 (cdl:1289,
