@@ -505,7 +505,6 @@ int surrogate_flux(sia_handle *h, const sia_inputs *in, ColumnArgs A) {
     launch_sg_mlp(P, M, st);
   }
   // the FP64 kernel on the listed fallback columns only
-  A.only = nullptr;
   launch_column_list(P, A, h->sg_fb_list, h->sg_nfb, st, sg_mlp_blocks());
   HIP_TRY(h, hipGetLastError());
   return SIA_OK;
@@ -1682,7 +1681,6 @@ int sia_ensemble_create(sia_handle *const *members, const sia_geometry *geometry
       EnsListMember Lm{};
       Lm.P = e->table[i].P;
       Lm.A = e->table[i].A;
-      Lm.A.only = nullptr;
       Lm.list = h->sg_fb_list;
       Lm.n_list = h->sg_nfb;
       g.ltable.push_back(Lm);

@@ -444,7 +444,7 @@ static inline size_t column_lds(int Mz) { return 3 * sizeof(double) * (size_t)Mz
 
 // block bid of the column grid with COLS_PER_WAVE = CPW; zs: column_lds(Mz)
 // bytes of LDS (16-byte aligned), wmax: WAVES_PER_BLOCK doubles of LDS. With a
-// list, the block's columns are list entries bid * COLS_PER_BLOCK .. (n_list
+// list, the block's columns are list entries bid * WAVES_PER_BLOCK * CPW .. (n_list
 // of them in all) instead of staggered columns. Returns the block's D_max in
 // thread 0.
 
@@ -452,16 +452,9 @@ template <int LAW, bool FULL, int CPW, int FASTNM1>
 __device__ __forceinline__ double column_block_impl(const DevParams &P, const ColumnArgs &A, int bid, double *zs,
                                                     double *wmax, const int *list, int n_list, bool strips) {
   constexpr int COLS_PER_WAVE = CPW;
-  constexpr int COLS_PER_BLOCK = WAVES_PER_BLOCK * COLS_PER_WAVE;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int Mx = P.Mx, My = P.My, Mz = P.Mz;
   const int nstag = 2 * Mx * My;
-  if (A.only) {
-    // a block none of whose columns is selected leaves at once
-    const int sb = bid * COLS_PER_BLOCK + (int)threadIdx.x;
-    const bool mine = (int)threadIdx.x < COLS_PER_BLOCK && sb < nstag && A.only[sb] == ONLY_MARK;
-    if (!__syncthreads_or(mine)) return 0.0;
-  }
   // LDS (column_lds bytes): per level (z_k, z_k - z_{k-1}) (0 at k = 0), one
   // 16-byte read a level; then z_k alone for kBelowHeight
   double2 *zh = reinterpret_cast<double2 *>(zs);
@@ -493,15 +486,12 @@ __device__ __forceinline__ double column_block_impl(const DevParams &P, const Co
     s = s0 + lane;
   }
 
-  // lane c < COLS_PER_WAVE: column s0 + c. c_ks: -4 not selected (A.only),
-  // -3 no column, -2 zero thickness (delta = 0, SIAFD.cc:635-641), -1
+  // lane c < COLS_PER_WAVE: column s0 + c. c_ks: -3 no column, -2 zero thickness (delta = 0, SIAFD.cc:635-641), -1
   // thickness above Lz, else kBelowHeight
   int c_ks = -3, c_qa = 0, c_qb = 0;
   double c_thk = 0.0, c_alpha = 0.0, c_c0 = 0.0, c_slope = 0.0;
   bool c_edge = false;
-  if (lane < COLS_PER_WAVE && s < nstag && A.only && A.only[s] != ONLY_MARK) {
-    c_ks = -4;
-  } else if (lane < COLS_PER_WAVE && s < nstag) {
+  if (lane < COLS_PER_WAVE && s < nstag) {
     const int q = s >> 1, o = s & 1;
     const int i = q / My, j = q - i * My;
     const int q2 = (o == 0) ? wrapi(i + 1, Mx) * My + j : i * My + wrapi(j + 1, My);
@@ -1515,7 +1505,7 @@ __global__ void k_mask_surface(DevParams P, const double *__restrict__ H_new, co
 template <int LAW>
 static void launch_column_law(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st, int extra,
                               bool allow_strips) {
-  const bool strips = allow_strips && use_column_strips(P, A);
+  const bool strips = allow_strips && column_strips_ok(P);
   const int nb = strips ? column_strip_blocks(P) : column_blocks(P);
   const size_t lds = column_lds(P.Mz);
   const bool wide = column_cpw(P) == COLS_PER_WAVE_WIDE;
@@ -1584,8 +1574,6 @@ static int strip_blocks(const DevParams &P, int cpw) {
 }
 int column_strip_blocks(const DevParams &P) { return strip_blocks(P, column_cpw(P)); }
 
-// the selection mode (ColumnArgs::only) keeps the staggered order
-bool use_column_strips(const DevParams &P, const ColumnArgs &A) { return A.only == nullptr && column_strips_ok(P); }
 
 int column_list_blocks(const DevParams &P) { return column_blocks(P) < 1024 ? column_blocks(P) : 1024; }
 

@@ -23,11 +23,7 @@ struct ColumnArgs {
   unsigned long long *dmax_bits;
   double *dmax_part;         // per-block maxima of D [max(column_blocks(P), column_strip_blocks(P))]
   int *err;
-  const unsigned char *only; // nullptr, or per staggered column: compute it only if only[s] == ONLY_MARK
 };
-
-// value of ColumnArgs::only that selects a staggered column (the surrogate's fallback status)
-constexpr unsigned char ONLY_MARK = 2;
 
 // levels allocated per staggered column of I (the column kernel's output on
 // full updates, read by the velocity tiles): Mz rounded up to 16, so that every
@@ -84,13 +80,12 @@ void launch_theta_thk(const DevParams &P, const double *H, const double *b, cons
                       const double *C3, const double *C4, double *theta, double *thks, int *err,
                       hipStream_t st);
 // blocks (= per-block D_max partials) of the column pass in staggered order,
-// and in XCD-interleaved strips (used where use_column_strips holds and the
+// and in XCD-interleaved strips (used where column_strips_ok holds and the
 // caller allows it); A.dmax_part must hold the larger of the two
 int column_blocks(const DevParams &P);
 int column_strip_blocks(const DevParams &P);
 // strips pay where a row of the grid holds several of them
 __host__ __device__ inline bool column_strips_ok(const DevParams &P) { return P.My >= 64; }
-bool use_column_strips(const DevParams &P, const ColumnArgs &A);
 // extra_partials: further per-block D_max partials that other kernels wrote
 // after the column pass's own; all are reduced into A.dmax_bits
 void launch_column(const DevParams &P, const ColumnArgs &A, bool full, hipStream_t st, int extra_partials = 0,
@@ -149,7 +144,6 @@ __device__ __forceinline__ ColumnArgs global_view(ColumnArgs a) {
   a.dmax_bits = glb(a.dmax_bits);
   a.dmax_part = glb(a.dmax_part);
   a.err = glb(a.err);
-  a.only = glb(a.only);
   return a;
 }
 
@@ -206,7 +200,7 @@ struct EnsLaunch {
 // goes to A.dmax_part[b - start[m]]
 struct EnsListMember {
   DevParams P;
-  ColumnArgs A;                     // A.only unused
+  ColumnArgs A;
   const int *list, *n_list;         // staggered columns, device count
 };
 void launch_ens_column_list(const EnsListMember *T, const int *start, int nm, int nblocks, int max_mz, int law,

@@ -48,7 +48,7 @@ constexpr int SG_NPARAM = SG_B5 + SG_NOUT;                // 58144
 constexpr int SG_NFRAG = 8 * 2 + 3 * 8 * 4 + 2 * 4;
 
 // column status of a surrogate update
-enum SgStatus : unsigned char { SG_ZERO = 0, SG_SURR = 1, SG_FALLBACK = ONLY_MARK };
+enum SgStatus : unsigned char { SG_ZERO = 0, SG_SURR = 1, SG_FALLBACK = 2 };
 
 // model constants used by the device kernels
 struct SgNorm {
