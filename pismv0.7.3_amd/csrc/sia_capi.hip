@@ -366,13 +366,12 @@ int update_impl(sia_handle *h, const sia_inputs *in, int fast, bool allow_sync) 
   const DevParams &P = h->dp;
   {
     Stage stage(h, SIA_EV_GRADIENT);
+    const ThetaArgs T{h->topgs, h->maxtl, h->C2, h->C3, h->C4, h->theta, h->thks, h->err};
     launch_gradient(P, in->land_ice_thickness, in->bedrock_altitude, in->surface_altitude, in->mask, h->hx,
-                    h->hy, st);
+                    h->hy, T, st);
     if (in->no_model_mask && in->usurfstore) {  // SIAFD_Regional (regional.cc:45-109)
       launch_regional_gradient(P, in->no_model_mask, in->usurfstore, h->hx, h->hy, st);
     }
-    launch_theta_thk(P, in->land_ice_thickness, in->bedrock_altitude, in->surface_altitude, in->mask,
-                     h->topgs, h->maxtl, h->C2, h->C3, h->C4, h->theta, h->thks, h->err, st);
   }
   ColumnArgs A{};
   A.z = h->d_z;

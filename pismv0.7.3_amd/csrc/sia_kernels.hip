@@ -5,12 +5,12 @@
 //
 // Kernels:
 //   k_smooth_mean / k_smooth_coeffs  BedSmoother (PISMBedSmoother.cc:162-269)
-//   k_gradient                       surface gradient, all 3 methods (SIAFD.cc:210-503)
-//   k_theta_thk                      get_theta + get_smoothed_thk (PISMBedSmoother.cc:286-420)
-//   k_column<LAW, FULL>              diffusive flux, D, D_max, and (FULL) the column
+//   k_gradient                       surface gradient, all 3 methods (SIAFD.cc:210-503),
+//                                    get_theta + get_smoothed_thk (PISMBedSmoother.cc:286-420)
+//   k_column<LAW, FULL, CPW>         diffusive flux, D, D_max, and (FULL) the column
 //                                    integral I (SIAFD.cc:546-722, 819-879); a 64-lane
-//                                    wavefront walks 16 staggered columns, one lane
-//                                    per level, wave-tree reductions and scans
+//                                    wavefront walks CPW (32 or 64) staggered columns,
+//                                    one lane per level, wave-tree reductions and scans
 //   k_velocity_tile<LAW, FASTLN>     u, v (SIAFD.cc:918-958), w (PISMStressBalance.cc:209-286)
 //                                    and (LAW >= 0) the volumetric strain heating
 //                                    (PISMStressBalance.cc:370-500) in one pass over
@@ -304,8 +304,15 @@ __device__ __forceinline__ void gradient_point(const DevParams &P, const Fields2
 
 // k_gradient: a block stages a tile of GT_I x GT_J map points and its ring of
 // one point (periodic wraps) in LDS, then each thread evaluates one point from
-// there with the same arithmetic as gradient_point
+// there with the same arithmetic as gradient_point, and theta and the smoothed
+// thickness of that point (theta_thk_point, which reads no neighbour)
 constexpr int GT_I = 8, GT_J = 32, GT_SJ = GT_J + 2, GT_N = (GT_I + 2) * GT_SJ;
+
+__device__ __forceinline__ void theta_thk_point(const DevParams &P, const Fields2D &F, const double *__restrict__ topgs,
+                                                const double *__restrict__ maxtl, const double *__restrict__ C2,
+                                                const double *__restrict__ C3, const double *__restrict__ C4,
+                                                double *__restrict__ theta, double *__restrict__ thks, int *err,
+                                                long q);
 
 struct TileFields {
   const double *sH, *sb, *sh;
@@ -319,7 +326,7 @@ struct TileFields {
 };
 
 __global__ __launch_bounds__(GT_I * GT_J) void k_gradient(DevParams P, Fields2D F, double *__restrict__ hx,
-                                                          double *__restrict__ hy) {
+                                                          double *__restrict__ hy, ThetaArgs T) {
   __shared__ double sH[GT_N], sb[GT_N], sh[GT_N];
   __shared__ int sm[GT_N];
   const int tj = (P.My + GT_J - 1) / GT_J;
@@ -336,7 +343,9 @@ __global__ __launch_bounds__(GT_I * GT_J) void k_gradient(DevParams P, Fields2D 
   __syncthreads();
   const int i = i0 + (int)threadIdx.x / GT_J, j = j0 + (int)threadIdx.x % GT_J;
   if (i >= P.Mx || j >= P.My) return;
-  gradient_at(P, TileFields{sH, sb, sh, sm, i0, j0}, hx, hy, (long)i * P.My + j, i, j);
+  const long q = (long)i * P.My + j;
+  gradient_at(P, TileFields{sH, sb, sh, sm, i0, j0}, hx, hy, q, i, j);
+  theta_thk_point(P, F, T.topgs, T.maxtl, T.C2, T.C3, T.C4, T.theta, T.thks, T.err, q);
 }
 
 // SIAFD_Regional::compute_surface_gradient's override (regional.cc:45-109): where
@@ -414,15 +423,6 @@ __device__ __forceinline__ void theta_thk_point(const DevParams &P, const Fields
     r = thk;
   }
   thks[q] = r;
-}
-
-__global__ void k_theta_thk(DevParams P, Fields2D F, const double *__restrict__ topgs,
-                            const double *__restrict__ maxtl, const double *__restrict__ C2,
-                            const double *__restrict__ C3, const double *__restrict__ C4,
-                            double *__restrict__ theta, double *__restrict__ thks, int *err) {
-  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= (long)P.Mx * P.My) return;
-  theta_thk_point(P, F, topgs, maxtl, C2, C3, C4, theta, thks, err, q);
 }
 
 // ---------------------------------------------------------------- column kernel
@@ -1595,10 +1595,10 @@ void launch_bed_smoother(const DevParams &P, double s2, double s3, double s4, co
 }
 
 void launch_gradient(const DevParams &P, const double *H, const double *b, const double *h,
-                     const double *mask, double *hx, double *hy, hipStream_t st) {
+                     const double *mask, double *hx, double *hy, const ThetaArgs &T, hipStream_t st) {
   Fields2D F{H, b, h, mask};
   const long tiles = (long)((P.Mx + GT_I - 1) / GT_I) * ((P.My + GT_J - 1) / GT_J);
-  hipLaunchKernelGGL(k_gradient, dim3((unsigned)tiles), dim3(GT_I * GT_J), 0, st, P, F, hx, hy);
+  hipLaunchKernelGGL(k_gradient, dim3((unsigned)tiles), dim3(GT_I * GT_J), 0, st, P, F, hx, hy, T);
 }
 
 void launch_regional_gradient(const DevParams &P, const double *nmm, const double *hst, double *hx, double *hy,
@@ -1607,15 +1607,6 @@ void launch_regional_gradient(const DevParams &P, const double *nmm, const doubl
   hipLaunchKernelGGL(k_regional_gradient, dim3(blocks_for(n, 256)), dim3(256), 0, st, P, nmm, hst, hx, hy);
 }
 
-void launch_theta_thk(const DevParams &P, const double *H, const double *b, const double *h,
-                      const double *mask, const double *topgs, const double *maxtl, const double *C2,
-                      const double *C3, const double *C4, double *theta, double *thks, int *err,
-                      hipStream_t st) {
-  const long n = (long)P.Mx * P.My;
-  Fields2D F{H, b, h, mask};
-  hipLaunchKernelGGL(k_theta_thk, dim3(blocks_for(n, 256)), dim3(256), 0, st, P, F, topgs, maxtl, C2, C3,
-                     C4, theta, thks, err);
-}
 
 template <int LAW>
 static void launch_vel_law(const DevParams &P, const VelArgs &A, hipStream_t st) {

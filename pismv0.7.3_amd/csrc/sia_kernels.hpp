@@ -71,14 +71,17 @@ void launch_clear_flags(int *err, int *err_acc, hipStream_t st);
 void launch_fill(double *a, long n, double v, hipStream_t st);
 void launch_bed_smoother(const DevParams &P, double s2, double s3, double s4, const double *b0, double *bs,
                          double *mt, double *c2, double *c3, double *c4, hipStream_t st);
+// theta (get_theta) and the smoothed thickness (get_smoothed_thk) of every map
+// point, computed by the gradient kernel beside the surface gradient
+struct ThetaArgs {
+  const double *topgs, *maxtl, *C2, *C3, *C4;
+  double *theta, *thks;
+  int *err;
+};
 void launch_gradient(const DevParams &P, const double *H, const double *b, const double *h,
-                     const double *mask, double *hx, double *hy, hipStream_t st);
+                     const double *mask, double *hx, double *hy, const ThetaArgs &T, hipStream_t st);
 void launch_regional_gradient(const DevParams &P, const double *nmm, const double *hst, double *hx, double *hy,
                               hipStream_t st);
-void launch_theta_thk(const DevParams &P, const double *H, const double *b, const double *h,
-                      const double *mask, const double *topgs, const double *maxtl, const double *C2,
-                      const double *C3, const double *C4, double *theta, double *thks, int *err,
-                      hipStream_t st);
 // blocks (= per-block D_max partials) of the column pass in staggered order,
 // and in XCD-interleaved strips (used where column_strips_ok holds and the
 // caller allows it); A.dmax_part must hold the larger of the two

@@ -107,12 +107,11 @@ extern "C" int sia_flux_update_noalloc(const sia_grid *grid, const sia_params *p
   } else {
     launch_bed_smoother(P, sc[0], sc[1], sc[2], in->bedrock_altitude, topgs, maxtl, C2, C3, C4, st);
   }
-  launch_gradient(P, in->land_ice_thickness, in->bedrock_altitude, in->surface_altitude, in->mask, hx, hy, st);
+  const ThetaArgs T{topgs, maxtl, C2, C3, C4, theta, thks, err};
+  launch_gradient(P, in->land_ice_thickness, in->bedrock_altitude, in->surface_altitude, in->mask, hx, hy, T, st);
   if (in->no_model_mask && in->usurfstore) {  // SIAFD_Regional (regional.cc:45-109)
     launch_regional_gradient(P, in->no_model_mask, in->usurfstore, hx, hy, st);
   }
-  launch_theta_thk(P, in->land_ice_thickness, in->bedrock_altitude, in->surface_altitude, in->mask, topgs, maxtl,
-                   C2, C3, C4, theta, thks, err, st);
   ColumnArgs A{};
   A.z = d_z;
   A.E = in->enthalpy;
