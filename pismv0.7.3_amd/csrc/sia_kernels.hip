@@ -709,8 +709,18 @@ constexpr int ENS_STRIP_CPW = 32;
 constexpr int LIST_COLS_PER_WAVE = 4;
 constexpr int LIST_COLS_PER_BLOCK = WAVES_PER_BLOCK * LIST_COLS_PER_WAVE;
 
+// wavefronts per SIMD the column kernels are compiled for (at most 512 / n
+// VGPRs each, the rest spilled). The level walk waits on its enthalpy loads
+// more than on issue, so occupancy pays for the spills. On the 1501x2801x401
+// sheet, against 4 (107 VGPRs, none spilled), fast updates: 5 waves -3 %, 6
+// -4 to -5 %, 7 -0 to -1 %, 8 +14 %; full updates: 6 -10 %, 7 -12 %, 8 -16 %.
+// The ensemble's column stage at 6: -10 %.
+constexpr int COLUMN_WAVES_FAST = 6, COLUMN_WAVES_FULL = 8;
+
 template <int LAW, bool FULL, int CPW>
-__global__ __launch_bounds__(256) void k_column(DevParams P, ColumnArgs A, int strips) {
+__global__ __launch_bounds__(256, FULL ? COLUMN_WAVES_FULL : COLUMN_WAVES_FAST) void k_column(DevParams P,
+                                                                                             ColumnArgs A,
+                                                                                             int strips) {
   extern __shared__ __align__(16) double zs[];  // column_lds(Mz) bytes
   __shared__ double wmax[WAVES_PER_BLOCK];
   const double m = column_block<LAW, FULL, CPW>(P, A, blockIdx.x, zs, wmax, nullptr, 0, strips != 0);
@@ -1716,8 +1726,8 @@ __global__ __launch_bounds__(256) void k_ens_gradient(const EnsMember *__restric
 }
 
 template <int LAW, bool FULL, int CPW>
-__global__ __launch_bounds__(256) void k_ens_column(const EnsMember *__restrict__ T, const int *__restrict__ start,
-                                                    int nm) {
+__global__ __launch_bounds__(256, COLUMN_WAVES_FAST) void k_ens_column(const EnsMember *__restrict__ T,
+                                                                       const int *__restrict__ start, int nm) {
   extern __shared__ __align__(16) double zs[];  // column_lds(Mz) bytes
   __shared__ double wmax[WAVES_PER_BLOCK];
   const int m = ens_member(start, nm, blockIdx.x);
