@@ -714,8 +714,9 @@ constexpr int LIST_COLS_PER_BLOCK = WAVES_PER_BLOCK * LIST_COLS_PER_WAVE;
 // more than on issue, so occupancy pays for the spills. On the 1501x2801x401
 // sheet, against 4 (107 VGPRs, none spilled), fast updates: 5 waves -3 %, 6
 // -4 to -5 %, 7 -0 to -1 %, 8 +14 %; full updates: 6 -10 %, 7 -12 %, 8 -16 %.
-// The ensemble's column stage at 6: -10 %.
-constexpr int COLUMN_WAVES_FAST = 6, COLUMN_WAVES_FULL = 8;
+// The 64-member ensemble's column stage (101-201 levels): 5 -6 %, 6 -10 %, 7
+// -11 %, 8 +10 %.
+constexpr int COLUMN_WAVES_FAST = 6, COLUMN_WAVES_FULL = 8, COLUMN_WAVES_ENS = 7;
 
 template <int LAW, bool FULL, int CPW>
 __global__ __launch_bounds__(256, FULL ? COLUMN_WAVES_FULL : COLUMN_WAVES_FAST) void k_column(DevParams P,
@@ -1726,7 +1727,7 @@ __global__ __launch_bounds__(256) void k_ens_gradient(const EnsMember *__restric
 }
 
 template <int LAW, bool FULL, int CPW>
-__global__ __launch_bounds__(256, COLUMN_WAVES_FAST) void k_ens_column(const EnsMember *__restrict__ T,
+__global__ __launch_bounds__(256, COLUMN_WAVES_ENS) void k_ens_column(const EnsMember *__restrict__ T,
                                                                        const int *__restrict__ start, int nm) {
   extern __shared__ __align__(16) double zs[];  // column_lds(Mz) bytes
   __shared__ double wmax[WAVES_PER_BLOCK];
